@@ -1,0 +1,142 @@
+/*
+ * pfem2gpu — MI355X (gfx950) particle hot path of a PFEM-2 solver.
+ *
+ * C ABI of the drop-in for pfem2ParticleHandler<dim> (reference:
+ * src/pfem2particlehandler.h:18-71). Each entry point names the reference
+ * method it replaces. All functions return PFEM2GPU_OK (0) or an error code;
+ * pfem2gpu_last_error() returns the message of the last failure on the
+ * calling thread.
+ *
+ * Conventions
+ *   - Vector fields are passed as an array of `dim` component pointers, each
+ *     indexed by DoF id (the reference's std::array<TrilinosWrappers::MPI::
+ *     Vector, dim>, src/pfem2fem.h:36-59).
+ *   - `where` says whether those pointers are host memory (PFEM2GPU_HOST) or
+ *     device memory (PFEM2GPU_DEVICE). Host transfers synchronise; device
+ *     calls are ordered on the handle's stream and return immediately.
+ *   - The library owns every device buffer it allocates; the caller owns all
+ *     pointers it passes. A handle is used by one host thread at a time.
+ */
+#ifndef PFEM2GPU_H
+#define PFEM2GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PFEM2GPU_OK = 0,
+  PFEM2GPU_ERR_ARG = 1,     /* invalid argument */
+  PFEM2GPU_ERR_HIP = 2,     /* HIP runtime error */
+  PFEM2GPU_ERR_STATE = 3,   /* call order violated (e.g. move before initialize_maps) */
+  PFEM2GPU_ERR_MEMORY = 4   /* device allocation failed */
+};
+
+enum { PFEM2GPU_HOST = 0, PFEM2GPU_DEVICE = 1 };
+
+typedef struct pfem2gpu_handle pfem2gpu_handle;
+
+/* Q1 mesh: what the handler reads from the triangulation and DoFHandler
+ * (src/pfem2particle.cpp:82-87 cell->vertex_dof_index, GridTools maps). */
+typedef struct {
+  int32_t dim;                  /* 2 or 3 */
+  int64_t n_vertices;
+  const double* vertices;       /* [dim][n_vertices], host memory */
+  int64_t n_cells;
+  const int32_t* cell_vertices; /* [n_cells][2^dim], GeometryInfo vertex order */
+  const int32_t* cell_dofs;     /* [n_cells][2^dim] DoF ids, or NULL = cell_vertices */
+  int64_t n_dofs;
+} pfem2gpu_mesh;
+
+/* The four particle parameters of pfem2ParameterHandler the handler reads
+ * (src/pfem2particlehandler.cpp:34-39,47). */
+typedef struct {
+  uint32_t particle_integration_steps;  /* n_sub; 0 is treated as 1 */
+  double time_step;                     /* dt of one solver step */
+  uint32_t max_particles_per_cell_part;
+  uint32_t particles_per_cell[3];       /* sub-cell parts per direction */
+} pfem2gpu_params;
+
+/* pfem2ParticleHandler(const FE_Q<dim>*) + setPfem2Solver(): builds the
+ * device mesh. src/pfem2particlehandler.cpp:10-15,29-40. `hip_stream` may be
+ * NULL (the library then creates its own stream). */
+int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, void* hip_stream,
+                    pfem2gpu_handle** out);
+
+/* ~pfem2ParticleHandler(). src/pfem2particlehandler.cpp:17-21 */
+int pfem2gpu_destroy(pfem2gpu_handle* h);
+
+const char* pfem2gpu_last_error(void);
+
+/* initialize_maps(): vertex -> cells and unit cell-centre directions.
+ * src/pfem2particlehandler.cpp:319-324 */
+int pfem2gpu_initialize_maps(pfem2gpu_handle* h);
+
+/* seed_particles(): one particle at the centre of every sub-cell part of
+ * every cell, velocity interpolated from V. src/pfem2particlehandler.cpp:42-109 */
+int pfem2gpu_seed_particles(pfem2gpu_handle* h, const double* const* V, int where);
+
+/* insert_particle() for n particles at positions x ([dim][n]) in the given
+ * cells, with velocities v ([dim][n], or NULL for zero). The reference
+ * location is found by the inverse map; particles not inside their cell are
+ * rejected and counted in *n_rejected (may be NULL).
+ * src/pfem2particlehandler.cpp:338-344 */
+int pfem2gpu_insert_particles(pfem2gpu_handle* h, int64_t n, const double* x, const int32_t* cell,
+                              const double* v, int where, int64_t* n_rejected);
+
+/* clear_particles(). src/pfem2particlehandler.cpp:326-330 */
+int pfem2gpu_clear_particles(pfem2gpu_handle* h);
+
+/* Sets every particle velocity to the interpolant of V (the gather the
+ * reference does when it seeds, src/pfem2particlehandler.cpp:94-103). */
+int pfem2gpu_set_velocity_from_field(pfem2gpu_handle* h, const double* const* V, int where);
+
+/* correct_particle_velocity(): v += interp(V - V_old).
+ * src/pfem2particlehandler.cpp:111-139. The increment is applied in the next
+ * kernel that reads particle velocities (normally fused into move). */
+int pfem2gpu_correct_particle_velocity(pfem2gpu_handle* h, const double* const* V,
+                                       const double* const* V_old, int where);
+
+/* move_particles(): n_sub x (interpolate V, explicit Euler, locate), then
+ * check_particle_distribution. src/pfem2particlehandler.cpp:141-171,355-503 */
+int pfem2gpu_move_particles(pfem2gpu_handle* h, const double* const* V, int where);
+
+/* project_particle_fields(): V_out = sum N v / sum N per DoF (0/0 -> NaN).
+ * src/pfem2particlehandler.cpp:173-215 */
+int pfem2gpu_project_particle_fields(pfem2gpu_handle* h, double* const* V_out, int where);
+
+/* Number of particles (particles.size()). Synchronises. */
+int pfem2gpu_particle_count(pfem2gpu_handle* h, int64_t* n);
+
+/* Enqueues a copy of the current particle count into *dst (device memory,
+ * int32) on the handle's stream; does not synchronise. */
+int pfem2gpu_particle_count_async(pfem2gpu_handle* h, int32_t* dst);
+
+/* Copies the particles to host arrays (any may be NULL): x, xi, v as
+ * [dim][n], cell [n]. Particles are ordered by cell. Synchronises.
+ * Replaces iteration over begin()/end(), src/pfem2particlehandler.cpp:621-631 */
+int pfem2gpu_download(pfem2gpu_handle* h, double* x, double* xi, double* v, int32_t* cell);
+
+/* Particles per cell ([n_cells], host). Synchronises. */
+int pfem2gpu_cell_counts(pfem2gpu_handle* h, int32_t* counts);
+
+/* The handle's HIP stream (hipStream_t). */
+void* pfem2gpu_stream(pfem2gpu_handle* h);
+
+int pfem2gpu_synchronize(pfem2gpu_handle* h);
+
+/* Per-kernel timing with HIP events on the handle's stream (the reference's
+ * TimerOutput sections, src/pfem2solver.cpp:14,173). enable != 0 starts
+ * recording; pfem2gpu_timings() synchronises and returns, for up to `cap`
+ * kernels, the name, launch count and total milliseconds since enabling. */
+int pfem2gpu_set_timing(pfem2gpu_handle* h, int enable);
+int pfem2gpu_timings(pfem2gpu_handle* h, int cap, const char** names, int64_t* launches, double* total_ms,
+                     int* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFEM2GPU_H */

@@ -1,0 +1,749 @@
+// TEST INFRASTRUCTURE ONLY — the CPU oracle.
+//
+// A from-scratch CPU restatement of pfem2dealii's particle hot path, used as the
+// parity checker for the HIP library and as the timed CPU baseline. Nothing in
+// the product (pfem2dealii_amd/) links, loads or calls this file; only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg do.
+//
+// Build: g++ -O2 -ffp-contract=off -shared -fPIC (oracle/Makefile). Contraction
+// must stay off: the reference evaluates every product and sum with its own
+// rounding (deal.II Tensor arithmetic, no FMA on x86-64 baseline builds).
+//
+// Two modes:
+//   ORC_MODE_CANONICAL (0): the reference's per-particle arithmetic, with the
+//     choices the reference leaves to std::vector order made deterministic:
+//     survivors per (cell, part) are the max+1 smallest canonical keys, the
+//     particle order inside a cell is (part, key), and projection sums run per
+//     cell in that order, then per node over incident cells in ascending id.
+//     This is the mode the GPU library must match bit for bit.
+//   ORC_MODE_REFERENCE (1): the reference's own loop structure: AoS
+//     std::vector, swap-with-back deletion, std::map counting, projection in
+//     vector order. Used as the CPU baseline and to bound how far the
+//     canonical choices move results.
+//
+// Reference files followed (paths relative to the reference repository):
+//   src/pfem2particlehandler.cpp:29-40    parameters, dt_sub = dt/(double)n_sub
+//   src/pfem2particlehandler.cpp:42-109   seed_particles (lattice at sub-cell centres)
+//   src/pfem2particlehandler.cpp:111-139  correct_particle_velocity
+//   src/pfem2particlehandler.cpp:141-171  move_particles (gather, Euler, locate)
+//   src/pfem2particlehandler.cpp:173-215  project_particle_fields
+//   src/pfem2particlehandler.cpp:319-324  initialize_maps (vertex->cells, centre dirs)
+//   src/pfem2particlehandler.cpp:346-353  compare_particle_association
+//   src/pfem2particlehandler.cpp:355-410  check_particle_distribution
+//   src/pfem2particlehandler.cpp:412-503  sort_particles_into_subdomains_and_cells
+//   src/pfem2particle.cpp:186-211         find_closest_vertex_of_cell
+//   src/pfem2particle.h:63-72             particle record
+// deal.II functions restated from their mathematics (not in the reference tree):
+//   FE_Q<dim>(1)::shape_value  -> Lagrange product form, factor order k = 0..dim-1
+//   MappingQ1 transform_unit_to_real_cell -> sum_i N_i(xi) X_i, sequential in i
+//   MappingQ1 transform_real_to_unit_cell -> closed form on axis-aligned boxes,
+//     Newton from an affine guess otherwise (parity vs deal.II unpinned)
+//   GeometryInfo::is_inside_unit_cell -> 0 <= xi_k <= 1 for all k, no epsilon
+//   Tensor /= s -> multiply by (1.0/s); Tensor norm -> sqrt(sum_k t_k*t_k)
+//   TriaAccessor::center -> (sum of vertices) / n_vertices
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <vector>
+
+namespace {
+
+constexpr int ORC_MODE_CANONICAL = 0;
+constexpr int ORC_MODE_REFERENCE = 1;
+
+// Newton settings for non-box Q1 cells (deal.II's own are not pinned here).
+constexpr int NEWTON_MAX_IT = 30;
+constexpr double NEWTON_TOL = 1e-12;
+
+struct Particle {
+  double x[3];
+  double xi[3];
+  int cell;
+  int dofs[8];
+  double v[3];
+  double vext[3];
+};
+
+inline uint64_t dbits(double a) {
+  uint64_t u;
+  std::memcpy(&u, &a, 8);
+  return u;
+}
+
+inline uint64_t mix64(uint64_t z) {
+  z ^= z >> 30;
+  z *= 0xbf58476d1ce4e5b9ULL;
+  z ^= z >> 27;
+  z *= 0x94d049bb133111ebULL;
+  z ^= z >> 31;
+  return z;
+}
+
+// Canonical particle key: a hash of the position bits. Used only to make the
+// survivor choice and the in-cell order deterministic.
+inline uint64_t particle_key(const double* x, int dim) {
+  uint64_t h = 0x9E3779B97F4A7C15ULL;
+  for (int k = 0; k < dim; ++k) h = mix64(h ^ dbits(x[k]));
+  return h;
+}
+
+struct Oracle {
+  int dim = 2, nvc = 4, mode = 0;
+  int64_t nv = 0, nc = 0, ndofs = 0;
+  std::vector<double> X;   // [nv][dim]
+  std::vector<int> cv;     // [nc][nvc]  vertex ids
+  std::vector<int> cd;     // [nc][nvc]  dof ids
+  std::vector<char> box;   // axis-aligned box cell flag
+  unsigned n_sub = 5;
+  double dt = 0.0, dt_sub = 0.0;
+  unsigned max_per_part = 5;
+  unsigned q[3] = {1, 1, 1};
+  double hpart[3] = {1.0, 1.0, 1.0};
+  std::vector<std::array<int, 3>> parts;  // cellPartsIndices (x index slowest)
+  bool maps_ready = false;
+  std::vector<std::vector<int>> v2c;                        // ascending cell ids
+  std::vector<std::vector<std::array<double, 3>>> v2cdir;   // unit centre directions
+  std::vector<Particle> P;
+  // statistics of the last move
+  int64_t st_walks = 0, st_deleted_locate = 0, st_deleted_overflow = 0, st_refilled = 0;
+
+  const double* vert(int v) const { return &X[(size_t)v * dim]; }
+
+  // ---- shape functions (A2) ----
+  // N_i(xi) = prod_k (bit_k(i) ? xi_k : 1 - xi_k), product in k order.
+  void shape(const double* xi, double* N) const {
+    for (int i = 0; i < nvc; ++i) {
+      double val = 1.0;
+      for (int k = 0; k < dim; ++k) {
+        const double l = ((i >> k) & 1) ? xi[k] : (1.0 - xi[k]);
+        val = val * l;
+      }
+      N[i] = val;
+    }
+  }
+  // dN_i/dxi_l
+  void shape_grad(const double* xi, double* dN /* [nvc][3] */) const {
+    for (int i = 0; i < nvc; ++i)
+      for (int l = 0; l < dim; ++l) {
+        double val = 1.0;
+        for (int k = 0; k < dim; ++k) {
+          double f;
+          if (k == l)
+            f = ((i >> k) & 1) ? 1.0 : -1.0;
+          else
+            f = ((i >> k) & 1) ? xi[k] : (1.0 - xi[k]);
+          val = val * f;
+        }
+        dN[i * 3 + l] = val;
+      }
+  }
+
+  // ---- forward map (deal.II transform_unit_to_real_cell restated) ----
+  void fwd(int cell, const double* xi, double* x) const {
+    double N[8];
+    shape(xi, N);
+    for (int k = 0; k < dim; ++k) x[k] = 0.0;
+    for (int i = 0; i < nvc; ++i) {
+      const double* Xi = vert(cv[(size_t)cell * nvc + i]);
+      for (int k = 0; k < dim; ++k) x[k] = x[k] + N[i] * Xi[k];
+    }
+  }
+
+  // Solve A d = r for dim 2/3 by Cramer's rule; A row-major [k][l]. Returns det.
+  double solve(const double A[3][3], const double* r, double* d) const {
+    if (dim == 2) {
+      const double det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+      d[0] = (r[0] * A[1][1] - A[0][1] * r[1]) / det;
+      d[1] = (A[0][0] * r[1] - r[0] * A[1][0]) / det;
+      return det;
+    }
+    const double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    const double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    const double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    const double det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+    // d_l = det(A with column l replaced by r) / det
+    const double d0n = r[0] * c00 + A[0][1] * (A[1][2] * r[2] - r[1] * A[2][2]) +
+                       A[0][2] * (r[1] * A[2][1] - A[1][1] * r[2]);
+    const double d1n = A[0][0] * (r[1] * A[2][2] - A[1][2] * r[2]) + r[0] * c01 +
+                       A[0][2] * (A[1][0] * r[2] - r[1] * A[2][0]);
+    const double d2n = A[0][0] * (A[1][1] * r[2] - r[1] * A[2][1]) +
+                       A[0][1] * (r[1] * A[2][0] - A[1][0] * r[2]) + r[0] * c02;
+    d[0] = d0n / det;
+    d[1] = d1n / det;
+    d[2] = d2n / det;
+    return det;
+  }
+
+  // ---- inverse map (A5) ----  returns false on "transformation failed"
+  bool inv(int cell, const double* x, double* xi) const {
+    const int* c = &cv[(size_t)cell * nvc];
+    const double* X0 = vert(c[0]);
+    if (box[cell]) {
+      for (int k = 0; k < dim; ++k) {
+        const double* Xk = vert(c[1 << k]);
+        xi[k] = (x[k] - X0[k]) / (Xk[k] - X0[k]);
+      }
+      return true;
+    }
+    // affine initial guess from vertex 0 and the edge vectors X_{2^l} - X_0
+    double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double r[3] = {0, 0, 0};
+    for (int l = 0; l < dim; ++l) {
+      const double* Xl = vert(c[1 << l]);
+      for (int k = 0; k < dim; ++k) A[k][l] = Xl[k] - X0[k];
+    }
+    for (int k = 0; k < dim; ++k) r[k] = x[k] - X0[k];
+    double det = solve(A, r, xi);
+    if (!(det != 0.0) || !std::isfinite(det)) return false;
+    for (int it = 0; it < NEWTON_MAX_IT; ++it) {
+      double N[8], dN[24];
+      shape(xi, N);
+      shape_grad(xi, dN);
+      double Fx[3] = {0, 0, 0};
+      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int i = 0; i < nvc; ++i) {
+        const double* Xi = vert(c[i]);
+        for (int k = 0; k < dim; ++k) {
+          Fx[k] = Fx[k] + N[i] * Xi[k];
+          for (int l = 0; l < dim; ++l) J[k][l] = J[k][l] + dN[i * 3 + l] * Xi[k];
+        }
+      }
+      double res[3] = {0, 0, 0}, d[3] = {0, 0, 0};
+      for (int k = 0; k < dim; ++k) res[k] = x[k] - Fx[k];
+      det = solve(J, res, d);
+      if (!(det != 0.0) || !std::isfinite(det)) return false;
+      double dmax = 0.0;
+      for (int k = 0; k < dim; ++k) {
+        xi[k] = xi[k] + d[k];
+        const double a = std::fabs(d[k]);
+        dmax = (a > dmax) ? a : dmax;
+      }
+      if (!(dmax == dmax)) return false;  // NaN
+      if (dmax <= NEWTON_TOL) return true;
+    }
+    return false;
+  }
+
+  bool inside(const double* xi) const {
+    for (int k = 0; k < dim; ++k)
+      if (!((xi[k] >= 0.0) && (xi[k] <= 1.0))) return false;
+    return true;
+  }
+
+  void set_dofs(Particle& p) const {
+    for (int i = 0; i < nvc; ++i) p.dofs[i] = cd[(size_t)p.cell * nvc + i];
+    for (int i = nvc; i < 8; ++i) p.dofs[i] = -1;
+  }
+
+  // gather u_c = sum_i N_i(xi) F_c[dofs_i] (A3), vertex-outer, component-inner
+  void gather(const Particle& p, const double* const* F, double* u) const {
+    double N[8];
+    shape(p.xi, N);
+    for (int k = 0; k < dim; ++k) u[k] = 0.0;
+    for (int i = 0; i < nvc; ++i)
+      for (int k = 0; k < dim; ++k) u[k] = u[k] + N[i] * F[k][p.dofs[i]];
+  }
+
+  // ---- closest vertex (A6), src/pfem2particle.cpp:186-211 ----
+  int closest_vertex(const Particle& p) const {
+    double xold[3];
+    fwd(p.cell, p.xi, xold);
+    double n2 = 0.0;
+    for (int k = 0; k < dim; ++k) n2 = n2 + p.vext[k] * p.vext[k];
+    const double inv_un = 1.0 / std::sqrt(n2);
+    double vn[3];
+    for (int k = 0; k < dim; ++k) vn[k] = p.vext[k] * inv_un;
+    double best = 0.0;
+    int bestv = 0;
+    for (int v = 0; v < nvc; ++v) {
+      const double* Xv = vert(cv[(size_t)p.cell * nvc + v]);
+      double d[3] = {0, 0, 0};
+      for (int k = 0; k < dim; ++k) d[k] = Xv[k] - xold[k];
+      double m2 = 0.0;
+      for (int k = 0; k < dim; ++k) m2 = m2 + d[k] * d[k];
+      const double inv_m = 1.0 / std::sqrt(m2);
+      for (int k = 0; k < dim; ++k) d[k] = d[k] * inv_m;
+      double a = vn[0] * d[0];
+      for (int k = 1; k < dim; ++k) a = a + vn[k] * d[k];
+      if (v == 0) {
+        best = a;
+      } else if (a > best) {
+        best = a;
+        bestv = v;
+      }
+    }
+    return bestv;
+  }
+
+  // ---- locate one particle after an Euler substep (A5-A9) ----
+  // returns false if the particle is to be deleted
+  bool locate(Particle& p) {
+    double xi[3];
+    if (inv(p.cell, p.x, xi) && inside(xi)) {
+      for (int k = 0; k < dim; ++k) p.xi[k] = xi[k];
+      return true;
+    }
+    ++st_walks;
+    const int cvl = closest_vertex(p);
+    const int cvg = cv[(size_t)p.cell * nvc + cvl];
+    const double* Xc = vert(cvg);
+    double vtp[3];
+    for (int k = 0; k < dim; ++k) vtp[k] = p.x[k] - Xc[k];
+    double m2 = 0.0;
+    for (int k = 0; k < dim; ++k) m2 = m2 + vtp[k] * vtp[k];
+    const double inv_m = 1.0 / std::sqrt(m2);
+    for (int k = 0; k < dim; ++k) vtp[k] = vtp[k] * inv_m;
+    const std::vector<int>& cand = v2c[cvg];
+    const std::vector<std::array<double, 3>>& dirs = v2cdir[cvg];
+    const int n = (int)cand.size();
+    std::vector<double> dots(n);
+    for (int i = 0; i < n; ++i) {
+      double a = dirs[i][0] * vtp[0];
+      for (int k = 1; k < dim; ++k) a = a + dirs[i][k] * vtp[k];
+      dots[i] = a;
+    }
+    // Ordering: descending dot, stable in the ascending-cell-id set order
+    // (libstdc++ std::sort on <= 16 elements is an insertion sort). Selection
+    // by repeated strict-> scan gives the same order, NaNs included.
+    std::vector<char> used(n, 0);
+    for (int r = 0; r < n; ++r) {
+      int best = -1;
+      for (int i = 0; i < n; ++i) {
+        if (used[i]) continue;
+        if (best < 0 || dots[i] > dots[best]) best = i;
+      }
+      used[best] = 1;
+      const int c = cand[best];
+      if (inv(c, p.x, xi) && inside(xi)) {
+        p.cell = c;
+        for (int k = 0; k < dim; ++k) p.xi[k] = xi[k];
+        set_dofs(p);
+        return true;
+      }
+    }
+    return false;
+  }
+
+  // part index per dimension: (int)(xi_k / h_k), h_k = 1.0/q_k (may equal q_k)
+  void part_of(const Particle& p, int* pi) const {
+    for (int k = 0; k < dim; ++k) pi[k] = (int)(p.xi[k] / hpart[k]);
+  }
+  int part_lin(const int* pi) const {
+    int lin = 0;
+    for (int k = 0; k < dim; ++k) lin = lin * (int)(q[k] + 1) + pi[k];
+    return lin;
+  }
+
+  // xi recovered for an inserted particle: the located xi, snapped to the
+  // exact sub-cell centre when x is bitwise the forward map of that centre.
+  void recover_xi(Particle& p) const {
+    int pi[3];
+    part_of(p, pi);
+    bool valid = true;
+    for (int k = 0; k < dim; ++k)
+      if (pi[k] < 0 || pi[k] >= (int)q[k]) valid = false;
+    if (!valid) return;
+    double xc[3], xic[3];
+    for (int k = 0; k < dim; ++k) xic[k] = ((double)pi[k] + 1.0 / 2) * hpart[k];
+    fwd(p.cell, xic, xc);
+    for (int k = 0; k < dim; ++k)
+      if (!(xc[k] == p.x[k])) return;
+    for (int k = 0; k < dim; ++k) p.xi[k] = xic[k];
+  }
+
+  Particle make_refill(int cell, const std::array<int, 3>& idx, const double* const* V) const {
+    Particle p;
+    std::memset(&p, 0, sizeof(p));
+    p.cell = cell;
+    for (int k = 0; k < dim; ++k) p.xi[k] = (idx[k] + 1.0 / 2) * hpart[k];
+    fwd(cell, p.xi, p.x);
+    set_dofs(p);
+    gather(p, V, p.v);
+    for (int k = 0; k < 3; ++k) p.vext[k] = 0.0;
+    return p;
+  }
+
+  struct Ord {
+    int cell, part;
+    uint64_t key;
+    const Particle* p;
+  };
+  static bool ord_less(const Ord& a, const Ord& b, int dim) {
+    if (a.cell != b.cell) return a.cell < b.cell;
+    if (a.part != b.part) return a.part < b.part;
+    if (a.key != b.key) return a.key < b.key;
+    for (int k = 0; k < dim; ++k) {
+      const uint64_t ua = dbits(a.p->x[k]), ub = dbits(b.p->x[k]);
+      if (ua != ub) return ua < ub;
+    }
+    for (int k = 0; k < dim; ++k) {
+      const uint64_t ua = dbits(a.p->v[k]), ub = dbits(b.p->v[k]);
+      if (ua != ub) return ua < ub;
+    }
+    return false;
+  }
+
+  void canonical_sort() {
+    std::vector<Ord> o(P.size());
+    for (size_t i = 0; i < P.size(); ++i) {
+      int pi[3];
+      part_of(P[i], pi);
+      o[i] = Ord{P[i].cell, part_lin(pi), particle_key(P[i].x, dim), &P[i]};
+    }
+    const int d = dim;
+    std::stable_sort(o.begin(), o.end(), [d](const Ord& a, const Ord& b) { return ord_less(a, b, d); });
+    std::vector<Particle> np;
+    np.reserve(P.size());
+    for (auto& e : o) np.push_back(*e.p);
+    P.swap(np);
+  }
+
+  // ---- check_particle_distribution (A10) ----
+  void redistribute_reference(const double* const* V) {
+    std::map<int, std::map<std::array<int, 3>, int>> counts;
+    for (size_t i = 0; i < P.size();) {
+      int pi[3];
+      part_of(P[i], pi);
+      std::array<int, 3> key = {0, 0, 0};
+      for (int k = 0; k < dim; ++k) key[k] = pi[k];
+      int& cnt = counts[P[i].cell][key];
+      if (cnt > (int)max_per_part) {
+        P[i] = P.back();
+        P.pop_back();
+        ++st_deleted_overflow;
+      } else {
+        ++cnt;
+        ++i;
+      }
+    }
+    for (auto& ci : counts)
+      for (const auto& idx : parts) {
+        std::array<int, 3> key = {0, 0, 0};
+        for (int k = 0; k < dim; ++k) key[k] = idx[k];
+        if (ci.second[key] == 0) {
+          P.push_back(make_refill(ci.first, idx, V));
+          ++st_refilled;
+        }
+      }
+  }
+
+  void redistribute_canonical(const double* const* V) {
+    canonical_sort();
+    std::vector<Particle> np;
+    np.reserve(P.size() + 16);
+    size_t i = 0;
+    while (i < P.size()) {
+      const int cell = P[i].cell;
+      size_t j = i;
+      std::map<int, int> cnt;  // part_lin -> count kept
+      while (j < P.size() && P[j].cell == cell) {
+        int pi[3];
+        part_of(P[j], pi);
+        const int lin = part_lin(pi);
+        int& c = cnt[lin];
+        if (c > (int)max_per_part) {
+          ++st_deleted_overflow;
+        } else {
+          ++c;
+          np.push_back(P[j]);
+        }
+        ++j;
+      }
+      for (const auto& idx : parts) {
+        int pi[3] = {idx[0], idx[1], idx[2]};
+        if (cnt.find(part_lin(pi)) == cnt.end()) {
+          np.push_back(make_refill(cell, idx, V));
+          ++st_refilled;
+        }
+      }
+      i = j;
+    }
+    P.swap(np);
+    canonical_sort();
+  }
+
+  void move(const double* const* V) {
+    st_walks = st_deleted_locate = st_deleted_overflow = st_refilled = 0;
+    for (unsigned s = 0; s < n_sub; ++s) {
+      for (auto& p : P) {
+        double u[3];
+        gather(p, V, u);
+        for (int k = 0; k < dim; ++k) p.vext[k] = u[k];
+        for (int k = 0; k < dim; ++k) {
+          const double t = u[k] * dt_sub;
+          p.x[k] = p.x[k] + t;
+        }
+      }
+      // sort_particles_into_subdomains_and_cells: swap-with-back deletion
+      for (size_t i = 0; i < P.size();) {
+        if (locate(P[i])) {
+          ++i;
+        } else {
+          P[i] = P.back();
+          P.pop_back();
+          ++st_deleted_locate;
+        }
+      }
+    }
+    if (mode == ORC_MODE_REFERENCE)
+      redistribute_reference(V);
+    else
+      redistribute_canonical(V);
+  }
+
+  void correct(const double* const* V, const double* const* Vold) {
+    std::vector<std::vector<double>> dV(dim, std::vector<double>(ndofs));
+    for (int k = 0; k < dim; ++k)
+      for (int64_t j = 0; j < ndofs; ++j) dV[k][j] = V[k][j] - Vold[k][j];
+    const double* dp[3] = {dV[0].data(), dV[1].data(), dim > 2 ? dV[2].data() : nullptr};
+    for (auto& p : P) {
+      double corr[3];
+      gather(p, dp, corr);
+      for (int k = 0; k < dim; ++k) p.v[k] = p.v[k] + corr[k];
+    }
+  }
+
+  void project(double* const* Vout) {
+    std::vector<std::vector<double>> S(dim, std::vector<double>(ndofs, 0.0));
+    std::vector<double> W(ndofs, 0.0);
+    if (mode == ORC_MODE_REFERENCE) {
+      for (const auto& p : P) {
+        double N[8];
+        shape(p.xi, N);
+        for (int i = 0; i < nvc; ++i) {
+          for (int k = 0; k < dim; ++k) S[k][p.dofs[i]] = S[k][p.dofs[i]] + N[i] * p.v[k];
+          W[p.dofs[i]] = W[p.dofs[i]] + N[i];
+        }
+      }
+    } else {
+      // per cell sequential sums in the stored order, then per node over
+      // incident cells in ascending cell id
+      std::vector<double> cs((size_t)nc * nvc * (dim + 1), 0.0);
+      for (const auto& p : P) {
+        double N[8];
+        shape(p.xi, N);
+        double* e = &cs[(size_t)p.cell * nvc * (dim + 1)];
+        for (int i = 0; i < nvc; ++i) {
+          for (int k = 0; k < dim; ++k) e[i * (dim + 1) + k] = e[i * (dim + 1) + k] + N[i] * p.v[k];
+          e[i * (dim + 1) + dim] = e[i * (dim + 1) + dim] + N[i];
+        }
+      }
+      for (int64_t c = 0; c < nc; ++c)
+        for (int i = 0; i < nvc; ++i) {
+          const int j = cd[(size_t)c * nvc + i];
+          const double* e = &cs[((size_t)c * nvc + i) * (dim + 1)];
+          for (int k = 0; k < dim; ++k) S[k][j] = S[k][j] + e[k];
+          W[j] = W[j] + e[dim];
+        }
+    }
+    for (int64_t j = 0; j < ndofs; ++j)
+      for (int k = 0; k < dim; ++k) Vout[k][j] = S[k][j] / W[j];
+  }
+
+  void initialize_maps() {
+    v2c.assign(nv, {});
+    for (int64_t c = 0; c < nc; ++c)
+      for (int i = 0; i < nvc; ++i) v2c[cv[(size_t)c * nvc + i]].push_back((int)c);
+    v2cdir.assign(nv, {});
+    for (int64_t v = 0; v < nv; ++v) {
+      auto& lst = v2c[v];
+      std::sort(lst.begin(), lst.end());
+      lst.erase(std::unique(lst.begin(), lst.end()), lst.end());
+      for (int c : lst) {
+        double ctr[3] = {0, 0, 0};
+        for (int i = 0; i < nvc; ++i) {
+          const double* Xi = vert(cv[(size_t)c * nvc + i]);
+          for (int k = 0; k < dim; ++k) ctr[k] = ctr[k] + Xi[k];
+        }
+        const double inv_n = 1.0 / (double)nvc;
+        std::array<double, 3> d = {0, 0, 0};
+        for (int k = 0; k < dim; ++k) d[k] = ctr[k] * inv_n - vert((int)v)[k];
+        double m2 = 0.0;
+        for (int k = 0; k < dim; ++k) m2 = m2 + d[k] * d[k];
+        const double inv_m = 1.0 / std::sqrt(m2);
+        for (int k = 0; k < dim; ++k) d[k] = d[k] * inv_m;
+        v2cdir[v].push_back(d);
+      }
+    }
+    maps_ready = true;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+void* orc_create(int dim, int64_t nv, const double* X /* [dim][nv] */, int64_t nc,
+                 const int* cell_vertices, const int* cell_dofs, int64_t ndofs,
+                 unsigned n_sub, double dt, unsigned max_per_part, const unsigned* q, int mode) {
+  if (dim != 2 && dim != 3) return nullptr;
+  Oracle* o = new Oracle();
+  o->dim = dim;
+  o->nvc = 1 << dim;
+  o->mode = mode;
+  o->nv = nv;
+  o->nc = nc;
+  o->ndofs = ndofs;
+  o->X.resize((size_t)nv * dim);
+  for (int64_t v = 0; v < nv; ++v)
+    for (int k = 0; k < dim; ++k) o->X[(size_t)v * dim + k] = X[(size_t)k * nv + v];
+  o->cv.assign(cell_vertices, cell_vertices + (size_t)nc * o->nvc);
+  o->cd.assign(cell_dofs, cell_dofs + (size_t)nc * o->nvc);
+  o->box.assign(nc, 0);
+  for (int64_t c = 0; c < nc; ++c) {
+    const double* X0 = o->vert(o->cv[(size_t)c * o->nvc]);
+    bool ok = true;
+    for (int i = 0; i < o->nvc && ok; ++i) {
+      const double* Xi = o->vert(o->cv[(size_t)c * o->nvc + i]);
+      for (int k = 0; k < dim; ++k) {
+        const double want = ((i >> k) & 1) ? o->vert(o->cv[(size_t)c * o->nvc + (1 << k)])[k] : X0[k];
+        if (!(Xi[k] == want)) ok = false;
+      }
+    }
+    o->box[c] = ok ? 1 : 0;
+  }
+  // src/pfem2particlehandler.cpp:34-39
+  o->n_sub = n_sub == 0 ? 1 : n_sub;
+  o->dt = dt;
+  o->dt_sub = dt / (double)o->n_sub;
+  o->max_per_part = max_per_part;
+  for (int k = 0; k < 3; ++k) o->q[k] = (k < dim) ? q[k] : 1;
+  for (int k = 0; k < dim; ++k) o->hpart[k] = 1.0 / o->q[k];
+  // cellPartsIndices: x index slowest (src/pfem2particlehandler.cpp:49-74)
+  for (unsigned i = 0; i < o->q[0]; ++i)
+    for (unsigned j = 0; j < (dim > 1 ? o->q[1] : 1u); ++j)
+      for (unsigned l = 0; l < (dim > 2 ? o->q[2] : 1u); ++l)
+        o->parts.push_back({(int)i, (int)j, (int)l});
+  return o;
+}
+
+void orc_destroy(void* h) { delete static_cast<Oracle*>(h); }
+
+void orc_initialize_maps(void* h) { static_cast<Oracle*>(h)->initialize_maps(); }
+
+// Insert particles at positions x ([dim][n]) in the given cells. xi is the
+// located reference position (snapped to the sub-cell centre when x is the
+// forward map of it). Returns the number of particles rejected because x is
+// not inside the given cell.
+int64_t orc_insert(void* h, int64_t n, const double* x, const int* cell, const double* v) {
+  Oracle* o = static_cast<Oracle*>(h);
+  int64_t rejected = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    Particle p;
+    std::memset(&p, 0, sizeof(p));
+    for (int k = 0; k < o->dim; ++k) p.x[k] = x[(size_t)k * n + i];
+    p.cell = cell[i];
+    if (p.cell < 0 || p.cell >= o->nc || !o->inv(p.cell, p.x, p.xi) || !o->inside(p.xi)) {
+      ++rejected;
+      continue;
+    }
+    o->recover_xi(p);
+    o->set_dofs(p);
+    for (int k = 0; k < o->dim; ++k) p.v[k] = v ? v[(size_t)k * n + i] : 0.0;
+    o->P.push_back(p);
+  }
+  if (o->mode == ORC_MODE_CANONICAL) o->canonical_sort();
+  return rejected;
+}
+
+// seed_particles (src/pfem2particlehandler.cpp:42-109): lattice at sub-cell
+// centres of every cell, velocity gathered from V ([dim][ndofs]).
+void orc_seed_lattice(void* h, const double* V) {
+  Oracle* o = static_cast<Oracle*>(h);
+  const double* Vp[3] = {V, V + o->ndofs, o->dim > 2 ? V + 2 * o->ndofs : nullptr};
+  for (int64_t c = 0; c < o->nc; ++c)
+    for (const auto& idx : o->parts) o->P.push_back(o->make_refill((int)c, idx, Vp));
+  if (o->mode == ORC_MODE_CANONICAL) o->canonical_sort();
+}
+
+void orc_set_velocity_from_field(void* h, const double* V) {
+  Oracle* o = static_cast<Oracle*>(h);
+  const double* Vp[3] = {V, V + o->ndofs, o->dim > 2 ? V + 2 * o->ndofs : nullptr};
+  for (auto& p : o->P) o->gather(p, Vp, p.v);
+}
+
+void orc_correct(void* h, const double* V, const double* Vold) {
+  Oracle* o = static_cast<Oracle*>(h);
+  const double* Vp[3] = {V, V + o->ndofs, o->dim > 2 ? V + 2 * o->ndofs : nullptr};
+  const double* Vo[3] = {Vold, Vold + o->ndofs, o->dim > 2 ? Vold + 2 * o->ndofs : nullptr};
+  o->correct(Vp, Vo);
+}
+
+int orc_move(void* h, const double* V) {
+  Oracle* o = static_cast<Oracle*>(h);
+  if (!o->maps_ready) return -1;
+  const double* Vp[3] = {V, V + o->ndofs, o->dim > 2 ? V + 2 * o->ndofs : nullptr};
+  o->move(Vp);
+  return 0;
+}
+
+void orc_project(void* h, double* Vout) {
+  Oracle* o = static_cast<Oracle*>(h);
+  double* Vp[3] = {Vout, Vout + o->ndofs, o->dim > 2 ? Vout + 2 * o->ndofs : nullptr};
+  o->project(Vp);
+}
+
+int64_t orc_count(void* h) { return (int64_t) static_cast<Oracle*>(h)->P.size(); }
+
+// Download the particles in the stored order: x, xi, v as [dim][n], cell [n].
+void orc_download(void* h, double* x, double* xi, double* v, int* cell) {
+  Oracle* o = static_cast<Oracle*>(h);
+  const size_t n = o->P.size();
+  for (size_t i = 0; i < n; ++i) {
+    const Particle& p = o->P[i];
+    for (int k = 0; k < o->dim; ++k) {
+      if (x) x[k * n + i] = p.x[k];
+      if (xi) xi[k * n + i] = p.xi[k];
+      if (v) v[k * n + i] = p.v[k];
+    }
+    if (cell) cell[i] = p.cell;
+  }
+}
+
+// [walks, deleted_in_locate, deleted_overflow, refilled] of the last move
+void orc_stats(void* h, int64_t* out) {
+  Oracle* o = static_cast<Oracle*>(h);
+  out[0] = o->st_walks;
+  out[1] = o->st_deleted_locate;
+  out[2] = o->st_deleted_overflow;
+  out[3] = o->st_refilled;
+}
+
+// Restated deal.II primitives, exposed for the closed-form tests.
+void orc_shape(void* h, const double* xi, double* N) { static_cast<Oracle*>(h)->shape(xi, N); }
+void orc_fwd(void* h, int cell, const double* xi, double* x) { static_cast<Oracle*>(h)->fwd(cell, xi, x); }
+int orc_inv(void* h, int cell, const double* x, double* xi) {
+  Oracle* o = static_cast<Oracle*>(h);
+  return o->inv(cell, x, xi) ? (o->inside(xi) ? 1 : 0) : -1;
+}
+int orc_is_box(void* h, int cell) { return static_cast<Oracle*>(h)->box[cell]; }
+// closest vertex for a particle at reference position xi_old in cell with velocity u
+int orc_closest_vertex(void* h, int cell, const double* xi_old, const double* u) {
+  Oracle* o = static_cast<Oracle*>(h);
+  Particle p;
+  std::memset(&p, 0, sizeof(p));
+  p.cell = cell;
+  for (int k = 0; k < o->dim; ++k) {
+    p.xi[k] = xi_old[k];
+    p.vext[k] = u[k];
+  }
+  return o->closest_vertex(p);
+}
+// vertex->cells map entry count and content (after initialize_maps)
+int orc_v2c(void* h, int v, int* cells, double* dirs) {
+  Oracle* o = static_cast<Oracle*>(h);
+  const auto& l = o->v2c[v];
+  for (size_t i = 0; i < l.size(); ++i) {
+    if (cells) cells[i] = l[i];
+    if (dirs)
+      for (int k = 0; k < o->dim; ++k) dirs[i * o->dim + k] = o->v2cdir[v][i][k];
+  }
+  return (int)l.size();
+}
+uint64_t orc_key(const double* x, int dim) { return particle_key(x, dim); }
+
+}  // extern "C"

@@ -1,0 +1,190 @@
+"""ctypes wrapper of the C++ oracle (oracle/pfem2_oracle.cpp). Test infrastructure.
+
+Arrays cross the boundary as numpy float64/int32, vector fields as [dim][n].
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+MODE_CANONICAL = 0
+MODE_REFERENCE = 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+_u = ctypes.POINTER(ctypes.c_uint)
+_l = ctypes.POINTER(ctypes.c_int64)
+
+
+def build_oracle(force=False):
+    """Compile liborc.so with oracle/Makefile (g++ -O2 -ffp-contract=off)."""
+    if force or not os.path.exists(_LIB):
+        subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE, "liborc.so"], check=True)
+    return _LIB
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build_oracle()
+        lib = ctypes.CDLL(_LIB)
+        lib.orc_create.restype = ctypes.c_void_p
+        lib.orc_create.argtypes = [ctypes.c_int, ctypes.c_int64, _d, ctypes.c_int64, _i, _i, ctypes.c_int64,
+                                   ctypes.c_uint, ctypes.c_double, ctypes.c_uint, _u, ctypes.c_int]
+        lib.orc_destroy.argtypes = [ctypes.c_void_p]
+        lib.orc_initialize_maps.argtypes = [ctypes.c_void_p]
+        lib.orc_insert.restype = ctypes.c_int64
+        lib.orc_insert.argtypes = [ctypes.c_void_p, ctypes.c_int64, _d, _i, _d]
+        lib.orc_seed_lattice.argtypes = [ctypes.c_void_p, _d]
+        lib.orc_set_velocity_from_field.argtypes = [ctypes.c_void_p, _d]
+        lib.orc_correct.argtypes = [ctypes.c_void_p, _d, _d]
+        lib.orc_move.restype = ctypes.c_int
+        lib.orc_move.argtypes = [ctypes.c_void_p, _d]
+        lib.orc_project.argtypes = [ctypes.c_void_p, _d]
+        lib.orc_count.restype = ctypes.c_int64
+        lib.orc_count.argtypes = [ctypes.c_void_p]
+        lib.orc_download.argtypes = [ctypes.c_void_p, _d, _d, _d, _i]
+        lib.orc_stats.argtypes = [ctypes.c_void_p, _l]
+        lib.orc_shape.argtypes = [ctypes.c_void_p, _d, _d]
+        lib.orc_fwd.argtypes = [ctypes.c_void_p, ctypes.c_int, _d, _d]
+        lib.orc_inv.restype = ctypes.c_int
+        lib.orc_inv.argtypes = [ctypes.c_void_p, ctypes.c_int, _d, _d]
+        lib.orc_is_box.restype = ctypes.c_int
+        lib.orc_is_box.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.orc_closest_vertex.restype = ctypes.c_int
+        lib.orc_closest_vertex.argtypes = [ctypes.c_void_p, ctypes.c_int, _d, _d]
+        lib.orc_v2c.restype = ctypes.c_int
+        lib.orc_v2c.argtypes = [ctypes.c_void_p, ctypes.c_int, _i, _d]
+        lib.orc_key.restype = ctypes.c_uint64
+        lib.orc_key.argtypes = [_d, ctypes.c_int]
+        _lib = lib
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_d) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(_i) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Oracle:
+    """CPU restatement of pfem2ParticleHandler<dim> (src/pfem2particlehandler.h:18-71)."""
+
+    def __init__(self, mesh, params, mode=MODE_CANONICAL):
+        lib = _load()
+        self.lib = lib
+        self.dim = int(mesh["dim"])
+        self.ndofs = int(mesh["n_dofs"])
+        self._X = _f64(mesh["X"])
+        self._cv = np.ascontiguousarray(mesh["cell_vertices"], dtype=np.int32)
+        self._cd = np.ascontiguousarray(mesh.get("cell_dofs", mesh["cell_vertices"]), dtype=np.int32)
+        q = np.array(list(params["particles_per_cell"]) + [1] * (3 - len(params["particles_per_cell"])),
+                     dtype=np.uint32)
+        self.h = lib.orc_create(self.dim, self._X.shape[1], _dp(self._X), self._cv.shape[0], _ip(self._cv),
+                                _ip(self._cd), self.ndofs, int(params["n_sub"]), float(params["dt"]),
+                                int(params["max_per_part"]), q.ctypes.data_as(_u), int(mode))
+        if not self.h:
+            raise ValueError("oracle: unsupported mesh")
+
+    def close(self):
+        if self.h:
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def initialize_maps(self):
+        self.lib.orc_initialize_maps(self.h)
+
+    def insert(self, x, cell, v=None):
+        x = _f64(x)
+        cell = np.ascontiguousarray(cell, dtype=np.int32)
+        v = _f64(v) if v is not None else None
+        return self.lib.orc_insert(self.h, cell.shape[0], _dp(x), _ip(cell), _dp(v))
+
+    def seed_lattice(self, V):
+        self.lib.orc_seed_lattice(self.h, _dp(_f64(V)))
+
+    def set_velocity_from_field(self, V):
+        self.lib.orc_set_velocity_from_field(self.h, _dp(_f64(V)))
+
+    def correct(self, V, Vold):
+        self.lib.orc_correct(self.h, _dp(_f64(V)), _dp(_f64(Vold)))
+
+    def move(self, V):
+        if self.lib.orc_move(self.h, _dp(_f64(V))) != 0:
+            raise RuntimeError("oracle: initialize_maps() not called")
+
+    def project(self):
+        out = np.empty((self.dim, self.ndofs), dtype=np.float64)
+        self.lib.orc_project(self.h, _dp(out))
+        return out
+
+    def count(self):
+        return int(self.lib.orc_count(self.h))
+
+    def download(self):
+        n = self.count()
+        x = np.empty((self.dim, n))
+        xi = np.empty((self.dim, n))
+        v = np.empty((self.dim, n))
+        cell = np.empty(n, dtype=np.int32)
+        self.lib.orc_download(self.h, _dp(x), _dp(xi), _dp(v), _ip(cell))
+        return {"x": x, "xi": xi, "v": v, "cell": cell}
+
+    def stats(self):
+        out = np.zeros(4, dtype=np.int64)
+        self.lib.orc_stats(self.h, out.ctypes.data_as(_l))
+        return dict(zip(["walks", "deleted_locate", "deleted_overflow", "refilled"], out.tolist()))
+
+    # restated deal.II primitives
+    def shape(self, xi):
+        xi = _f64(xi)
+        N = np.empty(1 << self.dim)
+        self.lib.orc_shape(self.h, _dp(xi), _dp(N))
+        return N
+
+    def fwd(self, cell, xi):
+        x = np.empty(self.dim)
+        self.lib.orc_fwd(self.h, int(cell), _dp(_f64(xi)), _dp(x))
+        return x
+
+    def inv(self, cell, x):
+        """(status, xi): status 1 inside, 0 outside, -1 transformation failed."""
+        xi = np.zeros(self.dim)
+        st = self.lib.orc_inv(self.h, int(cell), _dp(_f64(x)), _dp(xi))
+        return st, xi
+
+    def is_box(self, cell):
+        return bool(self.lib.orc_is_box(self.h, int(cell)))
+
+    def closest_vertex(self, cell, xi_old, u):
+        return self.lib.orc_closest_vertex(self.h, int(cell), _dp(_f64(xi_old)), _dp(_f64(u)))
+
+    def v2c(self, v):
+        n = self.lib.orc_v2c(self.h, int(v), None, None)
+        cells = np.empty(n, dtype=np.int32)
+        dirs = np.empty((n, self.dim))
+        self.lib.orc_v2c(self.h, int(v), _ip(cells), _dp(dirs))
+        return cells, dirs
+
+
+def key(x):
+    x = _f64(x)
+    return int(_load().orc_key(_dp(x), x.shape[0]))

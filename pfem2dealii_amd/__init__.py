@@ -1,0 +1,272 @@
+"""pfem2dealii_amd — MI355X particle hot path of a PFEM-2 solver.
+
+Python mirror of the reference's pfem2ParticleHandler<dim> interface
+(src/pfem2particlehandler.h:18-71) over the C ABI in include/pfem2gpu.h.
+Every method calls the HIP library libpfem2gpu.so; there is no CPU fallback:
+if the library is missing, importing the handler raises.
+
+Vector fields are [dim][n_dofs] float64 arrays: numpy arrays are host memory,
+torch CUDA tensors are device memory (used in place on the handle's stream).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpfem2gpu.so")
+
+HOST = 0
+DEVICE = 1
+
+_lib = None
+
+
+class Pfem2GpuError(RuntimeError):
+    pass
+
+
+_vp = ctypes.c_void_p
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class _Mesh(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("n_vertices", ctypes.c_int64), ("vertices", _vp),
+                ("n_cells", ctypes.c_int64), ("cell_vertices", _vp), ("cell_dofs", _vp),
+                ("n_dofs", ctypes.c_int64)]
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("particle_integration_steps", ctypes.c_uint32), ("time_step", ctypes.c_double),
+                ("max_particles_per_cell_part", ctypes.c_uint32), ("particles_per_cell", ctypes.c_uint32 * 3)]
+
+
+EXPORTS = [
+    "pfem2gpu_create", "pfem2gpu_destroy", "pfem2gpu_last_error", "pfem2gpu_initialize_maps",
+    "pfem2gpu_seed_particles", "pfem2gpu_insert_particles", "pfem2gpu_clear_particles",
+    "pfem2gpu_set_velocity_from_field", "pfem2gpu_correct_particle_velocity", "pfem2gpu_move_particles",
+    "pfem2gpu_project_particle_fields", "pfem2gpu_particle_count", "pfem2gpu_particle_count_async",
+    "pfem2gpu_download", "pfem2gpu_cell_counts",
+    "pfem2gpu_stream", "pfem2gpu_synchronize", "pfem2gpu_set_timing", "pfem2gpu_timings",
+]
+
+
+def load_library(path=None):
+    """Loads libpfem2gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise Pfem2GpuError("libpfem2gpu.so not built (run python -m pfem2dealii_amd.build): " + p)
+    lib = ctypes.CDLL(p)
+    lib.pfem2gpu_last_error.restype = ctypes.c_char_p
+    lib.pfem2gpu_create.argtypes = [ctypes.POINTER(_Mesh), ctypes.POINTER(_Params), _vp, ctypes.POINTER(_vp)]
+    lib.pfem2gpu_destroy.argtypes = [_vp]
+    lib.pfem2gpu_initialize_maps.argtypes = [_vp]
+    lib.pfem2gpu_seed_particles.argtypes = [_vp, _vp, ctypes.c_int]
+    lib.pfem2gpu_insert_particles.argtypes = [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int, _i64p]
+    lib.pfem2gpu_clear_particles.argtypes = [_vp]
+    lib.pfem2gpu_set_velocity_from_field.argtypes = [_vp, _vp, ctypes.c_int]
+    lib.pfem2gpu_correct_particle_velocity.argtypes = [_vp, _vp, _vp, ctypes.c_int]
+    lib.pfem2gpu_move_particles.argtypes = [_vp, _vp, ctypes.c_int]
+    lib.pfem2gpu_project_particle_fields.argtypes = [_vp, _vp, ctypes.c_int]
+    lib.pfem2gpu_particle_count.argtypes = [_vp, _i64p]
+    lib.pfem2gpu_particle_count_async.argtypes = [_vp, _vp]
+    lib.pfem2gpu_download.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    lib.pfem2gpu_cell_counts.argtypes = [_vp, _vp]
+    lib.pfem2gpu_stream.restype = _vp
+    lib.pfem2gpu_stream.argtypes = [_vp]
+    lib.pfem2gpu_synchronize.argtypes = [_vp]
+    lib.pfem2gpu_set_timing.argtypes = [_vp, ctypes.c_int]
+    lib.pfem2gpu_timings.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _i64p,
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(lib, rc, what):
+    if rc != 0:
+        msg = lib.pfem2gpu_last_error().decode(errors="replace")
+        raise Pfem2GpuError("%s failed (code %d): %s" % (what, rc, msg))
+
+
+def _is_device(a):
+    return hasattr(a, "is_cuda") and a.is_cuda
+
+
+class _FieldPtrs:
+    """Array of `dim` component pointers for a [dim][n] field, plus keep-alive."""
+
+    def __init__(self, F, dim, n, writable=False):
+        if _is_device(F):
+            import torch
+            if F.dtype != torch.float64 or tuple(F.shape) != (dim, n):
+                raise ValueError("device field must be float64 [%d, %d]" % (dim, n))
+            if not F.is_contiguous():
+                raise ValueError("device field must be contiguous")
+            self.keep = F
+            base = F.data_ptr()
+            self.where = DEVICE
+        else:
+            if writable:
+                if not (isinstance(F, np.ndarray) and F.dtype == np.float64 and F.flags.c_contiguous
+                        and F.shape == (dim, n)):
+                    raise ValueError("host output must be a C-contiguous float64 [%d, %d] array" % (dim, n))
+                A = F
+            else:
+                A = np.ascontiguousarray(F, dtype=np.float64)
+                if A.shape != (dim, n):
+                    raise ValueError("field must be [%d, %d]" % (dim, n))
+            self.keep = A
+            base = A.ctypes.data
+            self.where = HOST
+        self.arr = (_vp * 3)(*[base + 8 * n * k for k in range(dim)] + [None] * (3 - dim))
+
+
+class ParticleHandler:
+    """pfem2ParticleHandler<dim> on one MI355X (src/pfem2particlehandler.h:18-71)."""
+
+    def __init__(self, mesh, params, stream=None):
+        self.lib = load_library()
+        self.dim = int(mesh["dim"])
+        self.n_dofs = int(mesh["n_dofs"])
+        self.n_cells = int(np.asarray(mesh["cell_vertices"]).shape[0])
+        X = np.ascontiguousarray(mesh["X"], dtype=np.float64)
+        cv = np.ascontiguousarray(mesh["cell_vertices"], dtype=np.int32)
+        cd = np.ascontiguousarray(mesh.get("cell_dofs", cv), dtype=np.int32)
+        m = _Mesh(self.dim, X.shape[1], X.ctypes.data, cv.shape[0], cv.ctypes.data, cd.ctypes.data, self.n_dofs)
+        q = list(params["particles_per_cell"]) + [1] * (3 - len(params["particles_per_cell"]))
+        p = _Params(int(params["n_sub"]), float(params["dt"]), int(params["max_per_part"]),
+                    (ctypes.c_uint32 * 3)(*q))
+        h = _vp()
+        _check(self.lib, self.lib.pfem2gpu_create(ctypes.byref(m), ctypes.byref(p), stream, ctypes.byref(h)),
+               "pfem2gpu_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pfem2gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- the reference interface ----
+    def initialize_maps(self):
+        _check(self.lib, self.lib.pfem2gpu_initialize_maps(self.h), "initialize_maps")
+
+    def seed_particles(self, V):
+        f = _FieldPtrs(V, self.dim, self.n_dofs)
+        _check(self.lib, self.lib.pfem2gpu_seed_particles(self.h, f.arr, f.where), "seed_particles")
+
+    def insert_particles(self, x, cell, v=None):
+        """Inserts particles at x ([dim][n]) in `cell`; returns the number rejected."""
+        if _is_device(x):
+            where = DEVICE
+            n = int(cell.shape[0])
+            keep = (x, cell, v)
+            xp, cp = x.data_ptr(), cell.data_ptr()
+            vp = v.data_ptr() if v is not None else None
+        else:
+            where = HOST
+            xa = np.ascontiguousarray(x, dtype=np.float64)
+            ca = np.ascontiguousarray(cell, dtype=np.int32)
+            va = np.ascontiguousarray(v, dtype=np.float64) if v is not None else None
+            n = int(ca.shape[0])
+            if xa.shape != (self.dim, n) or (va is not None and va.shape != (self.dim, n)):
+                raise ValueError("x and v must be [dim][n]")
+            keep = (xa, ca, va)
+            xp, cp = xa.ctypes.data, ca.ctypes.data
+            vp = va.ctypes.data if va is not None else None
+        rej = ctypes.c_int64(0)
+        _check(self.lib, self.lib.pfem2gpu_insert_particles(self.h, n, xp, cp, vp, where, ctypes.byref(rej)),
+               "insert_particles")
+        del keep
+        return int(rej.value)
+
+    def clear_particles(self):
+        _check(self.lib, self.lib.pfem2gpu_clear_particles(self.h), "clear_particles")
+
+    def set_velocity_from_field(self, V):
+        f = _FieldPtrs(V, self.dim, self.n_dofs)
+        _check(self.lib, self.lib.pfem2gpu_set_velocity_from_field(self.h, f.arr, f.where),
+               "set_velocity_from_field")
+
+    def correct_particle_velocity(self, V, V_old):
+        a = _FieldPtrs(V, self.dim, self.n_dofs)
+        b = _FieldPtrs(V_old, self.dim, self.n_dofs)
+        if a.where != b.where:
+            raise ValueError("V and V_old must both be host or both be device arrays")
+        _check(self.lib, self.lib.pfem2gpu_correct_particle_velocity(self.h, a.arr, b.arr, a.where),
+               "correct_particle_velocity")
+
+    def move_particles(self, V):
+        f = _FieldPtrs(V, self.dim, self.n_dofs)
+        _check(self.lib, self.lib.pfem2gpu_move_particles(self.h, f.arr, f.where), "move_particles")
+
+    def project_particle_fields(self, out=None):
+        """Projected nodal velocity; into `out` (numpy or torch CUDA) if given."""
+        if out is None:
+            out = np.empty((self.dim, self.n_dofs), dtype=np.float64)
+        f = _FieldPtrs(out, self.dim, self.n_dofs, writable=True)
+        _check(self.lib, self.lib.pfem2gpu_project_particle_fields(self.h, f.arr, f.where),
+               "project_particle_fields")
+        return out
+
+    # ---- readback ----
+    def count(self):
+        n = ctypes.c_int64(0)
+        _check(self.lib, self.lib.pfem2gpu_particle_count(self.h, ctypes.byref(n)), "particle_count")
+        return int(n.value)
+
+    def count_async(self, dst_ptr):
+        """Enqueues a copy of the particle count into device int32 at dst_ptr."""
+        _check(self.lib, self.lib.pfem2gpu_particle_count_async(self.h, dst_ptr), "particle_count_async")
+
+    def download(self, xi=True):
+        n = self.count()
+        x = np.empty((self.dim, n))
+        v = np.empty((self.dim, n))
+        r = np.empty((self.dim, n)) if xi else None
+        cell = np.empty(n, dtype=np.int32)
+        _check(self.lib, self.lib.pfem2gpu_download(self.h, x.ctypes.data, r.ctypes.data if xi else None,
+                                                    v.ctypes.data, cell.ctypes.data), "download")
+        out = {"x": x, "v": v, "cell": cell}
+        if xi:
+            out["xi"] = r
+        return out
+
+    def cell_counts(self):
+        c = np.empty(self.n_cells, dtype=np.int32)
+        _check(self.lib, self.lib.pfem2gpu_cell_counts(self.h, c.ctypes.data), "cell_counts")
+        return c
+
+    def stream(self):
+        return self.lib.pfem2gpu_stream(self.h)
+
+    def synchronize(self):
+        _check(self.lib, self.lib.pfem2gpu_synchronize(self.h), "synchronize")
+
+    def set_timing(self, enable):
+        _check(self.lib, self.lib.pfem2gpu_set_timing(self.h, 1 if enable else 0), "set_timing")
+
+    def timings(self):
+        """{kernel name: (launches, total ms)} since set_timing(True)."""
+        cap = 64
+        names = (ctypes.c_char_p * cap)()
+        launches = (ctypes.c_int64 * cap)()
+        ms = (ctypes.c_double * cap)()
+        n = ctypes.c_int(0)
+        _check(self.lib, self.lib.pfem2gpu_timings(self.h, cap, names, launches, ms, ctypes.byref(n)), "timings")
+        return {names[i].decode(): (int(launches[i]), float(ms[i])) for i in range(min(n.value, cap))}

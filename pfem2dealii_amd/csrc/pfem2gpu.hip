@@ -1,0 +1,844 @@
+// Host side of the particle path: device buffers, step orchestration and the
+// C ABI declared in include/pfem2gpu.h.
+//
+// Layout in HBM (per handle):
+//   particles   SoA FP64 x[dim], v[dim] + int32 cell, sorted by cell; the
+//               order inside a cell is (part bucket, canonical key);
+//               cell_start[n_cells+1] delimits the cell segments and its last
+//               entry is the particle count.
+//   staging T   the same SoA arrays plus part (u16) and key (u64): the move
+//               passes scatter into it, the settle kernel compacts back.
+//   mesh        vertex coordinates SoA, cell->vertex and cell->dof tables
+//               (16/32 B per cell, one vector load), box flags, the
+//               vertex->cells CSR with unit centre directions, and the
+//               dof->(cell, vertex) CSR used by the node quotient.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pfem2gpu.h"
+#include "pfem2gpu_kernels.h"
+
+using namespace pfem2gpu;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return fail(PFEM2GPU_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));         \
+  } while (0)
+
+struct Timed {
+  const char* name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct pfem2gpu_handle {
+  int dim = 2, nvc = 4;
+  int nc = 0, nv = 0, ndofs = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  Prm prm_host{};
+  int refill_max = 0;  // upper bound of particles added by one redistribution
+
+  // mesh
+  double* X[3] = {nullptr, nullptr, nullptr};
+  int* cv = nullptr;
+  int* cd = nullptr;
+  unsigned char* box = nullptr;
+  int* v2c_off = nullptr;
+  int* v2c = nullptr;
+  double* v2c_dir = nullptr;
+  int* d2c_off = nullptr;
+  int* d2c = nullptr;
+  Prm* prm = nullptr;
+  bool maps_ready = false;
+  std::vector<double> hX;  // host copies used by initialize_maps
+  std::vector<int> hcv;
+
+  // particles
+  long cap = 0;
+  double* px[3] = {nullptr, nullptr, nullptr};
+  double* pv[3] = {nullptr, nullptr, nullptr};
+  int* pcell = nullptr;
+  double* tx[3] = {nullptr, nullptr, nullptr};
+  double* tv[3] = {nullptr, nullptr, nullptr};
+  int* tcell = nullptr;
+  unsigned short* tpart = nullptr;
+  unsigned long long* tkey = nullptr;
+  int* cell_start = nullptr;  // [nc+1]
+  int* fstart = nullptr;      // [nc+1] next layout (swapped with cell_start)
+  int* count = nullptr;       // [2*(nc+1)]: count | fill
+  int* tstart = nullptr;      // [nc+1]
+  int* fcount = nullptr;      // [nc+1]
+  int* scan_blk = nullptr;
+  int scan_nb = 0;
+  double* contrib = nullptr;  // [nc][nvc][dim+1]
+  double* dV[3] = {nullptr, nullptr, nullptr};
+  bool pending_corr = false;
+  double* stageF[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  unsigned long long* d_rejected = nullptr;
+  int* h_pinned = nullptr;
+  long n_ub = 0;  // host-side upper bound of the particle count
+  int cpw = 64, pmax = 1, settle_smem = 0;
+
+  // timing
+  bool timing = false;
+  std::vector<Timed> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<long, double>> totals;
+  std::vector<std::string> names_store;
+};
+
+namespace {
+
+int grid_for(long n) {
+  long g = (n + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > 16384) g = 16384;
+  return (int)g;
+}
+
+hipEvent_t take_event(pfem2gpu_handle* h) {
+  if (!h->pool.empty()) {
+    hipEvent_t e = h->pool.back();
+    h->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void t_begin(pfem2gpu_handle* h, const char* name) {
+  if (!h->timing) return;
+  Timed t{name, take_event(h), take_event(h)};
+  if (t.a && t.b) {
+    (void)hipEventRecord(t.a, h->stream);
+    h->pending.push_back(t);
+  }
+}
+
+void t_end(pfem2gpu_handle* h) {
+  if (!h->timing || h->pending.empty()) return;
+  (void)hipEventRecord(h->pending.back().b, h->stream);
+}
+
+int t_collect(pfem2gpu_handle* h) {
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (auto& t : h->pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      auto& e = h->totals[t.name];
+      e.first += 1;
+      e.second += ms;
+    }
+    h->pool.push_back(t.a);
+    h->pool.push_back(t.b);
+  }
+  h->pending.clear();
+  return PFEM2GPU_OK;
+}
+
+#define LAUNCH(h, name, kernel, grid, smem, ...)                                  \
+  do {                                                                            \
+    t_begin(h, name);                                                             \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), smem, h->stream, __VA_ARGS__); \
+    t_end(h);                                                                     \
+    HIPCHK(hipGetLastError());                                                    \
+  } while (0)
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  if (hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)) != hipSuccess)
+    return fail(PFEM2GPU_ERR_MEMORY, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+  return PFEM2GPU_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+#define CHK(expr)                      \
+  do {                                 \
+    int rc_ = (expr);                  \
+    if (rc_ != PFEM2GPU_OK) return rc_; \
+  } while (0)
+
+DMesh dmesh(pfem2gpu_handle* h) {
+  DMesh m;
+  for (int k = 0; k < 3; ++k) m.X[k] = h->X[k];
+  m.cv = h->cv;
+  m.cd = h->cd;
+  m.box = h->box;
+  m.v2c_off = h->v2c_off;
+  m.v2c = h->v2c;
+  m.v2c_dir = h->v2c_dir;
+  m.cell_inst = nullptr;
+  m.prm = h->prm;
+  return m;
+}
+
+DParts dparts(pfem2gpu_handle* h) {
+  DParts p;
+  for (int k = 0; k < 3; ++k) {
+    p.x[k] = h->px[k];
+    p.v[k] = h->pv[k];
+  }
+  p.cell = h->pcell;
+  return p;
+}
+
+DTmp dtmp(pfem2gpu_handle* h) {
+  DTmp t;
+  for (int k = 0; k < 3; ++k) {
+    t.x[k] = h->tx[k];
+    t.v[k] = h->tv[k];
+  }
+  t.cell = h->tcell;
+  t.part = h->tpart;
+  t.key = h->tkey;
+  return t;
+}
+
+void free_particles(pfem2gpu_handle* h) {
+  for (int k = 0; k < 3; ++k) {
+    dfree(h->px[k]);
+    dfree(h->pv[k]);
+    dfree(h->tx[k]);
+    dfree(h->tv[k]);
+  }
+  dfree(h->pcell);
+  dfree(h->tcell);
+  dfree(h->tpart);
+  dfree(h->tkey);
+  h->cap = 0;
+}
+
+// Grows the particle arrays to hold at least `need` particles, keeping the
+// first `keep` current ones.
+int grow(pfem2gpu_handle* h, long need, long keep) {
+  if (need <= h->cap) return PFEM2GPU_OK;
+  long ncap = std::max(need, h->cap + h->cap / 2);
+  ncap = std::max(ncap, 1024L);
+  if (ncap > 0x7fffffffL) return fail(PFEM2GPU_ERR_MEMORY, "particle capacity exceeds int32 indexing");
+  double *nx[3] = {nullptr, nullptr, nullptr}, *nvv[3] = {nullptr, nullptr, nullptr};
+  int* ncell = nullptr;
+  for (int k = 0; k < h->dim; ++k) {
+    CHK(dalloc(&nx[k], ncap));
+    CHK(dalloc(&nvv[k], ncap));
+  }
+  CHK(dalloc(&ncell, ncap));
+  if (keep > 0) {
+    for (int k = 0; k < h->dim; ++k) {
+      HIPCHK(hipMemcpyAsync(nx[k], h->px[k], keep * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      HIPCHK(hipMemcpyAsync(nvv[k], h->pv[k], keep * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    }
+    HIPCHK(hipMemcpyAsync(ncell, h->pcell, keep * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  free_particles(h);
+  for (int k = 0; k < h->dim; ++k) {
+    h->px[k] = nx[k];
+    h->pv[k] = nvv[k];
+    CHK(dalloc(&h->tx[k], ncap));
+    CHK(dalloc(&h->tv[k], ncap));
+  }
+  h->pcell = ncell;
+  CHK(dalloc(&h->tcell, ncap));
+  CHK(dalloc(&h->tpart, ncap));
+  CHK(dalloc(&h->tkey, ncap));
+  h->cap = ncap;
+  return PFEM2GPU_OK;
+}
+
+int read_count(pfem2gpu_handle* h, long* n) {
+  HIPCHK(hipMemcpyAsync(h->h_pinned, h->cell_start + h->nc, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  *n = *h->h_pinned;
+  h->n_ub = *n;
+  return PFEM2GPU_OK;
+}
+
+// Makes room for `extra` particles beyond the current count.
+int reserve(pfem2gpu_handle* h, long extra) {
+  if (h->n_ub + extra <= h->cap) return PFEM2GPU_OK;
+  long n = 0;
+  CHK(read_count(h, &n));
+  if (n + extra <= h->cap) return PFEM2GPU_OK;
+  return grow(h, 2 * (n + extra), n);
+}
+
+int scan(pfem2gpu_handle* h, const int* in, int n, int* out) {
+  const int nb = (n + kScanChunk - 1) / kScanChunk;
+  LAUNCH(h, "scan", k_scan_reduce, nb, 0, in, n, h->scan_blk);
+  LAUNCH(h, "scan", k_scan_blocks, 1, 0, h->scan_blk, nb, out + n);
+  LAUNCH(h, "scan", k_scan_down, nb, 0, in, n, h->scan_blk, out);
+  return PFEM2GPU_OK;
+}
+
+// Field pointers for a call: device pointers pass through, host arrays are
+// staged into slot `slot` (synchronous copy).
+int field_in(pfem2gpu_handle* h, const double* const* F, int where, int slot, DField* out) {
+  if (!F) return fail(PFEM2GPU_ERR_ARG, "null field");
+  for (int k = 0; k < 3; ++k) out->c[k] = nullptr;
+  for (int k = 0; k < h->dim; ++k) {
+    if (!F[k]) return fail(PFEM2GPU_ERR_ARG, "null field component");
+    if (where == PFEM2GPU_DEVICE) {
+      out->c[k] = F[k];
+    } else if (where == PFEM2GPU_HOST) {
+      HIPCHK(hipMemcpyAsync(h->stageF[slot][k], F[k], h->ndofs * sizeof(double), hipMemcpyHostToDevice, h->stream));
+      out->c[k] = h->stageF[slot][k];
+    } else {
+      return fail(PFEM2GPU_ERR_ARG, "where must be PFEM2GPU_HOST or PFEM2GPU_DEVICE");
+    }
+  }
+  if (where == PFEM2GPU_HOST) HIPCHK(hipStreamSynchronize(h->stream));
+  return PFEM2GPU_OK;
+}
+
+DField dvfield(pfem2gpu_handle* h) {
+  DField f;
+  for (int k = 0; k < 3; ++k) f.c[k] = h->dV[k];
+  return f;
+}
+
+template <int D>
+int flush_correction(pfem2gpu_handle* h) {
+  if (!h->pending_corr) return PFEM2GPU_OK;
+  LAUNCH(h, "k_correct", k_correct<D>, grid_for(h->n_ub), 0, dmesh(h), h->cell_start + h->nc, dparts(h),
+         dvfield(h));
+  h->pending_corr = false;
+  return PFEM2GPU_OK;
+}
+
+int flush(pfem2gpu_handle* h) { return h->dim == 2 ? flush_correction<2>(h) : flush_correction<3>(h); }
+
+// count -> scan -> scatter -> settle(count) -> scan -> settle(write). With
+// advect = 0 this re-bins the first n_cur + n_add particles without moving
+// them and without the per-part cap or refills.
+template <int D>
+int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
+  const DMesh m = dmesh(h);
+  const DParts P = dparts(h);
+  const DTmp T = dtmp(h);
+  const long nmax = h->n_ub + n_add;
+  const int g = grid_for(nmax);
+  const int* nptr = h->cell_start + h->nc;
+  int* cnt = h->count;
+  int* fill = h->count + (h->nc + 1);
+  const int use_dv = (advect && h->pending_corr) ? 1 : 0;
+  const DField dv = dvfield(h);
+  if (!advect) CHK(flush_correction<D>(h));
+  LAUNCH(h, "k_fill", k_fill_int, grid_for(2L * (h->nc + 1)), 0, h->count, 2L * (h->nc + 1), 0);
+  LAUNCH(h, advect ? "k_move_count" : "k_rebin_count", k_move<D>, g, 0, 0, advect, m, nptr, n_add, P, Vf, dv, 0,
+         cnt, h->tstart, fill, T);
+  CHK(scan(h, cnt, h->nc, h->tstart));
+  LAUNCH(h, advect ? "k_move_scatter" : "k_rebin_scatter", k_move<D>, g, 0, 1, advect, m, nptr, n_add, P, Vf, dv,
+         use_dv, cnt, h->tstart, fill, T);
+  if (use_dv) h->pending_corr = false;
+  const int nchunks = (h->nc + h->cpw - 1) / h->cpw;
+  const int gs = std::min(nchunks, 16384);
+  const int cap_on = advect, refill_on = advect;
+  LAUNCH(h, "k_settle_count", k_settle<D>, gs, h->settle_smem, 0, cap_on, refill_on, m, h->nc, h->cpw, h->pmax,
+         h->tstart, T, h->fcount, h->fstart, P, Vf);
+  CHK(scan(h, h->fcount, h->nc, h->fstart));
+  LAUNCH(h, "k_settle_write", k_settle<D>, gs, h->settle_smem, 1, cap_on, refill_on, m, h->nc, h->cpw, h->pmax,
+         h->tstart, T, h->fcount, h->fstart, P, Vf);
+  std::swap(h->cell_start, h->fstart);
+  h->n_ub = nmax + (advect ? h->refill_max : 0);
+  return PFEM2GPU_OK;
+}
+
+int do_rebin(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
+  return h->dim == 2 ? rebin_or_move<2>(h, advect, n_add, Vf) : rebin_or_move<3>(h, advect, n_add, Vf);
+}
+
+// Stages n particles (device pointers) and appends + re-bins them.
+int append_staged(pfem2gpu_handle* h, long n, const double* const* sx, const double* const* sv, const int* scell,
+                  int64_t* n_rejected) {
+  HIPCHK(hipMemsetAsync(h->d_rejected, 0, sizeof(unsigned long long), h->stream));
+  const DMesh m = dmesh(h);
+  const double* z = nullptr;
+  const double* svv[3] = {sv ? sv[0] : z, sv ? sv[1] : z, (sv && h->dim > 2) ? sv[2] : z};
+  const double* sxx[3] = {sx[0], sx[1], h->dim > 2 ? sx[2] : z};
+  if (h->dim == 2) {
+    LAUNCH(h, "k_append", k_append<2>, grid_for(n), 0, m, h->nc, h->cell_start + h->nc, (int)n, sxx[0], sxx[1],
+           sxx[2], svv[0], svv[1], svv[2], scell, dparts(h), h->d_rejected);
+  } else {
+    LAUNCH(h, "k_append", k_append<3>, grid_for(n), 0, m, h->nc, h->cell_start + h->nc, (int)n, sxx[0], sxx[1],
+           sxx[2], svv[0], svv[1], svv[2], scell, dparts(h), h->d_rejected);
+  }
+  DField none;
+  for (int k = 0; k < 3; ++k) none.c[k] = nullptr;
+  CHK(do_rebin(h, 0, (int)n, none));
+  if (n_rejected) {
+    unsigned long long r = 0;
+    HIPCHK(hipMemcpyAsync(&r, h->d_rejected, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *n_rejected = (int64_t)r;
+  }
+  return PFEM2GPU_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+const char* pfem2gpu_last_error(void) { return g_err.c_str(); }
+
+int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, void* hip_stream,
+                    pfem2gpu_handle** out) {
+  if (!mesh || !params || !out) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (mesh->dim != 2 && mesh->dim != 3) return fail(PFEM2GPU_ERR_ARG, "dim must be 2 or 3");
+  if (mesh->n_cells <= 0 || mesh->n_vertices <= 0 || mesh->n_dofs <= 0 || !mesh->vertices || !mesh->cell_vertices)
+    return fail(PFEM2GPU_ERR_ARG, "empty mesh");
+  if (mesh->n_cells > 0x7fffffff / 8 || mesh->n_vertices > 0x7fffffff || mesh->n_dofs > 0x7fffffff)
+    return fail(PFEM2GPU_ERR_ARG, "mesh too large for int32 indexing");
+  const int dim = mesh->dim, nvc = 1 << dim;
+  for (int k = 0; k < dim; ++k)
+    if (params->particles_per_cell[k] == 0 || params->particles_per_cell[k] > 64)
+      return fail(PFEM2GPU_ERR_ARG, "particles_per_cell must be in [1, 64]");
+  const int32_t* cdofs = mesh->cell_dofs ? mesh->cell_dofs : mesh->cell_vertices;
+  for (long i = 0; i < mesh->n_cells * nvc; ++i) {
+    if (mesh->cell_vertices[i] < 0 || mesh->cell_vertices[i] >= mesh->n_vertices)
+      return fail(PFEM2GPU_ERR_ARG, "cell_vertices entry out of range");
+    if (cdofs[i] < 0 || cdofs[i] >= mesh->n_dofs) return fail(PFEM2GPU_ERR_ARG, "cell_dofs entry out of range");
+  }
+
+  pfem2gpu_handle* h = new pfem2gpu_handle();
+  h->dim = dim;
+  h->nvc = nvc;
+  h->nc = (int)mesh->n_cells;
+  h->nv = (int)mesh->n_vertices;
+  h->ndofs = (int)mesh->n_dofs;
+  if (hip_stream) {
+    h->stream = (hipStream_t)hip_stream;
+  } else {
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete h;
+      return fail(PFEM2GPU_ERR_HIP, "hipStreamCreate failed");
+    }
+    h->own_stream = true;
+  }
+  auto bail = [&](int rc) {
+    pfem2gpu_destroy(h);
+    return rc;
+  };
+
+  // parameters (src/pfem2particlehandler.cpp:34-39)
+  Prm& p = h->prm_host;
+  p.n_sub = params->particle_integration_steps == 0 ? 1 : (int)params->particle_integration_steps;
+  p.dt_sub = params->time_step / (double)p.n_sub;
+  p.maxpp = (int)params->max_particles_per_cell_part;
+  p.P = 1;
+  p.Pvalid = 1;
+  for (int k = 0; k < 3; ++k) {
+    p.q[k] = k < dim ? (int)params->particles_per_cell[k] : 1;
+    p.Q[k] = k < dim ? p.q[k] + 1 : 1;
+    p.h[k] = 1.0 / (double)(unsigned)p.q[k];
+  }
+  for (int k = 0; k < dim; ++k) {
+    p.P *= p.Q[k];
+    p.Pvalid *= p.q[k];
+  }
+  h->refill_max = h->nc * p.Pvalid;
+  h->pmax = p.P;
+  h->cpw = 64;
+  while (h->cpw > 1 && 2L * h->cpw * h->pmax * (long)sizeof(int) > 48 * 1024) h->cpw /= 2;
+  h->settle_smem = 2 * h->cpw * h->pmax * (int)sizeof(int);
+
+  int rc;
+  // mesh upload
+  h->hX.assign(mesh->vertices, mesh->vertices + (size_t)dim * h->nv);
+  h->hcv.assign(mesh->cell_vertices, mesh->cell_vertices + (size_t)h->nc * nvc);
+  for (int k = 0; k < dim; ++k) {
+    if ((rc = dalloc(&h->X[k], h->nv))) return bail(rc);
+    if (hipMemcpy(h->X[k], mesh->vertices + (size_t)k * h->nv, h->nv * sizeof(double), hipMemcpyHostToDevice))
+      return bail(fail(PFEM2GPU_ERR_HIP, "upload vertices"));
+  }
+  if ((rc = dalloc(&h->cv, (size_t)h->nc * nvc)) || (rc = dalloc(&h->cd, (size_t)h->nc * nvc)) ||
+      (rc = dalloc(&h->box, h->nc)))
+    return bail(rc);
+  if (hipMemcpy(h->cv, mesh->cell_vertices, (size_t)h->nc * nvc * 4, hipMemcpyHostToDevice) ||
+      hipMemcpy(h->cd, cdofs, (size_t)h->nc * nvc * 4, hipMemcpyHostToDevice))
+    return bail(fail(PFEM2GPU_ERR_HIP, "upload cells"));
+  // axis-aligned box flag: vertex i has coordinate k of vertex 2^k if bit k
+  // of i is set, else that of vertex 0
+  {
+    std::vector<unsigned char> box(h->nc);
+    const double* V = mesh->vertices;
+    for (int c = 0; c < h->nc; ++c) {
+      const int32_t* cvr = mesh->cell_vertices + (size_t)c * nvc;
+      bool ok = true;
+      for (int i = 0; i < nvc && ok; ++i)
+        for (int k = 0; k < dim; ++k) {
+          const double want = ((i >> k) & 1) ? V[(size_t)k * h->nv + cvr[1 << k]] : V[(size_t)k * h->nv + cvr[0]];
+          if (!(V[(size_t)k * h->nv + cvr[i]] == want)) ok = false;
+        }
+      box[c] = ok ? 1 : 0;
+    }
+    if (hipMemcpy(h->box, box.data(), h->nc, hipMemcpyHostToDevice)) return bail(fail(PFEM2GPU_ERR_HIP, "box"));
+  }
+  // dof -> (cell*nvc + i), ascending cell
+  {
+    std::vector<int> off(h->ndofs + 1, 0), lst((size_t)h->nc * nvc);
+    for (long e = 0; e < (long)h->nc * nvc; ++e) off[cdofs[e] + 1]++;
+    for (int j = 0; j < h->ndofs; ++j) off[j + 1] += off[j];
+    std::vector<int> pos(off.begin(), off.end() - 1);
+    for (long e = 0; e < (long)h->nc * nvc; ++e) lst[pos[cdofs[e]]++] = (int)e;
+    if ((rc = dalloc(&h->d2c_off, h->ndofs + 1)) || (rc = dalloc(&h->d2c, lst.size()))) return bail(rc);
+    if (hipMemcpy(h->d2c_off, off.data(), off.size() * 4, hipMemcpyHostToDevice) ||
+        hipMemcpy(h->d2c, lst.data(), lst.size() * 4, hipMemcpyHostToDevice))
+      return bail(fail(PFEM2GPU_ERR_HIP, "upload dof map"));
+  }
+  if ((rc = dalloc(&h->prm, 1))) return bail(rc);
+  if (hipMemcpy(h->prm, &h->prm_host, sizeof(Prm), hipMemcpyHostToDevice)) return bail(fail(PFEM2GPU_ERR_HIP, "prm"));
+
+  // per-cell work arrays
+  h->scan_nb = (h->nc + kScanChunk - 1) / kScanChunk;
+  if ((rc = dalloc(&h->cell_start, h->nc + 1)) || (rc = dalloc(&h->fstart, h->nc + 1)) ||
+      (rc = dalloc(&h->count, 2 * (size_t)(h->nc + 1))) || (rc = dalloc(&h->tstart, h->nc + 1)) ||
+      (rc = dalloc(&h->fcount, h->nc + 1)) || (rc = dalloc(&h->scan_blk, h->scan_nb + 1)) ||
+      (rc = dalloc(&h->contrib, (size_t)h->nc * nvc * (dim + 1))) || (rc = dalloc(&h->d_rejected, 1)))
+    return bail(rc);
+  for (int k = 0; k < dim; ++k)
+    if ((rc = dalloc(&h->dV[k], h->ndofs)) || (rc = dalloc(&h->stageF[0][k], h->ndofs)) ||
+        (rc = dalloc(&h->stageF[1][k], h->ndofs)))
+      return bail(rc);
+  if (hipHostMalloc(reinterpret_cast<void**>(&h->h_pinned), sizeof(int) * 4, 0) != hipSuccess)
+    return bail(fail(PFEM2GPU_ERR_MEMORY, "hipHostMalloc"));
+  if (hipMemsetAsync(h->cell_start, 0, (h->nc + 1) * sizeof(int), h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    return bail(fail(PFEM2GPU_ERR_HIP, "init"));
+  if ((rc = grow(h, 1024, 0))) return bail(rc);
+  *out = h;
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_destroy(pfem2gpu_handle* h) {
+  if (!h) return PFEM2GPU_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_particles(h);
+  for (int k = 0; k < 3; ++k) {
+    dfree(h->X[k]);
+    dfree(h->dV[k]);
+    dfree(h->stageF[0][k]);
+    dfree(h->stageF[1][k]);
+  }
+  dfree(h->cv);
+  dfree(h->cd);
+  dfree(h->box);
+  dfree(h->v2c_off);
+  dfree(h->v2c);
+  dfree(h->v2c_dir);
+  dfree(h->d2c_off);
+  dfree(h->d2c);
+  dfree(h->prm);
+  dfree(h->cell_start);
+  dfree(h->fstart);
+  dfree(h->count);
+  dfree(h->tstart);
+  dfree(h->fcount);
+  dfree(h->scan_blk);
+  dfree(h->contrib);
+  dfree(h->d_rejected);
+  if (h->h_pinned) (void)hipHostFree(h->h_pinned);
+  for (auto& t : h->pending) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (auto e : h->pool) (void)hipEventDestroy(e);
+  if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return PFEM2GPU_OK;
+}
+
+// GridTools::vertex_to_cell_map + vertex_to_cell_centers_directions, built on
+// the host and uploaded. Centre = vertex sum * (1/n_vertices); direction
+// normalised by multiplying with 1/norm (deal.II Tensor division).
+int pfem2gpu_initialize_maps(pfem2gpu_handle* h) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  const int dim = h->dim, nvc = h->nvc;
+  std::vector<int> off(h->nv + 1, 0);
+  for (long e = 0; e < (long)h->nc * nvc; ++e) off[h->hcv[e] + 1]++;
+  for (int v = 0; v < h->nv; ++v) off[v + 1] += off[v];
+  std::vector<int> lst(off[h->nv]);
+  {
+    std::vector<int> pos(off.begin(), off.end() - 1);
+    for (int c = 0; c < h->nc; ++c)
+      for (int i = 0; i < nvc; ++i) lst[pos[h->hcv[(size_t)c * nvc + i]]++] = c;
+  }
+  // cells are visited in ascending order; drop repeats (degenerate cells)
+  std::vector<int> off2(h->nv + 1, 0), lst2;
+  lst2.reserve(lst.size());
+  for (int v = 0; v < h->nv; ++v) {
+    int last = -1;
+    for (int e = off[v]; e < off[v + 1]; ++e)
+      if (lst[e] != last) {
+        lst2.push_back(lst[e]);
+        last = lst[e];
+      }
+    off2[v + 1] = (int)lst2.size();
+  }
+  std::vector<double> dirs(lst2.size() * dim);
+  const double inv_n = 1.0 / (double)nvc;
+  for (int v = 0; v < h->nv; ++v)
+    for (int e = off2[v]; e < off2[v + 1]; ++e) {
+      const int c = lst2[e];
+      double ctr[3] = {0, 0, 0}, d[3] = {0, 0, 0};
+      for (int i = 0; i < nvc; ++i)
+        for (int k = 0; k < dim; ++k) ctr[k] = ctr[k] + h->hX[(size_t)k * h->nv + h->hcv[(size_t)c * nvc + i]];
+      for (int k = 0; k < dim; ++k) d[k] = ctr[k] * inv_n - h->hX[(size_t)k * h->nv + v];
+      double m2 = 0.0;
+      for (int k = 0; k < dim; ++k) m2 = m2 + d[k] * d[k];
+      const double inv_m = 1.0 / std::sqrt(m2);
+      for (int k = 0; k < dim; ++k) dirs[(size_t)e * dim + k] = d[k] * inv_m;
+    }
+  dfree(h->v2c_off);
+  dfree(h->v2c);
+  dfree(h->v2c_dir);
+  CHK(dalloc(&h->v2c_off, off2.size()));
+  CHK(dalloc(&h->v2c, lst2.size()));
+  CHK(dalloc(&h->v2c_dir, dirs.size()));
+  HIPCHK(hipMemcpy(h->v2c_off, off2.data(), off2.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->v2c, lst2.data(), std::max<size_t>(1, lst2.size()) * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->v2c_dir, dirs.data(), std::max<size_t>(1, dirs.size()) * 8, hipMemcpyHostToDevice));
+  h->maps_ready = true;
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_insert_particles(pfem2gpu_handle* h, int64_t n, const double* x, const int32_t* cell, const double* v,
+                              int where, int64_t* n_rejected) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  if (n_rejected) *n_rejected = 0;
+  if (n < 0 || (n > 0 && (!x || !cell))) return fail(PFEM2GPU_ERR_ARG, "bad particle arrays");
+  if (n == 0) return PFEM2GPU_OK;
+  if (n > 0x7fffffffL) return fail(PFEM2GPU_ERR_ARG, "too many particles");
+  if (where != PFEM2GPU_HOST && where != PFEM2GPU_DEVICE) return fail(PFEM2GPU_ERR_ARG, "bad where");
+  CHK(flush(h));
+  CHK(reserve(h, n));
+  const int dim = h->dim;
+  const double* sx[3] = {nullptr, nullptr, nullptr};
+  const double* sv[3] = {nullptr, nullptr, nullptr};
+  const int* sc = nullptr;
+  if (where == PFEM2GPU_DEVICE) {
+    for (int k = 0; k < dim; ++k) {
+      sx[k] = x + (size_t)k * n;
+      if (v) sv[k] = v + (size_t)k * n;
+    }
+    sc = cell;
+  } else {
+    // stage host arrays in the T buffers (free between steps)
+    for (int k = 0; k < dim; ++k) {
+      HIPCHK(hipMemcpyAsync(h->tx[k], x + (size_t)k * n, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+      sx[k] = h->tx[k];
+      if (v) {
+        HIPCHK(hipMemcpyAsync(h->tv[k], v + (size_t)k * n, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        sv[k] = h->tv[k];
+      }
+    }
+    HIPCHK(hipMemcpyAsync(h->tcell, cell, n * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    sc = h->tcell;
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  // the staged T arrays are consumed by k_append before the re-bin reuses T
+  return append_staged(h, n, sx, v ? sv : nullptr, sc, n_rejected);
+}
+
+int pfem2gpu_seed_particles(pfem2gpu_handle* h, const double* const* V, int where) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  DField F;
+  CHK(field_in(h, V, where, 0, &F));
+  CHK(flush(h));
+  const long n = (long)h->nc * h->prm_host.Pvalid;
+  if (n > 0x7fffffffL) return fail(PFEM2GPU_ERR_ARG, "too many particles");
+  CHK(reserve(h, n));
+  const DMesh m = dmesh(h);
+  if (h->dim == 2)
+    LAUNCH(h, "k_seed", k_seed<2>, grid_for(n), 0, m, h->nc, h->prm_host.Pvalid, F, h->tx[0], h->tx[1], h->tx[2],
+           h->tv[0], h->tv[1], h->tv[2], h->tcell);
+  else
+    LAUNCH(h, "k_seed", k_seed<3>, grid_for(n), 0, m, h->nc, h->prm_host.Pvalid, F, h->tx[0], h->tx[1], h->tx[2],
+           h->tv[0], h->tv[1], h->tv[2], h->tcell);
+  const double* sx[3] = {h->tx[0], h->tx[1], h->tx[2]};
+  const double* sv[3] = {h->tv[0], h->tv[1], h->tv[2]};
+  return append_staged(h, n, sx, sv, h->tcell, nullptr);
+}
+
+int pfem2gpu_clear_particles(pfem2gpu_handle* h) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  HIPCHK(hipMemsetAsync(h->cell_start, 0, (h->nc + 1) * sizeof(int), h->stream));
+  h->pending_corr = false;
+  h->n_ub = 0;
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_set_velocity_from_field(pfem2gpu_handle* h, const double* const* V, int where) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  DField F;
+  CHK(field_in(h, V, where, 0, &F));
+  h->pending_corr = false;  // overwritten anyway
+  if (h->dim == 2)
+    LAUNCH(h, "k_setvel", k_setvel<2>, grid_for(h->n_ub), 0, dmesh(h), h->cell_start + h->nc, dparts(h), F);
+  else
+    LAUNCH(h, "k_setvel", k_setvel<3>, grid_for(h->n_ub), 0, dmesh(h), h->cell_start + h->nc, dparts(h), F);
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_correct_particle_velocity(pfem2gpu_handle* h, const double* const* V, const double* const* V_old,
+                                       int where) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  CHK(flush(h));
+  DField A, B;
+  CHK(field_in(h, V, where, 0, &A));
+  CHK(field_in(h, V_old, where, 1, &B));
+  if (h->dim == 2)
+    LAUNCH(h, "k_sub", k_sub<2>, grid_for(h->ndofs), 0, h->ndofs, A, B, h->dV[0], h->dV[1], h->dV[2]);
+  else
+    LAUNCH(h, "k_sub", k_sub<3>, grid_for(h->ndofs), 0, h->ndofs, A, B, h->dV[0], h->dV[1], h->dV[2]);
+  h->pending_corr = true;
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_move_particles(pfem2gpu_handle* h, const double* const* V, int where) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  if (!h->maps_ready) return fail(PFEM2GPU_ERR_STATE, "initialize_maps() must be called before move_particles()");
+  DField F;
+  CHK(field_in(h, V, where, 0, &F));
+  CHK(reserve(h, h->refill_max));
+  return do_rebin(h, 1, 0, F);
+}
+
+int pfem2gpu_project_particle_fields(pfem2gpu_handle* h, double* const* V_out, int where) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  if (!V_out) return fail(PFEM2GPU_ERR_ARG, "null output");
+  if (where != PFEM2GPU_HOST && where != PFEM2GPU_DEVICE) return fail(PFEM2GPU_ERR_ARG, "bad where");
+  CHK(flush(h));
+  const long ne = (long)h->nc * h->nvc;
+  double* out[3] = {nullptr, nullptr, nullptr};
+  for (int k = 0; k < h->dim; ++k) out[k] = (where == PFEM2GPU_DEVICE) ? V_out[k] : h->stageF[1][k];
+  if (h->dim == 2) {
+    LAUNCH(h, "k_contrib", k_contrib<2>, grid_for(ne), 0, dmesh(h), h->nc, h->cell_start, dparts(h), h->contrib);
+    LAUNCH(h, "k_node", k_node<2>, grid_for(h->ndofs), 0, h->ndofs, h->d2c_off, h->d2c, h->contrib, out[0], out[1],
+           out[2]);
+  } else {
+    LAUNCH(h, "k_contrib", k_contrib<3>, grid_for(ne), 0, dmesh(h), h->nc, h->cell_start, dparts(h), h->contrib);
+    LAUNCH(h, "k_node", k_node<3>, grid_for(h->ndofs), 0, h->ndofs, h->d2c_off, h->d2c, h->contrib, out[0], out[1],
+           out[2]);
+  }
+  if (where == PFEM2GPU_HOST) {
+    for (int k = 0; k < h->dim; ++k)
+      HIPCHK(hipMemcpyAsync(V_out[k], out[k], h->ndofs * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_particle_count(pfem2gpu_handle* h, int64_t* n) {
+  if (!h || !n) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  long c = 0;
+  CHK(read_count(h, &c));
+  *n = c;
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_particle_count_async(pfem2gpu_handle* h, int32_t* dst) {
+  if (!h || !dst) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(dst, h->cell_start + h->nc, sizeof(int), hipMemcpyDeviceToDevice, h->stream));
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_download(pfem2gpu_handle* h, double* x, double* xi, double* v, int32_t* cell) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  CHK(flush(h));
+  long n = 0;
+  CHK(read_count(h, &n));
+  if (n == 0) return PFEM2GPU_OK;
+  for (int k = 0; k < h->dim; ++k) {
+    if (x) HIPCHK(hipMemcpyAsync(x + (size_t)k * n, h->px[k], n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (v) HIPCHK(hipMemcpyAsync(v + (size_t)k * n, h->pv[k], n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  if (cell) HIPCHK(hipMemcpyAsync(cell, h->pcell, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  if (xi) {
+    if (h->dim == 2)
+      LAUNCH(h, "k_xi", k_xi<2>, grid_for(n), 0, dmesh(h), h->cell_start + h->nc, dparts(h), h->tx[0], h->tx[1],
+             h->tx[2]);
+    else
+      LAUNCH(h, "k_xi", k_xi<3>, grid_for(n), 0, dmesh(h), h->cell_start + h->nc, dparts(h), h->tx[0], h->tx[1],
+             h->tx[2]);
+    for (int k = 0; k < h->dim; ++k)
+      HIPCHK(hipMemcpyAsync(xi + (size_t)k * n, h->tx[k], n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_cell_counts(pfem2gpu_handle* h, int32_t* counts) {
+  if (!h || !counts) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  std::vector<int> cs(h->nc + 1);
+  HIPCHK(hipMemcpyAsync(cs.data(), h->cell_start, (h->nc + 1) * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int c = 0; c < h->nc; ++c) counts[c] = cs[c + 1] - cs[c];
+  return PFEM2GPU_OK;
+}
+
+void* pfem2gpu_stream(pfem2gpu_handle* h) { return h ? (void*)h->stream : nullptr; }
+
+int pfem2gpu_synchronize(pfem2gpu_handle* h) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_set_timing(pfem2gpu_handle* h, int enable) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  CHK(t_collect(h));
+  h->totals.clear();
+  h->timing = enable != 0;
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_timings(pfem2gpu_handle* h, int cap, const char** names, int64_t* launches, double* total_ms,
+                     int* n_out) {
+  if (!h || !n_out) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  CHK(t_collect(h));
+  h->names_store.clear();
+  for (auto& e : h->totals) h->names_store.push_back(e.first);
+  int i = 0;
+  for (auto& e : h->totals) {
+    if (i >= cap) break;
+    if (names) names[i] = h->names_store[i].c_str();
+    if (launches) launches[i] = e.second.first;
+    if (total_ms) total_ms[i] = e.second.second;
+    ++i;
+  }
+  *n_out = (int)h->totals.size();
+  return PFEM2GPU_OK;
+}
+
+}  // extern "C"

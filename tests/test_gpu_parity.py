@@ -1,0 +1,143 @@
+"""GPU vs oracle parity through the C ABI (ctypes). Marked gpu.
+
+Each test prints the case name before touching the GPU so a failing run names
+its case."""
+import sys
+
+import numpy as np
+import pytest
+
+from pfem2dealii_amd import workloads as W
+from tests.harness import (GpuStepper, OracleStepper, assert_fields_match, assert_particles_match,
+                           perturbed_mesh, seed_in_cells)
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def _run_pair(cfg, steps, mesh=None, x=None, cells=None):
+    mesh = mesh or cfg["mesh"]
+    x = cfg["x"] if x is None else x
+    cells = cfg["cells"] if cells is None else cells
+    V = cfg["V"]
+    o = OracleStepper(cfg, mesh=mesh)
+    g = GpuStepper(cfg, mesh=mesh)
+    assert o.insert(x, cells, V) == g.insert(x, cells, V)
+    assert_particles_match(g.download(), o.download())
+    for s in range(steps):
+        Vo = o.step(V)
+        Vg = g.step(V)
+        assert_fields_match(Vg, Vo)
+        # canonical order makes the projection bitwise identical
+        assert np.array_equal(np.isnan(Vg), np.isnan(Vo))
+        assert np.array_equal(np.nan_to_num(Vg, nan=7.0), np.nan_to_num(Vo, nan=7.0)), "projection not bitwise"
+        assert_particles_match(g.download(), o.download())
+    return o, g
+
+
+def test_channel2d_one_step(gpu_available):
+    _say("channel2d 64x32 jittered lattice, 1 step")
+    cfg = W.config("channel2d")
+    o, g = _run_pair(cfg, 1)
+    assert np.array_equal(g.h.cell_counts(), np.bincount(o.download()["cell"], minlength=2048))
+
+
+def test_channel2d_three_steps(gpu_available):
+    _say("channel2d 64x32, 3 steps")
+    _run_pair(W.config("channel2d"), 3)
+
+
+def test_taylor_green_small(gpu_available):
+    _say("taylor_green2d scale 1/16 (128x64), 3 steps")
+    _run_pair(W.config("taylor_green2d", scale=1.0 / 16), 3)
+
+
+def test_abc3d_small(gpu_available):
+    _say("abc3d scale 1/8 (16^3), 3 steps")
+    _run_pair(W.config("abc3d", scale=1.0 / 8), 3)
+
+
+def test_perturbed_2d_newton(gpu_available):
+    _say("perturbed 2D mesh 24x12 (Newton inverse map), 3 steps")
+    cfg = W.config("channel2d", scale=0.375)
+    mesh = perturbed_mesh(cfg["mesh"], 0.2, seed=11)
+    x, cells = seed_in_cells(mesh, 9, seed=12)
+    _run_pair(cfg, 3, mesh=mesh, x=x, cells=cells)
+
+
+def test_perturbed_3d_newton(gpu_available):
+    _say("perturbed 3D mesh 8^3 (Newton inverse map), 2 steps")
+    cfg = W.config("abc3d", scale=1.0 / 16)
+    mesh = perturbed_mesh(cfg["mesh"], 0.15, seed=13)
+    x, cells = seed_in_cells(mesh, 6, seed=14)
+    _run_pair(cfg, 2, mesh=mesh, x=x, cells=cells)
+
+
+def test_seed_lattice(gpu_available):
+    _say("seed_particles lattice 2D and 3D")
+    for name, scale in (("channel2d", 0.5), ("abc3d", 1.0 / 16)):
+        cfg = W.config(name, scale=scale)
+        from oracle import Oracle
+        from pfem2dealii_amd import ParticleHandler
+        o = Oracle(cfg["mesh"], cfg["params"])
+        o.initialize_maps()
+        o.seed_lattice(cfg["V"])
+        h = ParticleHandler(cfg["mesh"], cfg["params"])
+        h.initialize_maps()
+        h.seed_particles(cfg["V"])
+        assert_particles_match(h.download(), o.download())
+        for _ in range(2):
+            o.move(cfg["V"])
+            h.move_particles(cfg["V"])
+            assert_particles_match(h.download(), o.download())
+
+
+def test_insert_rejects_outside(gpu_available):
+    _say("insert with particles outside their cell")
+    cfg = W.config("channel2d", scale=0.25)
+    x = cfg["x"].copy()
+    cells = cfg["cells"].copy()
+    cells[::7] = (cells[::7] + 3) % cells.max()
+    cells[5] = -1
+    cells[9] = 10 ** 6
+    o = OracleStepper(cfg)
+    g = GpuStepper(cfg)
+    ro = o.insert(x, cells, cfg["V"])
+    rg = g.insert(x, cells, cfg["V"])
+    assert ro == rg and rg > 0
+    assert_particles_match(g.download(), o.download())
+
+
+def test_empty_handler(gpu_available):
+    _say("empty handler: move and project with no particles")
+    cfg = W.config("channel2d", scale=0.25)
+    g = GpuStepper(cfg)
+    g.h.move_particles(cfg["V"])
+    assert g.h.count() == 0
+    Vp = g.h.project_particle_fields()
+    assert np.isnan(Vp).all()
+
+
+def test_device_pointers(gpu_available):
+    _say("device-resident fields through torch tensors")
+    import torch
+    cfg = W.config("channel2d", scale=0.5)
+    o = OracleStepper(cfg)
+    g = GpuStepper(cfg)
+    o.insert(cfg["x"], cfg["cells"], cfg["V"])
+    g.insert(cfg["x"], cfg["cells"], cfg["V"])
+    Vd = torch.tensor(cfg["V"], dtype=torch.float64, device="cuda")
+    out = torch.empty_like(Vd)
+    for _ in range(2):
+        Vo = o.step(cfg["V"])
+        torch.cuda.synchronize()
+        g.h.move_particles(Vd)
+        g.h.project_particle_fields(out)
+        g.h.correct_particle_velocity(Vd, out)
+        g.h.synchronize()
+        assert_fields_match(out.cpu().numpy(), Vo)
+        assert_particles_match(g.download(), o.download())

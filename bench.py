@@ -139,10 +139,10 @@ def main():
     h.set_timing(False)
     nodes = mesh["X"].shape[1]
     b_alg = alg_bytes_per_particle_step(dim, nodes, psteps / args.steps)
-    step_ms = sum(ms for (_, ms) in tim.values()) / args.profile_steps
+    step_ms = sum(ms for (_, ms) in tim.values()) / max(1, args.profile_steps)
     achieved_step = b_alg * psteps / args.steps / (ms_per_step * 1e-3) / 1e9
-    kern = {k: {"launches": n, "avg_ms": ms / n, "ms_per_step": ms / args.profile_steps} for k, (n, ms) in tim.items()}
-    dom = max(kern.items(), key=lambda kv: kv[1]["ms_per_step"])
+    kern = {k: {"launches": n, "avg_ms": ms / n, "ms_per_step": ms / max(1, args.profile_steps)} for k, (n, ms) in tim.items()}
+    dom = max(kern.items(), key=lambda kv: kv[1]["ms_per_step"]) if kern else ("none", {"avg_ms": 0.0, "ms_per_step": 0.0})
     # the dominant kernel's share of the step's algorithmic bytes is not
     # separable; its achieved rate is the algorithmic bytes of the whole step
     # over its own duration, an upper bound on its share.

@@ -6,7 +6,7 @@
 //               order inside a cell is (part bucket, canonical key);
 //               cell_start[n_cells+1] delimits the cell segments and its last
 //               entry is the particle count.
-//   staging T   the same SoA arrays plus part (u16) and key (u64): the move
+//   staging T   the same SoA x, v arrays (+ cell for insertion): the move
 //               passes scatter into it, the settle kernel compacts back.
 //   mesh        vertex coordinates SoA, cell->vertex and cell->dof tables
 //               (16/32 B per cell, one vector load), box flags, the
@@ -69,6 +69,11 @@ struct pfem2gpu_handle {
   int* d2c_off = nullptr;
   int* d2c = nullptr;
   Prm* prm = nullptr;
+  double* geom = nullptr;  // [nc][2*dim] box corners (X_0[0] = NaN for other cells)
+  int2* cvo = nullptr;     // [nc][nvc] (v2c offset, count) per cell vertex
+  int* ptab = nullptr;     // [pmax] part decode table
+  double* Vc = nullptr;    // [nc][nvc][dim] advecting field, cell-major
+  int pshift = 0;
   bool maps_ready = false;
   std::vector<double> hX;  // host copies used by initialize_maps
   std::vector<int> hcv;
@@ -81,13 +86,15 @@ struct pfem2gpu_handle {
   double* tx[3] = {nullptr, nullptr, nullptr};
   double* tv[3] = {nullptr, nullptr, nullptr};
   int* tcell = nullptr;
-  unsigned short* tpart = nullptr;
-  unsigned long long* tkey = nullptr;
   int* cell_start = nullptr;  // [nc+1]
   int* fstart = nullptr;      // [nc+1] next layout (swapped with cell_start)
-  int* count = nullptr;       // [2*(nc+1)]: count | fill
-  int* tstart = nullptr;      // [nc+1]
-  int* fcount = nullptr;      // [nc+1]
+  int* count = nullptr;       // [3*nchunks+2]: bucket counts | spill counts | spill fill | spill total | ticket
+  int* sstart = nullptr;      // [nchunks+1] spill offsets per chunk
+  unsigned long long* lb = nullptr;  // [nchunks] look-back status words of the settle kernel
+  int* d_err = nullptr;       // device-side error flags (look-back timeout)
+  int nchunks = 0, cshift = 4;
+  int capc = 0;               // bucket capacity per chunk
+  long tsize = 0;             // staging slots: nchunks*capc + 2*cap
   int* scan_blk = nullptr;
   int scan_nb = 0;
   double* contrib = nullptr;  // [nc][nvc][dim+1]
@@ -97,7 +104,9 @@ struct pfem2gpu_handle {
   unsigned long long* d_rejected = nullptr;
   int* h_pinned = nullptr;
   long n_ub = 0;  // host-side upper bound of the particle count
-  int cpw = 64, pmax = 1, settle_smem = 0;
+  int cpw = 16, pmax = 1, nmax = 512, nslots = 0, settle_smem = 0;  // cpw = 1 << cshift
+  unsigned char* dirty = nullptr;  // per settle chunk: sums left to k_contrib_dirty
+  bool contrib_valid = false;      // per (cell, vertex) sums match the particles
 
   // timing
   bool timing = false;
@@ -197,6 +206,9 @@ DMesh dmesh(pfem2gpu_handle* h) {
   m.v2c_dir = h->v2c_dir;
   m.cell_inst = nullptr;
   m.prm = h->prm;
+  m.geom = h->geom;
+  m.cvo = h->cvo;
+  m.ptab = h->ptab;
   return m;
 }
 
@@ -217,8 +229,6 @@ DTmp dtmp(pfem2gpu_handle* h) {
     t.v[k] = h->tv[k];
   }
   t.cell = h->tcell;
-  t.part = h->tpart;
-  t.key = h->tkey;
   return t;
 }
 
@@ -231,63 +241,105 @@ void free_particles(pfem2gpu_handle* h) {
   }
   dfree(h->pcell);
   dfree(h->tcell);
-  dfree(h->tpart);
-  dfree(h->tkey);
   h->cap = 0;
+  h->tsize = 0;
 }
 
 // Grows the particle arrays to hold at least `need` particles, keeping the
-// first `keep` current ones.
-int grow(pfem2gpu_handle* h, long need, long keep) {
-  if (need <= h->cap) return PFEM2GPU_OK;
-  long ncap = std::max(need, h->cap + h->cap / 2);
-  ncap = std::max(ncap, 1024L);
-  if (ncap > 0x7fffffffL) return fail(PFEM2GPU_ERR_MEMORY, "particle capacity exceeds int32 indexing");
-  double *nx[3] = {nullptr, nullptr, nullptr}, *nvv[3] = {nullptr, nullptr, nullptr};
-  int* ncell = nullptr;
-  for (int k = 0; k < h->dim; ++k) {
-    CHK(dalloc(&nx[k], ncap));
-    CHK(dalloc(&nvv[k], ncap));
+// first `keep` current ones, and the staging arrays to nchunks*capc bucket
+// slots plus two spill regions of the particle capacity.
+int grow(pfem2gpu_handle* h, long need, long keep, int capc) {
+  long ncap = h->cap;
+  if (need > h->cap) {
+    ncap = std::max(need, h->cap + h->cap / 2);
+    ncap = std::max(ncap, 1024L);
+    if (ncap > 0x7fffffffL) return fail(PFEM2GPU_ERR_MEMORY, "particle capacity exceeds int32 indexing");
   }
-  CHK(dalloc(&ncell, ncap));
-  if (keep > 0) {
+  capc = std::max(capc, h->capc);
+  const long tsize = (long)h->nchunks * capc + 2 * ncap;
+  if (ncap != h->cap) {
+    double *nx[3] = {nullptr, nullptr, nullptr}, *nvv[3] = {nullptr, nullptr, nullptr};
+    int* ncell = nullptr;
     for (int k = 0; k < h->dim; ++k) {
-      HIPCHK(hipMemcpyAsync(nx[k], h->px[k], keep * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-      HIPCHK(hipMemcpyAsync(nvv[k], h->pv[k], keep * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      CHK(dalloc(&nx[k], ncap));
+      CHK(dalloc(&nvv[k], ncap));
     }
-    HIPCHK(hipMemcpyAsync(ncell, h->pcell, keep * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
+    CHK(dalloc(&ncell, ncap));
+    if (keep > 0) {
+      for (int k = 0; k < h->dim; ++k) {
+        HIPCHK(hipMemcpyAsync(nx[k], h->px[k], keep * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(nvv[k], h->pv[k], keep * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      }
+      HIPCHK(hipMemcpyAsync(ncell, h->pcell, keep * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < 3; ++k) {
+      dfree(h->px[k]);
+      dfree(h->pv[k]);
+    }
+    dfree(h->pcell);
+    for (int k = 0; k < h->dim; ++k) {
+      h->px[k] = nx[k];
+      h->pv[k] = nvv[k];
+    }
+    h->pcell = ncell;
+    h->cap = ncap;
   }
+  if (tsize != h->tsize) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < 3; ++k) {
+      dfree(h->tx[k]);
+      dfree(h->tv[k]);
+    }
+    dfree(h->tcell);
+    for (int k = 0; k < h->dim; ++k) {
+      CHK(dalloc(&h->tx[k], tsize));
+      CHK(dalloc(&h->tv[k], tsize));
+    }
+    CHK(dalloc(&h->tcell, tsize));
+    h->tsize = tsize;
+    h->capc = capc;
+  }
+  return PFEM2GPU_OK;
+}
+
+// bucket capacity for about `n` particles: 2.5x the mean chunk population
+int capc_for(const pfem2gpu_handle* h, long n) {
+  long c = (5 * n) / (2 * (long)h->nchunks) + 64;
+  c = std::max(c, 256L);
+  c = (c + 63) / 64 * 64;
+  return (int)std::min(c, 1L << 20);
+}
+
+// Synchronises and reports device-side failures (a look-back wait that
+// timed out would have produced a wrong layout).
+int check_device(pfem2gpu_handle* h) {
+  HIPCHK(hipMemcpyAsync(h->h_pinned + 1, h->d_err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
-  free_particles(h);
-  for (int k = 0; k < h->dim; ++k) {
-    h->px[k] = nx[k];
-    h->pv[k] = nvv[k];
-    CHK(dalloc(&h->tx[k], ncap));
-    CHK(dalloc(&h->tv[k], ncap));
-  }
-  h->pcell = ncell;
-  CHK(dalloc(&h->tcell, ncap));
-  CHK(dalloc(&h->tpart, ncap));
-  CHK(dalloc(&h->tkey, ncap));
-  h->cap = ncap;
+  if (h->h_pinned[1] != 0)
+    return fail(PFEM2GPU_ERR_HIP, "redistribution look-back wait timed out; particle layout is invalid");
   return PFEM2GPU_OK;
 }
 
 int read_count(pfem2gpu_handle* h, long* n) {
   HIPCHK(hipMemcpyAsync(h->h_pinned, h->cell_start + h->nc, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  CHK(check_device(h));
   *n = *h->h_pinned;
   h->n_ub = *n;
   return PFEM2GPU_OK;
 }
 
-// Makes room for `extra` particles beyond the current count.
-int reserve(pfem2gpu_handle* h, long extra) {
-  if (h->n_ub + extra <= h->cap) return PFEM2GPU_OK;
+// Makes room for `extra` particles beyond the current count (particle
+// arrays) and for re-binning the current plus `rebin_extra` particles
+// (staging buckets). Reads the count back only when the host-side bound says
+// the capacity might not suffice.
+int reserve(pfem2gpu_handle* h, long extra, long rebin_extra) {
+  if (h->n_ub + extra <= h->cap && capc_for(h, h->n_ub + rebin_extra) <= h->capc) return PFEM2GPU_OK;
   long n = 0;
   CHK(read_count(h, &n));
-  if (n + extra <= h->cap) return PFEM2GPU_OK;
-  return grow(h, 2 * (n + extra), n);
+  const int capc = capc_for(h, n + rebin_extra);
+  if (n + extra <= h->cap && capc <= h->capc) return PFEM2GPU_OK;
+  return grow(h, n + extra <= h->cap ? h->cap : 2 * (n + extra), n, capc);
 }
 
 int scan(pfem2gpu_handle* h, const int* in, int n, int* out) {
@@ -330,44 +382,52 @@ int flush_correction(pfem2gpu_handle* h) {
   LAUNCH(h, "k_correct", k_correct<D>, grid_for(h->n_ub), 0, dmesh(h), h->cell_start + h->nc, dparts(h),
          dvfield(h));
   h->pending_corr = false;
+  h->contrib_valid = false;
   return PFEM2GPU_OK;
 }
 
 int flush(pfem2gpu_handle* h) { return h->dim == 2 ? flush_correction<2>(h) : flush_correction<3>(h); }
 
-// count -> scan -> scatter -> settle(count) -> scan -> settle(write). With
-// advect = 0 this re-bins the first n_cur + n_add particles without moving
-// them and without the per-part cap or refills.
+// move (or re-bin) into chunk buckets -> bin the spill -> settle(count) ->
+// scan -> settle(write). With advect = 0 this re-bins the first n_cur + n_add
+// particles without moving them and without the per-part cap or refills.
 template <int D>
 int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
   const DMesh m = dmesh(h);
   const DParts P = dparts(h);
   const DTmp T = dtmp(h);
-  const long nmax = h->n_ub + n_add;
-  const int g = grid_for(nmax);
+  const long nmax_p = h->n_ub + n_add;
+  const int g = grid_for((nmax_p + kMoveChunk / 64 - 1) / (kMoveChunk / 64));  // one wavefront per kMoveChunk
   const int* nptr = h->cell_start + h->nc;
-  int* cnt = h->count;
-  int* fill = h->count + (h->nc + 1);
+  const int nch = h->nchunks;
+  int* bcount = h->count;
+  int* scount = h->count + nch;
+  int* sfill = h->count + 2 * nch;
+  int* spill_n = h->count + 3 * nch;
+  const long spill_base = (long)nch * h->capc;
+  const long spill2_base = spill_base + h->cap;
   const int use_dv = (advect && h->pending_corr) ? 1 : 0;
   const DField dv = dvfield(h);
   if (!advect) CHK(flush_correction<D>(h));
-  LAUNCH(h, "k_fill", k_fill_int, grid_for(2L * (h->nc + 1)), 0, h->count, 2L * (h->nc + 1), 0);
-  LAUNCH(h, advect ? "k_move_count" : "k_rebin_count", k_move<D>, g, 0, 0, advect, m, nptr, n_add, P, Vf, dv, 0,
-         cnt, h->tstart, fill, T);
-  CHK(scan(h, cnt, h->nc, h->tstart));
-  LAUNCH(h, advect ? "k_move_scatter" : "k_rebin_scatter", k_move<D>, g, 0, 1, advect, m, nptr, n_add, P, Vf, dv,
-         use_dv, cnt, h->tstart, fill, T);
+  LAUNCH(h, "k_fill", k_fill_int, grid_for(3L * nch + 2), 0, h->count, 3L * nch + 2, 0);
+  LAUNCH(h, "k_fill", k_fill_int, grid_for(2L * nch), 0, reinterpret_cast<int*>(h->lb), 2L * nch, 0);
+  if (advect)
+    LAUNCH(h, "k_cellfield", k_cellfield<D>, grid_for((long)h->nc * h->nvc), 0, h->nc, h->cd, Vf, h->Vc);
+  LAUNCH(h, advect ? "k_move" : "k_rebin", k_move<D>, g, 0, advect, m, h->Vc, nptr, n_add, P, dv, use_dv, h->cshift,
+         h->capc, bcount, spill_base, spill_n, T);
   if (use_dv) h->pending_corr = false;
-  const int nchunks = (h->nc + h->cpw - 1) / h->cpw;
-  const int gs = std::min(nchunks, 16384);
+  LAUNCH(h, "k_spill", k_spill_count, grid_for(nmax_p), 0, spill_n, spill_base, h->tcell, h->cshift, scount);
+  CHK(scan(h, scount, nch, h->sstart));
+  LAUNCH(h, "k_spill", k_spill_scatter<D>, grid_for(nmax_p), 0, spill_n, spill_base, spill2_base, h->cshift,
+         h->sstart, sfill, T);
+  const int gs = std::min(nch, 2048);
   const int cap_on = advect, refill_on = advect;
-  LAUNCH(h, "k_settle_count", k_settle<D>, gs, h->settle_smem, 0, cap_on, refill_on, m, h->nc, h->cpw, h->pmax,
-         h->tstart, T, h->fcount, h->fstart, P, Vf);
-  CHK(scan(h, h->fcount, h->nc, h->fstart));
-  LAUNCH(h, "k_settle_write", k_settle<D>, gs, h->settle_smem, 1, cap_on, refill_on, m, h->nc, h->cpw, h->pmax,
-         h->tstart, T, h->fcount, h->fstart, P, Vf);
+  LAUNCH(h, "k_settle", k_settle<D>, gs, h->settle_smem, cap_on, refill_on, m, h->nc, h->cshift, h->pshift, h->nmax,
+         h->nslots, h->capc, bcount, h->sstart, spill2_base, T, h->count + 3 * nch + 1, h->lb, h->d_err, h->fstart, P,
+         Vf, h->contrib, h->dirty);
   std::swap(h->cell_start, h->fstart);
-  h->n_ub = nmax + (advect ? h->refill_max : 0);
+  h->n_ub = nmax_p + (advect ? h->refill_max : 0);
+  h->contrib_valid = true;
   return PFEM2GPU_OK;
 }
 
@@ -462,15 +522,34 @@ int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, vo
     p.Q[k] = k < dim ? p.q[k] + 1 : 1;
     p.h[k] = 1.0 / (double)(unsigned)p.q[k];
   }
+  p.qpow2 = 1;
+  for (int k = 0; k < 3; ++k) p.qf[k] = (double)p.q[k];
   for (int k = 0; k < dim; ++k) {
     p.P *= p.Q[k];
     p.Pvalid *= p.q[k];
+    if (p.q[k] & (p.q[k] - 1)) p.qpow2 = 0;
   }
   h->refill_max = h->nc * p.Pvalid;
-  h->pmax = p.P;
-  h->cpw = 64;
-  while (h->cpw > 1 && 2L * h->cpw * h->pmax * (long)sizeof(int) > 48 * 1024) h->cpw /= 2;
-  h->settle_smem = 2 * h->cpw * h->pmax * (int)sizeof(int);
+  h->pshift = 0;
+  while ((1 << h->pshift) < p.P) ++h->pshift;
+  h->pmax = 1 << h->pshift;
+  // settle stage: 16 cells per workgroup (one chunk), as many staged
+  // particles as fit in 64 KiB of LDS beside one refill slot per part
+  h->cshift = 4;
+  h->cpw = 1 << h->cshift;
+  h->nchunks = (h->nc + h->cpw - 1) >> h->cshift;
+  h->nmax = 64;
+  for (int nm = 64; nm <= 4096; nm += 64) {
+    const int slots = nm + h->cpw * p.Pvalid;
+    if (settle_layout(dim, h->cpw, h->pmax, nm, slots).bytes > 64 * 1024) break;
+    h->nmax = nm;
+  }
+  h->nslots = h->nmax + h->cpw * p.Pvalid;
+  h->settle_smem = settle_layout(dim, h->cpw, h->pmax, h->nmax, h->nslots).bytes;
+  if (h->settle_smem > 64 * 1024) {
+    delete h;
+    return fail(PFEM2GPU_ERR_ARG, "particles_per_cell too large for the redistribution stage");
+  }
 
   int rc;
   // mesh upload
@@ -504,6 +583,45 @@ int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, vo
     }
     if (hipMemcpy(h->box, box.data(), h->nc, hipMemcpyHostToDevice)) return bail(fail(PFEM2GPU_ERR_HIP, "box"));
   }
+  // box corners per cell (vertex 0 and vertex 2^dim-1) and the correctly
+  // rounded reciprocal of each edge length; NaN in X_0[0] marks other cells
+  {
+    std::vector<double> geom((size_t)h->nc * 3 * dim);
+    std::vector<unsigned char> boxh(h->nc);
+    if (hipMemcpy(boxh.data(), h->box, h->nc, hipMemcpyDeviceToHost)) return bail(fail(PFEM2GPU_ERR_HIP, "box"));
+    const double* V = mesh->vertices;
+    for (int c = 0; c < h->nc; ++c) {
+      const int32_t* cvr = mesh->cell_vertices + (size_t)c * nvc;
+      double* g = &geom[(size_t)c * 3 * dim];
+      for (int k = 0; k < dim; ++k) {
+        g[k] = V[(size_t)k * h->nv + cvr[0]];
+        g[dim + k] = V[(size_t)k * h->nv + cvr[nvc - 1]];
+        g[2 * dim + k] = 1.0 / (g[dim + k] - g[k]);
+      }
+      if (!boxh[c]) g[0] = std::nan("");
+    }
+    if ((rc = dalloc(&h->geom, geom.size())) || (rc = dalloc(&h->Vc, (size_t)h->nc * nvc * dim))) return bail(rc);
+    if (hipMemcpy(h->geom, geom.data(), geom.size() * 8, hipMemcpyHostToDevice))
+      return bail(fail(PFEM2GPU_ERR_HIP, "upload geometry"));
+  }
+  // part decode table: refillable part -> INT_MIN | pi_0 | pi_1 << 8 | pi_2 << 16
+  {
+    std::vector<int> tab(h->pmax, 0);
+    for (int pp = 0; pp < p.P; ++pp) {
+      int rem = pp, code = 0;
+      bool valid = true;
+      for (int d = dim - 1; d >= 0; --d) {
+        const int pk = rem % p.Q[d];
+        rem /= p.Q[d];
+        valid = valid && pk < p.q[d];
+        code |= pk << (8 * d);
+      }
+      tab[pp] = valid ? (int)(0x80000000u | (unsigned)code) : 0;
+    }
+    if ((rc = dalloc(&h->ptab, tab.size()))) return bail(rc);
+    if (hipMemcpy(h->ptab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice))
+      return bail(fail(PFEM2GPU_ERR_HIP, "upload part table"));
+  }
   // dof -> (cell*nvc + i), ascending cell
   {
     std::vector<int> off(h->ndofs + 1, 0), lst((size_t)h->nc * nvc);
@@ -522,9 +640,11 @@ int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, vo
   // per-cell work arrays
   h->scan_nb = (h->nc + kScanChunk - 1) / kScanChunk;
   if ((rc = dalloc(&h->cell_start, h->nc + 1)) || (rc = dalloc(&h->fstart, h->nc + 1)) ||
-      (rc = dalloc(&h->count, 2 * (size_t)(h->nc + 1))) || (rc = dalloc(&h->tstart, h->nc + 1)) ||
-      (rc = dalloc(&h->fcount, h->nc + 1)) || (rc = dalloc(&h->scan_blk, h->scan_nb + 1)) ||
-      (rc = dalloc(&h->contrib, (size_t)h->nc * nvc * (dim + 1))) || (rc = dalloc(&h->d_rejected, 1)))
+      (rc = dalloc(&h->count, 3 * (size_t)h->nchunks + 2)) || (rc = dalloc(&h->sstart, h->nchunks + 1)) ||
+      (rc = dalloc(&h->lb, h->nchunks)) || (rc = dalloc(&h->d_err, 1)) ||
+      (rc = dalloc(&h->scan_blk, h->scan_nb + 1)) ||
+      (rc = dalloc(&h->contrib, (size_t)h->nc * nvc * (dim + 1))) || (rc = dalloc(&h->d_rejected, 1)) ||
+      (rc = dalloc(&h->dirty, h->nchunks)))
     return bail(rc);
   for (int k = 0; k < dim; ++k)
     if ((rc = dalloc(&h->dV[k], h->ndofs)) || (rc = dalloc(&h->stageF[0][k], h->ndofs)) ||
@@ -532,10 +652,11 @@ int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, vo
       return bail(rc);
   if (hipHostMalloc(reinterpret_cast<void**>(&h->h_pinned), sizeof(int) * 4, 0) != hipSuccess)
     return bail(fail(PFEM2GPU_ERR_MEMORY, "hipHostMalloc"));
-  if (hipMemsetAsync(h->cell_start, 0, (h->nc + 1) * sizeof(int), h->stream) != hipSuccess ||
+  if (hipMemsetAsync(h->d_err, 0, sizeof(int), h->stream) != hipSuccess ||
+      hipMemsetAsync(h->cell_start, 0, (h->nc + 1) * sizeof(int), h->stream) != hipSuccess ||
       hipStreamSynchronize(h->stream) != hipSuccess)
     return bail(fail(PFEM2GPU_ERR_HIP, "init"));
-  if ((rc = grow(h, 1024, 0))) return bail(rc);
+  if ((rc = grow(h, 1024, 0, 256))) return bail(rc);
   *out = h;
   return PFEM2GPU_OK;
 }
@@ -559,14 +680,20 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
   dfree(h->d2c_off);
   dfree(h->d2c);
   dfree(h->prm);
+  dfree(h->geom);
+  dfree(h->cvo);
+  dfree(h->ptab);
+  dfree(h->Vc);
   dfree(h->cell_start);
   dfree(h->fstart);
   dfree(h->count);
-  dfree(h->tstart);
-  dfree(h->fcount);
+  dfree(h->sstart);
+  dfree(h->lb);
+  dfree(h->d_err);
   dfree(h->scan_blk);
   dfree(h->contrib);
   dfree(h->d_rejected);
+  dfree(h->dirty);
   if (h->h_pinned) (void)hipHostFree(h->h_pinned);
   for (auto& t : h->pending) {
     (void)hipEventDestroy(t.a);
@@ -619,6 +746,15 @@ int pfem2gpu_initialize_maps(pfem2gpu_handle* h) {
       const double inv_m = 1.0 / std::sqrt(m2);
       for (int k = 0; k < dim; ++k) dirs[(size_t)e * dim + k] = d[k] * inv_m;
     }
+  std::vector<int2> cvo((size_t)h->nc * nvc);
+  for (int c = 0; c < h->nc; ++c)
+    for (int i = 0; i < nvc; ++i) {
+      const int v = h->hcv[(size_t)c * nvc + i];
+      cvo[(size_t)c * nvc + i] = make_int2(off2[v], off2[v + 1] - off2[v]);
+    }
+  dfree(h->cvo);
+  CHK(dalloc(&h->cvo, cvo.size()));
+  HIPCHK(hipMemcpy(h->cvo, cvo.data(), cvo.size() * sizeof(int2), hipMemcpyHostToDevice));
   dfree(h->v2c_off);
   dfree(h->v2c);
   dfree(h->v2c_dir);
@@ -641,7 +777,7 @@ int pfem2gpu_insert_particles(pfem2gpu_handle* h, int64_t n, const double* x, co
   if (n > 0x7fffffffL) return fail(PFEM2GPU_ERR_ARG, "too many particles");
   if (where != PFEM2GPU_HOST && where != PFEM2GPU_DEVICE) return fail(PFEM2GPU_ERR_ARG, "bad where");
   CHK(flush(h));
-  CHK(reserve(h, n));
+  CHK(reserve(h, n, n));
   const int dim = h->dim;
   const double* sx[3] = {nullptr, nullptr, nullptr};
   const double* sv[3] = {nullptr, nullptr, nullptr};
@@ -677,7 +813,7 @@ int pfem2gpu_seed_particles(pfem2gpu_handle* h, const double* const* V, int wher
   CHK(flush(h));
   const long n = (long)h->nc * h->prm_host.Pvalid;
   if (n > 0x7fffffffL) return fail(PFEM2GPU_ERR_ARG, "too many particles");
-  CHK(reserve(h, n));
+  CHK(reserve(h, n, n));
   const DMesh m = dmesh(h);
   if (h->dim == 2)
     LAUNCH(h, "k_seed", k_seed<2>, grid_for(n), 0, m, h->nc, h->prm_host.Pvalid, F, h->tx[0], h->tx[1], h->tx[2],
@@ -694,6 +830,7 @@ int pfem2gpu_clear_particles(pfem2gpu_handle* h) {
   if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
   HIPCHK(hipMemsetAsync(h->cell_start, 0, (h->nc + 1) * sizeof(int), h->stream));
   h->pending_corr = false;
+  h->contrib_valid = false;
   h->n_ub = 0;
   return PFEM2GPU_OK;
 }
@@ -703,6 +840,7 @@ int pfem2gpu_set_velocity_from_field(pfem2gpu_handle* h, const double* const* V,
   DField F;
   CHK(field_in(h, V, where, 0, &F));
   h->pending_corr = false;  // overwritten anyway
+  h->contrib_valid = false;
   if (h->dim == 2)
     LAUNCH(h, "k_setvel", k_setvel<2>, grid_for(h->n_ub), 0, dmesh(h), h->cell_start + h->nc, dparts(h), F);
   else
@@ -730,7 +868,7 @@ int pfem2gpu_move_particles(pfem2gpu_handle* h, const double* const* V, int wher
   if (!h->maps_ready) return fail(PFEM2GPU_ERR_STATE, "initialize_maps() must be called before move_particles()");
   DField F;
   CHK(field_in(h, V, where, 0, &F));
-  CHK(reserve(h, h->refill_max));
+  CHK(reserve(h, h->refill_max, 0));
   return do_rebin(h, 1, 0, F);
 }
 
@@ -742,15 +880,26 @@ int pfem2gpu_project_particle_fields(pfem2gpu_handle* h, double* const* V_out, i
   const long ne = (long)h->nc * h->nvc;
   double* out[3] = {nullptr, nullptr, nullptr};
   for (int k = 0; k < h->dim; ++k) out[k] = (where == PFEM2GPU_DEVICE) ? V_out[k] : h->stageF[1][k];
+  const int nchunks = h->nchunks;
   if (h->dim == 2) {
-    LAUNCH(h, "k_contrib", k_contrib<2>, grid_for(ne), 0, dmesh(h), h->nc, h->cell_start, dparts(h), h->contrib);
+    if (h->contrib_valid)
+      LAUNCH(h, "k_contrib_dirty", k_contrib_dirty<2>, std::min(nchunks, 16384), 0, dmesh(h), h->nc, h->cpw,
+             h->dirty, h->cell_start, dparts(h), h->contrib);
+    else
+      LAUNCH(h, "k_contrib", k_contrib<2>, grid_for(ne), 0, dmesh(h), h->nc, h->cell_start, dparts(h), h->contrib);
     LAUNCH(h, "k_node", k_node<2>, grid_for(h->ndofs), 0, h->ndofs, h->d2c_off, h->d2c, h->contrib, out[0], out[1],
            out[2]);
   } else {
-    LAUNCH(h, "k_contrib", k_contrib<3>, grid_for(ne), 0, dmesh(h), h->nc, h->cell_start, dparts(h), h->contrib);
+    if (h->contrib_valid)
+      LAUNCH(h, "k_contrib_dirty", k_contrib_dirty<3>, std::min(nchunks, 16384), 0, dmesh(h), h->nc, h->cpw,
+             h->dirty, h->cell_start, dparts(h), h->contrib);
+    else
+      LAUNCH(h, "k_contrib", k_contrib<3>, grid_for(ne), 0, dmesh(h), h->nc, h->cell_start, dparts(h), h->contrib);
     LAUNCH(h, "k_node", k_node<3>, grid_for(h->ndofs), 0, h->ndofs, h->d2c_off, h->d2c, h->contrib, out[0], out[1],
            out[2]);
   }
+  h->contrib_valid = true;
+  (void)ne;
   if (where == PFEM2GPU_HOST) {
     for (int k = 0; k < h->dim; ++k)
       HIPCHK(hipMemcpyAsync(V_out[k], out[k], h->ndofs * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -811,8 +960,7 @@ void* pfem2gpu_stream(pfem2gpu_handle* h) { return h ? (void*)h->stream : nullpt
 
 int pfem2gpu_synchronize(pfem2gpu_handle* h) {
   if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
-  HIPCHK(hipStreamSynchronize(h->stream));
-  return PFEM2GPU_OK;
+  return check_device(h);
 }
 
 int pfem2gpu_set_timing(pfem2gpu_handle* h, int enable) {

@@ -28,6 +28,8 @@ struct Prm {
   int Q[3];        // q_k + 1: part buckets per direction (xi_k == 1 gives index q_k)
   int P;           // prod Q_k: part buckets per cell
   int Pvalid;      // prod q_k: refillable parts per cell
+  int qpow2;       // every q_k is a power of two: xi / h_k == xi * q_k exactly
+  double qf[3];    // q_k as double
 };
 
 struct DMesh {
@@ -40,6 +42,9 @@ struct DMesh {
   const double* v2c_dir;         // [nnz][D] unit directions vertex -> cell centre
   const int* cell_inst;          // [nc] instance of each cell, or null (single)
   const Prm* prm;                // per-instance parameters
+  const double* geom;            // [nc][3D]: X_0, X_{2^D-1}, 1/(X_{2^D-1} - X_0) of box cells; X_0[0] = NaN otherwise
+  const int2* cvo;               // [nc][2^D]: (v2c offset, count) of each cell vertex
+  const int* ptab;               // [ninst][pmax]: refillable part -> INT_MIN | pi_0 | pi_1<<8 | pi_2<<16, else 0
 };
 
 struct DField {
@@ -56,8 +61,6 @@ struct DTmp {
   double* x[3];
   double* v[3];
   int* cell;
-  unsigned short* part;
-  unsigned long long* key;
 };
 
 __device__ __forceinline__ int inst_of(const DMesh& m, int c) { return m.cell_inst ? m.cell_inst[c] : 0; }
@@ -152,6 +155,12 @@ __device__ __forceinline__ double solve(const double (*A)[3], const double* r, d
   }
 }
 
+// Newton inverse of a general (non-box) Q1 cell from the affine guess of
+// vertex 0 and its edges. Kept out of line: the hot loop only runs the box
+// closed form, and inlining this would raise register pressure everywhere.
+template <int D>
+__device__ bool inv_map_newton(const DMesh& m, const int* vid, const double* x, double* xi);
+
 // Inverse Q1 map (MappingQ1::transform_real_to_unit_cell restated): closed
 // form on axis-aligned boxes, Newton from the affine guess otherwise. Returns
 // false when the transformation fails.
@@ -166,6 +175,11 @@ __device__ bool inv_map(const DMesh& m, int c, const int* vid, const double* x, 
     }
     return true;
   }
+  return inv_map_newton<D>(m, vid, x, xi);
+}
+
+template <int D>
+__device__ bool inv_map_newton(const DMesh& m, const int* vid, const double* x, double* xi) {
   double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   double r[3] = {0, 0, 0};
   double X0[D];
@@ -221,6 +235,22 @@ __device__ __forceinline__ bool inside(const double* xi) {
   return in;
 }
 
+// Correctly rounded d / h from the correctly rounded reciprocal r = 1/h:
+// q0 = d*r is refined by one FMA residual step (Markstein), which returns
+// RN(d/h) for normal operands. Zero, tiny, non-finite or non-normal inputs
+// take the IEEE division, so the result always equals d / h bitwise.
+__device__ __forceinline__ double div_by(double d, double h, double r) {
+  const double q0 = d * r;
+  if (!(fabs(d) >= 0x1p-960) || !(fabs(q0) <= 0x1p+960)) return d / h;
+  const double rem = __builtin_fma(-q0, h, d);
+  return __builtin_fma(rem, r, q0);
+}
+
+// part index (int)(xi_k / h_k) with h_k = 1.0/q_k (src/pfem2particlehandler.cpp:370)
+__device__ __forceinline__ int part_index(const Prm& p, int k, double xi) {
+  return p.qpow2 ? (int)(xi * p.qf[k]) : (int)(xi / p.h[k]);
+}
+
 // u_k = sum_i N_i(xi) F_k[dof_i], vertex-outer, component-inner, from 0.0.
 template <int D>
 __device__ __forceinline__ void gather(const DField& F, const int* dof, const double* xi, double* u) {
@@ -242,7 +272,7 @@ __device__ __forceinline__ int part_lin(const Prm& p, const double* xi, bool* va
   bool ok = true;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    const int pk = (int)(xi[k] / p.h[k]);
+    const int pk = part_index(p, k, xi[k]);
     ok = ok && (pk >= 0) && (pk < p.q[k]);
     lin = lin * p.Q[k] + pk;
   }
@@ -254,21 +284,26 @@ __device__ __forceinline__ int part_lin(const Prm& p, const double* xi, bool* va
 // snapped to the exact sub-cell centre when x is bitwise the forward map of
 // that centre (particles placed by seeding or refill keep the exact centre,
 // src/pfem2particlehandler.cpp:88-91,390-394).
+//
+// The forward map is only evaluated when xi is within 1e-9 of the centre:
+// x == F(centre) bitwise implies |F^-1(x) - centre| is a few ulps, so a
+// larger distance already rules the snap out.
 template <int D>
-__device__ __forceinline__ void recover_xi(const DMesh& m, const Prm& p, int c, const int* vid, const double* x,
-                                           double* xi) {
-  inv_map<D>(m, c, vid, x, xi);
+__device__ __forceinline__ void snap_centre(const DMesh& m, const Prm& p, const int* vid, const double* x,
+                                            double* xi) {
   int pi[D];
   bool valid = true;
+  double xic[D];
+  bool near = true;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    pi[k] = (int)(xi[k] / p.h[k]);
+    pi[k] = part_index(p, k, xi[k]);
     valid = valid && (pi[k] >= 0) && (pi[k] < p.q[k]);
+    xic[k] = ((double)pi[k] + 1.0 / 2) * p.h[k];
+    near = near && (fabs(xi[k] - xic[k]) <= 1e-9);
   }
-  if (!valid) return;
-  double xic[D], xc[D];
-#pragma unroll
-  for (int k = 0; k < D; ++k) xic[k] = ((double)pi[k] + 1.0 / 2) * p.h[k];
+  if (!valid || !near) return;
+  double xc[D];
   fwd<D>(m, vid, xic, xc);
   bool eq = true;
 #pragma unroll
@@ -277,6 +312,154 @@ __device__ __forceinline__ void recover_xi(const DMesh& m, const Prm& p, int c, 
 #pragma unroll
     for (int k = 0; k < D; ++k) xi[k] = xic[k];
   }
+}
+
+template <int D>
+__device__ __forceinline__ void recover_xi(const DMesh& m, const Prm& p, int c, const int* vid, const double* x,
+                                           double* xi) {
+  inv_map<D>(m, c, vid, x, xi);
+  snap_centre<D>(m, p, vid, x, xi);
+}
+
+// Per-lane copy of what the substep loop reads about the particle's current
+// cell: the box corners (vertex 0 and vertex 2^D-1; every vertex coordinate
+// of a box is one of them) and, in 2D, the advecting field at the cell's four
+// DoFs. It is reloaded only when the particle changes cell, so the 2D
+// Euler/locate loop runs from registers. In 3D the 24 field values would cost
+// more occupancy than the reloads they save, so they are read from the
+// cell-major field table (L1/L2) each substep.
+template <int D>
+struct CellCache {
+  int c;
+  bool box;
+  double X0[D];
+  double X1[D];
+  double RH[D];  // correctly rounded 1 / (X1 - X0)
+  double Vn[D == 2 ? 4 : 1][D];
+};
+
+// Vc: advecting field in cell-major order, [nc][2^D][D] (rebuilt per move).
+template <int D>
+__device__ __forceinline__ void load_cell(const DMesh& m, const double* __restrict__ Vc, int c, CellCache<D>& cc,
+                                          bool with_field) {
+  cc.c = c;
+  const double* g = m.geom + (size_t)c * 3 * D;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    cc.X0[k] = g[k];
+    cc.X1[k] = g[D + k];
+    cc.RH[k] = g[2 * D + k];
+  }
+  cc.box = (cc.X0[0] == cc.X0[0]);
+  if constexpr (D == 2) {
+    if (!with_field) return;
+    const double2* f = reinterpret_cast<const double2*>(Vc + (size_t)c * 8);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double2 t = f[i];
+      cc.Vn[i][0] = t.x;
+      cc.Vn[i][1] = t.y;
+    }
+  }
+}
+
+template <int D>
+__device__ __forceinline__ bool inv_cached(const DMesh& m, const CellCache<D>& cc, const double* x, double* xi) {
+  if (cc.box) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) xi[k] = div_by(x[k] - cc.X0[k], cc.X1[k] - cc.X0[k], cc.RH[k]);
+    return true;
+  }
+  int vid[1 << D];
+  load_ids<D>(m.cv, cc.c, vid);
+  return inv_map_newton<D>(m, vid, x, xi);
+}
+
+// u_k = sum_i N_i(xi) V_k[dof_i] from the cached / cell-major field
+template <int D>
+__device__ __forceinline__ void gather_cached(const double* __restrict__ Vc, const CellCache<D>& cc, const double* xi,
+                                              double* u) {
+  double N[1 << D];
+  shape<D>(xi, N);
+#pragma unroll
+  for (int k = 0; k < D; ++k) u[k] = 0.0;
+  if constexpr (D == 2) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) u[k] = u[k] + N[i] * cc.Vn[i][k];
+  } else {
+    const double* f = Vc + (size_t)cc.c * (8 * D);
+#pragma unroll
+    for (int i = 0; i < (1 << D); ++i)
+#pragma unroll
+      for (int k = 0; k < D; ++k) u[k] = u[k] + N[i] * f[i * D + k];
+  }
+}
+
+// Forward map of a box cell from its corners (bitwise the generic forward map:
+// the same vertex coordinates in the same order).
+template <int D>
+__device__ __forceinline__ void fwd_box(const CellCache<D>& cc, const double* xi, double* x) {
+  double N[1 << D];
+  shape<D>(xi, N);
+#pragma unroll
+  for (int k = 0; k < D; ++k) x[k] = 0.0;
+#pragma unroll
+  for (int i = 0; i < (1 << D); ++i)
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = x[k] + N[i] * (((i >> k) & 1) ? cc.X1[k] : cc.X0[k]);
+}
+
+// snap_centre for a cached cell (forward map only when xi is near a centre;
+// box cells evaluate it from their corners without further loads)
+template <int D>
+__device__ __forceinline__ void snap_cached(const DMesh& m, const Prm& p, const CellCache<D>& cc, const double* x,
+                                            double* xi) {
+  int pi[D];
+  bool valid = true, near = true;
+  double xic[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    pi[k] = part_index(p, k, xi[k]);
+    valid = valid && (pi[k] >= 0) && (pi[k] < p.q[k]);
+    xic[k] = ((double)pi[k] + 1.0 / 2) * p.h[k];
+    near = near && (fabs(xi[k] - xic[k]) <= 1e-9);
+  }
+  if (!valid || !near) return;
+  double xc[D];
+  if (cc.box) {
+    fwd_box<D>(cc, xic, xc);
+  } else {
+    int vid[1 << D];
+    load_ids<D>(m.cv, cc.c, vid);
+    fwd<D>(m, vid, xic, xc);
+  }
+  bool eq = true;
+#pragma unroll
+  for (int k = 0; k < D; ++k) eq = eq && (xc[k] == x[k]);
+  if (eq) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) xi[k] = xic[k];
+  }
+}
+
+// Reference position of a located particle in cell c from the geometry table
+// (box cells: no vertex loads at all).
+template <int D>
+__device__ __forceinline__ void recover_xi_c(const DMesh& m, const Prm& p, int c, const double* x, double* xi) {
+  CellCache<D> cc;
+  cc.c = c;
+  const double* g = m.geom + (size_t)c * 3 * D;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    cc.X0[k] = g[k];
+    cc.X1[k] = g[D + k];
+    cc.RH[k] = g[2 * D + k];
+  }
+  cc.box = (cc.X0[0] == cc.X0[0]);
+  inv_cached<D>(m, cc, x, xi);
+  snap_cached<D>(m, p, cc, x, xi);
 }
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
@@ -305,19 +488,19 @@ __device__ __forceinline__ double dot_dir(const double* __restrict__ dir, const 
   return s;
 }
 
-// Locate after an Euler substep (src/pfem2particlehandler.cpp:420-492 with
-// src/pfem2particle.cpp:186-211). On entry (c, vid, xi) describe the cell and
-// reference position before the substep and u is the advecting velocity of
-// the substep. Returns false if the particle is to be deleted.
+// Neighbour walk after a particle left its cell (src/pfem2particlehandler.cpp:
+// 438-492 with src/pfem2particle.cpp:186-211): closest vertex of the old cell
+// along the substep velocity u, seen from the old position F(xi_old); the
+// cells around that vertex in descending alignment of their centre direction
+// with the vertex->particle direction (ties in ascending cell id); the first
+// one whose inverse map contains x wins. Returns the new cell (xi_new set) or
+// -1 when the particle is to be deleted. Out of line: it runs for the ~10% of
+// particles that cross a face in a substep.
 template <int D>
-__device__ bool locate(const DMesh& m, int& c, int* vid, double* xi, const double* x, const double* u) {
+__device__ int walk(const DMesh& m, int c, const int* vid, const double* xi, const double* x,
+                                 const double* u, double* xi_new) {
   constexpr int NV = 1 << D;
   double xn[D];
-  if (inv_map<D>(m, c, vid, x, xn) && inside<D>(xn)) {
-#pragma unroll
-    for (int k = 0; k < D; ++k) xi[k] = xn[k];
-    return true;
-  }
   // closest vertex of the current cell along the velocity, from the old position
   double xold[D];
   fwd<D>(m, vid, xi, xold);
@@ -394,15 +577,12 @@ __device__ bool locate(const DMesh& m, int& c, int* vid, double* xi, const doubl
       if (cc == c) continue;  // the current cell already failed the same test
       load_ids<D>(m.cv, cc, vid2);
       if (inv_map<D>(m, cc, vid2, x, xn) && inside<D>(xn)) {
-        c = cc;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) vid[i] = vid2[i];
-#pragma unroll
-        for (int k = 0; k < D; ++k) xi[k] = xn[k];
-        return true;
+        for (int k = 0; k < D; ++k) xi_new[k] = xn[k];
+        return cc;
       }
     }
-    return false;
+    return -1;
   }
   // more than MAXC cells at this vertex: same order, selected without storage
   const bool nan_mode = !(vtp[0] == vtp[0]);
@@ -432,15 +612,118 @@ __device__ bool locate(const DMesh& m, int& c, int* vid, double* xi, const doubl
     if (cc == c) continue;
     load_ids<D>(m.cv, cc, vid2);
     if (inv_map<D>(m, cc, vid2, x, xn) && inside<D>(xn)) {
-      c = cc;
 #pragma unroll
-      for (int i = 0; i < NV; ++i) vid[i] = vid2[i];
-#pragma unroll
-      for (int k = 0; k < D; ++k) xi[k] = xn[k];
-      return true;
+      for (int k = 0; k < D; ++k) xi_new[k] = xn[k];
+      return cc;
     }
   }
-  return false;
+  return -1;
+}
+
+// Neighbour walk from a box cell: the same steps and arithmetic as walk(),
+// with the current cell's vertex coordinates taken from its corners, the
+// vertex's candidate list from the per-cell table and box candidates tested
+// from the geometry table (one load each).
+template <int D>
+__device__ int walk_box(const DMesh& m, const CellCache<D>& cc, const double* xi, const double* x, const double* u,
+                        double* xi_new) {
+  constexpr int NV = 1 << D;
+  double xold[D];
+  fwd_box<D>(cc, xi, xold);
+  double n2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) n2 = n2 + u[k] * u[k];
+  const double inv_un = 1.0 / sqrt(n2);
+  double vn[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) vn[k] = u[k] * inv_un;
+  double best = 0.0;
+  int bestv = 0;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    double d[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) d[k] = (((v >> k) & 1) ? cc.X1[k] : cc.X0[k]) - xold[k];
+    double m2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) m2 = m2 + d[k] * d[k];
+    const double inv_m = 1.0 / sqrt(m2);
+    double a = vn[0] * (d[0] * inv_m);
+#pragma unroll
+    for (int k = 1; k < D; ++k) a = a + vn[k] * (d[k] * inv_m);
+    if (v == 0) {
+      best = a;
+    } else if (a > best) {
+      best = a;
+      bestv = v;
+    }
+  }
+  double vtp[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) vtp[k] = x[k] - (((bestv >> k) & 1) ? cc.X1[k] : cc.X0[k]);
+  double t2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) t2 = t2 + vtp[k] * vtp[k];
+  const double inv_t = 1.0 / sqrt(t2);
+#pragma unroll
+  for (int k = 0; k < D; ++k) vtp[k] = vtp[k] * inv_t;
+  const int2 oc = m.cvo[(size_t)cc.c * NV + bestv];
+  const int o0 = oc.x, nn = oc.y;
+  if (nn > 8) return -2;  // caller falls back to the generic walk
+  double dots[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    dots[j] = 0.0;
+    if (j < nn) dots[j] = dot_dir<D>(m.v2c_dir + (size_t)(o0 + j) * D, vtp);
+  }
+  unsigned used = 0;
+  for (int r = 0; r < nn; ++r) {
+    int bj = -1;
+    double bd = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (j < nn && !((used >> j) & 1u)) {
+        if (bj < 0 || dots[j] > bd) {
+          bj = j;
+          bd = dots[j];
+        }
+      }
+    }
+    used |= 1u << bj;
+    const int c2 = m.v2c[o0 + bj];
+    if (c2 == cc.c) continue;  // the current cell already failed the same test
+    const double* g = m.geom + (size_t)c2 * 3 * D;
+    double xn[D];
+    bool ok;
+    if (g[0] == g[0]) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) xn[k] = div_by(x[k] - g[k], g[D + k] - g[k], g[2 * D + k]);
+      ok = true;
+    } else {
+      int vid2[NV];
+      load_ids<D>(m.cv, c2, vid2);
+      ok = inv_map_newton<D>(m, vid2, x, xn);
+    }
+    if (ok && inside<D>(xn)) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) xi_new[k] = xn[k];
+      return c2;
+    }
+  }
+  return -1;
+}
+
+// Neighbour walk for the particle in cache cc (box fast path or generic).
+template <int D>
+__device__ __forceinline__ int walk_cached(const DMesh& m, const CellCache<D>& cc, const double* xi, const double* x,
+                                           const double* u, double* xi_new) {
+  if (cc.box) {
+    const int r = walk_box<D>(m, cc, xi, x, u, xi_new);
+    if (r != -2) return r;
+  }
+  int vid[1 << D];
+  load_ids<D>(m.cv, cc.c, vid);
+  return walk<D>(m, cc.c, vid, xi, x, u, xi_new);
 }
 
 }  // namespace pfem2gpu

@@ -68,6 +68,85 @@ def cpu_baseline(workload, budget_s=12.0):
                                                            len(cfg["cells"]), done, dt)}
 
 
+def batched_comparison(steps, warmup, n_inst=256, scale=1.0, log=None):
+    """256 independent 2D channel instances (32x16..512x256 Q1, Poisson(12)
+    particles per cell, Poiseuille amplitude U[0.5, 2]): one batch handle
+    (one launch per phase, hipGraph per step, autotuned launch shape) against
+    the same instances as plain handles launched one after another on one
+    stream. Particles are counted at the start of each step."""
+    import torch
+    from pfem2dealii_amd import BatchHandler, ParticleHandler, workloads as W
+    from pfem2dealii_amd.autotune import tune
+    inst = W.batched_instances(n_inst, seed=4, scale=scale)
+    b = BatchHandler([i["mesh"] for i in inst], [i["params"] for i in inst])
+    b.initialize_maps()
+    co = b.cell_offsets
+    x = np.concatenate([i["x"] for i in inst], axis=1)
+    cells = np.concatenate([i["cells"] + co[k] for k, i in enumerate(inst)]).astype(np.int32)
+    Vh = np.concatenate([i["V"] for i in inst], axis=1)
+    assert b.insert_particles(x, cells) == 0
+    del x, cells
+    V = torch.tensor(Vh, dtype=torch.float64, device="cuda")
+    b.set_velocity_from_field(V)
+    Vp = torch.empty_like(V)
+    n0 = b.count()
+    torch.cuda.synchronize()
+    bstep = lambda: b.step(V, Vp, use_graph=True)  # noqa: E731
+    bstep()
+    tuning, _ = tune(b, bstep, signature={"workload": "batched", "n": n_inst, "scale": scale}, log=log)
+    for _ in range(warmup):
+        bstep()
+    stream = torch.cuda.ExternalStream(b.stream())
+    counts = torch.zeros(steps, dtype=torch.int32, device="cuda")
+    b.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for s in range(steps):
+        b.count_async(counts.data_ptr() + 4 * s)
+        bstep()
+    e1.record(stream)
+    b.synchronize()
+    b_ms = e0.elapsed_time(e1)
+    b_ps = int(counts.sum().item())
+    b.close()
+    del V, Vp
+
+    pstream = torch.cuda.Stream()
+    hs, Vs, Vps = [], [], []
+    for i in inst:
+        h = ParticleHandler(i["mesh"], i["params"], stream=pstream.cuda_stream)
+        h.initialize_maps()
+        h.insert_particles(i["x"], i["cells"])
+        Vi = torch.tensor(i["V"], dtype=torch.float64, device="cuda")
+        h.set_velocity_from_field(Vi)
+        hs.append(h)
+        Vs.append(Vi)
+        Vps.append(torch.empty_like(Vi))
+    torch.cuda.synchronize()
+    for _ in range(1 + warmup):
+        for h, Vi, Vpi in zip(hs, Vs, Vps):
+            h.step(Vi, Vpi, use_graph=False)
+    pc = torch.zeros((steps, n_inst), dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    e0.record(pstream)
+    for s in range(steps):
+        for k, (h, Vi, Vpi) in enumerate(zip(hs, Vs, Vps)):
+            h.count_async(pc.data_ptr() + 4 * (s * n_inst + k))
+            h.step(Vi, Vpi, use_graph=False)
+    e1.record(pstream)
+    torch.cuda.synchronize()
+    p_ms = e0.elapsed_time(e1)
+    p_ps = int(pc.sum().item())
+    for h in hs:
+        h.close()
+    b_rate = b_ps / (b_ms * 1e-3)
+    p_rate = p_ps / (p_ms * 1e-3)
+    return {"instances": n_inst, "particles_at_start": n0, "steps": steps,
+            "batched_particle_steps_per_s": b_rate, "batched_ms_per_step": b_ms / steps,
+            "plain_particle_steps_per_s": p_rate, "plain_ms_per_step": p_ms / steps,
+            "speedup": b_rate / p_rate, "tuning": tuning}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +156,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3, help="steps re-run with per-kernel events")
+    ap.add_argument("--batched-steps", type=int, default=5, help="steps of the 256-instance comparison (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU design is documented, not built); "
@@ -108,6 +188,9 @@ def main():
         h.project_particle_fields(Vp)
         h.correct_particle_velocity(V, Vp)
 
+    step()
+    from pfem2dealii_amd.autotune import tune
+    tuning, _ = tune(h, step, signature={"workload": args.workload, "scale": args.scale})
     for _ in range(args.warmup):
         step()
     h.synchronize()
@@ -160,12 +243,14 @@ def main():
         "config": {"workload": args.workload if args.scale == 1.0 else "%s@%g" % (args.workload, args.scale),
                    "mesh": "x".join(str(a) for a in mesh["n"]), "particles_at_start": n0,
                    "n_sub": cfg["params"]["n_sub"], "dt": cfg["params"]["dt"], "parallelism": "1 GPU",
-                   "form": "plain"},
+                   "form": "plain", "tuning": tuning},
         "roofline": roofline,
         "kernels_ms_per_step": {k: round(v["ms_per_step"], 4) for k, v in sorted(kern.items())},
         "wall_s": round(wall, 3), "setup_s": round(setup_s, 1), "profiled_particles": pc,
     }
     h.close()
+    if args.batched_steps > 0:
+        out["batched"] = batched_comparison(args.batched_steps, 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload)
         out["cpu_baseline"]["host"] = {"nproc": os.cpu_count()}

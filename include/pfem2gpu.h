@@ -122,6 +122,38 @@ int pfem2gpu_download(pfem2gpu_handle* h, double* x, double* xi, double* v, int3
 /* Particles per cell ([n_cells], host). Synchronises. */
 int pfem2gpu_cell_counts(pfem2gpu_handle* h, int32_t* counts);
 
+/* ---- Batched form (no counterpart in the reference) ----
+ * Many independent flow instances in one handle: meshes[i] and params[i]
+ * describe instance i; their cells, vertices and DoFs are concatenated in
+ * instance order, so every later call works on the whole batch with one
+ * kernel launch per phase. Cell ids passed to insert and vector fields are
+ * batch-global (instance offset + local id); pfem2gpu_batch_layout returns
+ * the n_instances+1 cell and DoF offsets. Results of each instance are
+ * bitwise those of a plain handle over that instance alone. */
+int pfem2gpu_batch_create(int32_t n_instances, const pfem2gpu_mesh* meshes, const pfem2gpu_params* params,
+                          void* hip_stream, pfem2gpu_handle** out);
+int pfem2gpu_batch_layout(pfem2gpu_handle* h, int64_t* cell_offsets, int64_t* dof_offsets, int32_t* n_instances);
+
+/* One step of the time loop with the FEM solve replaced by a prescribed
+ * field, in the order move_particles(V) -> project_particle_fields(V_proj)
+ * -> correct_particle_velocity(V, V_proj) (src/pfem2solver.cpp:162-166
+ * with the FEM step standing between project and the next correct). V and
+ * V_proj are device arrays. With use_graph != 0 the launch sequence is
+ * captured once into a hipGraph and replayed on later calls with the same
+ * pointers (a first step with no pending correction runs eagerly). */
+int pfem2gpu_step(pfem2gpu_handle* h, const double* const* V, double* const* V_proj, int use_graph);
+
+/* Launch shape of the two main kernels (chosen by the autotuner in
+ * pfem2dealii_amd/autotune.py): move workgroup size, particles per
+ * wavefront run of the move work queue, settle workgroups. */
+typedef struct {
+  int32_t move_block;   /* 64, 128 or 256 */
+  int32_t move_run;     /* multiple of 64 in [64, 8192] */
+  int32_t settle_grid;  /* [1, 65536] */
+} pfem2gpu_tuning;
+int pfem2gpu_set_tuning(pfem2gpu_handle* h, const pfem2gpu_tuning* t);
+int pfem2gpu_get_tuning(pfem2gpu_handle* h, pfem2gpu_tuning* t);
+
 /* The handle's HIP stream (hipStream_t). */
 void* pfem2gpu_stream(pfem2gpu_handle* h);
 

@@ -41,6 +41,14 @@ class _Params(ctypes.Structure):
                 ("max_particles_per_cell_part", ctypes.c_uint32), ("particles_per_cell", ctypes.c_uint32 * 3)]
 
 
+class Tuning(ctypes.Structure):
+    """Launch shape of the move and settle kernels (pfem2gpu_tuning)."""
+    _fields_ = [("move_block", ctypes.c_int32), ("move_run", ctypes.c_int32), ("settle_grid", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {"move_block": self.move_block, "move_run": self.move_run, "settle_grid": self.settle_grid}
+
+
 EXPORTS = [
     "pfem2gpu_create", "pfem2gpu_destroy", "pfem2gpu_last_error", "pfem2gpu_initialize_maps",
     "pfem2gpu_seed_particles", "pfem2gpu_insert_particles", "pfem2gpu_clear_particles",
@@ -48,6 +56,7 @@ EXPORTS = [
     "pfem2gpu_project_particle_fields", "pfem2gpu_particle_count", "pfem2gpu_particle_count_async",
     "pfem2gpu_download", "pfem2gpu_cell_counts",
     "pfem2gpu_stream", "pfem2gpu_synchronize", "pfem2gpu_set_timing", "pfem2gpu_timings",
+    "pfem2gpu_batch_create", "pfem2gpu_batch_layout", "pfem2gpu_step", "pfem2gpu_set_tuning", "pfem2gpu_get_tuning",
 ]
 
 
@@ -81,6 +90,12 @@ def load_library(path=None):
     lib.pfem2gpu_set_timing.argtypes = [_vp, ctypes.c_int]
     lib.pfem2gpu_timings.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _i64p,
                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+    lib.pfem2gpu_batch_create.argtypes = [ctypes.c_int32, ctypes.POINTER(_Mesh), ctypes.POINTER(_Params), _vp,
+                                          ctypes.POINTER(_vp)]
+    lib.pfem2gpu_batch_layout.argtypes = [_vp, _i64p, _i64p, ctypes.POINTER(ctypes.c_int32)]
+    lib.pfem2gpu_step.argtypes = [_vp, _vp, _vp, ctypes.c_int]
+    lib.pfem2gpu_set_tuning.argtypes = [_vp, ctypes.POINTER(Tuning)]
+    lib.pfem2gpu_get_tuning.argtypes = [_vp, ctypes.POINTER(Tuning)]
     if path is None:
         _lib = lib
     return lib
@@ -125,6 +140,20 @@ class _FieldPtrs:
         self.arr = (_vp * 3)(*[base + 8 * n * k for k in range(dim)] + [None] * (3 - dim))
 
 
+def _pack_mesh(mesh, keep):
+    X = np.ascontiguousarray(mesh["X"], dtype=np.float64)
+    cv = np.ascontiguousarray(mesh["cell_vertices"], dtype=np.int32)
+    cd = np.ascontiguousarray(mesh.get("cell_dofs", cv), dtype=np.int32)
+    keep.extend([X, cv, cd])
+    return _Mesh(int(mesh["dim"]), X.shape[1], X.ctypes.data, cv.shape[0], cv.ctypes.data, cd.ctypes.data,
+                 int(mesh["n_dofs"]))
+
+
+def _pack_params(params):
+    q = list(params["particles_per_cell"]) + [1] * (3 - len(params["particles_per_cell"]))
+    return _Params(int(params["n_sub"]), float(params["dt"]), int(params["max_per_part"]), (ctypes.c_uint32 * 3)(*q))
+
+
 class ParticleHandler:
     """pfem2ParticleHandler<dim> on one MI355X (src/pfem2particlehandler.h:18-71)."""
 
@@ -133,13 +162,9 @@ class ParticleHandler:
         self.dim = int(mesh["dim"])
         self.n_dofs = int(mesh["n_dofs"])
         self.n_cells = int(np.asarray(mesh["cell_vertices"]).shape[0])
-        X = np.ascontiguousarray(mesh["X"], dtype=np.float64)
-        cv = np.ascontiguousarray(mesh["cell_vertices"], dtype=np.int32)
-        cd = np.ascontiguousarray(mesh.get("cell_dofs", cv), dtype=np.int32)
-        m = _Mesh(self.dim, X.shape[1], X.ctypes.data, cv.shape[0], cv.ctypes.data, cd.ctypes.data, self.n_dofs)
-        q = list(params["particles_per_cell"]) + [1] * (3 - len(params["particles_per_cell"]))
-        p = _Params(int(params["n_sub"]), float(params["dt"]), int(params["max_per_part"]),
-                    (ctypes.c_uint32 * 3)(*q))
+        keep = []
+        m = _pack_mesh(mesh, keep)
+        p = _pack_params(params)
         h = _vp()
         _check(self.lib, self.lib.pfem2gpu_create(ctypes.byref(m), ctypes.byref(p), stream, ctypes.byref(h)),
                "pfem2gpu_create")
@@ -255,6 +280,24 @@ class ParticleHandler:
     def stream(self):
         return self.lib.pfem2gpu_stream(self.h)
 
+    def step(self, V, V_proj, use_graph=True):
+        """move -> project(V_proj) -> correct(V, V_proj) with device arrays; a
+        hipGraph of the sequence is captured once and replayed."""
+        a = _FieldPtrs(V, self.dim, self.n_dofs)
+        b = _FieldPtrs(V_proj, self.dim, self.n_dofs, writable=True)
+        if a.where != DEVICE or b.where != DEVICE:
+            raise ValueError("step() needs device (torch CUDA) fields")
+        _check(self.lib, self.lib.pfem2gpu_step(self.h, a.arr, b.arr, 1 if use_graph else 0), "step")
+
+    def set_tuning(self, move_block=256, move_run=512, settle_grid=2048):
+        t = Tuning(int(move_block), int(move_run), int(settle_grid))
+        _check(self.lib, self.lib.pfem2gpu_set_tuning(self.h, ctypes.byref(t)), "set_tuning")
+
+    def get_tuning(self):
+        t = Tuning()
+        _check(self.lib, self.lib.pfem2gpu_get_tuning(self.h, ctypes.byref(t)), "get_tuning")
+        return t.as_dict()
+
     def synchronize(self):
         _check(self.lib, self.lib.pfem2gpu_synchronize(self.h), "synchronize")
 
@@ -270,3 +313,31 @@ class ParticleHandler:
         n = ctypes.c_int(0)
         _check(self.lib, self.lib.pfem2gpu_timings(self.h, cap, names, launches, ms, ctypes.byref(n)), "timings")
         return {names[i].decode(): (int(launches[i]), float(ms[i])) for i in range(min(n.value, cap))}
+
+
+class BatchHandler(ParticleHandler):
+    """Many independent instances in one handle (pfem2gpu_batch_create): one
+    kernel launch per phase for the whole batch. Cell ids and DoF-indexed
+    fields are batch-global; see cell_offsets / dof_offsets."""
+
+    def __init__(self, meshes, params, stream=None):
+        self.lib = load_library()
+        n = len(meshes)
+        if n == 0 or len(params) != n:
+            raise ValueError("need one params dict per mesh")
+        self.dim = int(meshes[0]["dim"])
+        keep = []
+        marr = (_Mesh * n)(*[_pack_mesh(m, keep) for m in meshes])
+        parr = (_Params * n)(*[_pack_params(p) for p in params])
+        h = _vp()
+        _check(self.lib, self.lib.pfem2gpu_batch_create(n, marr, parr, stream, ctypes.byref(h)), "batch_create")
+        self.h = h
+        co = np.zeros(n + 1, dtype=np.int64)
+        do = np.zeros(n + 1, dtype=np.int64)
+        ni = ctypes.c_int32(0)
+        _check(self.lib, self.lib.pfem2gpu_batch_layout(self.h, co.ctypes.data_as(_i64p), do.ctypes.data_as(_i64p),
+                                                        ctypes.byref(ni)), "batch_layout")
+        self.cell_offsets = co
+        self.dof_offsets = do
+        self.n_cells = int(co[-1])
+        self.n_dofs = int(do[-1])

@@ -1,0 +1,74 @@
+"""Autotuner for the launch shape of the particle step.
+
+Tunes the three knobs of pfem2gpu_set_tuning by coordinate descent on the
+live handle (the measured steps advance the simulation; the launch shape does
+not change results, only timing):
+  move_block  workgroup size of the move kernel (64, 128, 256)
+  move_run    consecutive particles per wavefront run of its work queue
+  settle_grid workgroups of the redistribution/projection kernel
+Each candidate is timed over `steps` steps with HIP events on the handle's
+stream after one untimed step. Results are cached per workload signature.
+"""
+import json
+import os
+
+MOVE_BLOCKS = (64, 128, 256)
+MOVE_RUNS = (128, 256, 512, 1024, 2048)
+SETTLE_GRIDS = (512, 1024, 2048, 4096, 8192)
+
+_cache = {}
+
+
+def _time_steps(handle, step, steps):
+    import torch
+    stream = torch.cuda.ExternalStream(handle.stream())
+    step()
+    handle.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        step()
+    b.record(stream)
+    handle.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def tune(handle, step, signature=None, steps=2, cache_path=None, log=None):
+    """Chooses and applies the fastest launch shape; returns (tuning, ms/step)."""
+    key = json.dumps(signature, sort_keys=True) if signature is not None else None
+    if key is not None:
+        if cache_path and not _cache and os.path.exists(cache_path):
+            try:
+                _cache.update(json.load(open(cache_path)))
+            except (OSError, ValueError):
+                pass
+        if key in _cache:
+            handle.set_tuning(**_cache[key]["tuning"])
+            return _cache[key]["tuning"], _cache[key]["ms"]
+    best = dict(handle.get_tuning())
+    handle.set_tuning(**best)
+    best_ms = _time_steps(handle, step, steps)
+    trials = [dict(best, move_block=b, move_run=r) for b in MOVE_BLOCKS for r in MOVE_RUNS]
+    for stage in ("move", "settle"):
+        if stage == "settle":
+            trials = [dict(best, settle_grid=g) for g in SETTLE_GRIDS]
+        for t in trials:
+            if t == best:
+                continue
+            handle.set_tuning(**t)
+            ms = _time_steps(handle, step, steps)
+            if log:
+                log("tune %s: %.3f ms/step" % (t, ms))
+            if ms < best_ms:
+                best, best_ms = dict(t), ms
+    handle.set_tuning(**best)
+    if key is not None:
+        _cache[key] = {"tuning": best, "ms": best_ms}
+        if cache_path:
+            try:
+                with open(cache_path, "w") as f:
+                    json.dump(_cache, f)
+            except OSError:
+                pass
+    return best, best_ms

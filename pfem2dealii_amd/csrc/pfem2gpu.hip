@@ -69,6 +69,11 @@ struct pfem2gpu_handle {
   int* d2c_off = nullptr;
   int* d2c = nullptr;
   Prm* prm = nullptr;
+  int* cell_inst = nullptr;          // [nc] instance of each cell (batches only)
+  int ninst = 1;
+  std::vector<Prm> prm_inst;         // per-instance parameters (host copy)
+  std::vector<long> inst_cell_off;   // [ninst+1]
+  std::vector<long> inst_dof_off;    // [ninst+1]
   double* geom = nullptr;  // [nc][2*dim] box corners (X_0[0] = NaN for other cells)
   int2* cvo = nullptr;     // [nc][nvc] (v2c offset, count) per cell vertex
   int* ptab = nullptr;     // [pmax] part decode table
@@ -86,8 +91,7 @@ struct pfem2gpu_handle {
   double* tx[3] = {nullptr, nullptr, nullptr};
   double* tv[3] = {nullptr, nullptr, nullptr};
   int* tcell = nullptr;
-  int* cell_start = nullptr;  // [nc+1]
-  int* fstart = nullptr;      // [nc+1] next layout (swapped with cell_start)
+  int* cell_start = nullptr;  // [nc+1], rewritten in place by the settle kernel
   int* count = nullptr;       // [3*nchunks+2]: bucket counts | spill counts | spill fill | spill total | ticket
   int* sstart = nullptr;      // [nchunks+1] spill offsets per chunk
   unsigned long long* lb = nullptr;  // [nchunks] look-back status words of the settle kernel
@@ -107,6 +111,17 @@ struct pfem2gpu_handle {
   int cpw = 16, pmax = 1, nmax = 512, nslots = 0, settle_smem = 0;  // cpw = 1 << cshift
   unsigned char* dirty = nullptr;  // per settle chunk: sums left to k_contrib_dirty
   bool contrib_valid = false;      // per (cell, vertex) sums match the particles
+
+  // tuning (set by pfem2gpu_set_tuning; chosen by the autotuner)
+  int move_block = 256;     // workgroup size of the move kernel
+  int move_run = 512;       // consecutive particles per wavefront run
+  int settle_grid = 2048;   // workgroups of the settle kernel
+  // graph-captured harness step
+  bool capturing = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  const double* graph_V[3] = {nullptr, nullptr, nullptr};
+  double* graph_Vp[3] = {nullptr, nullptr, nullptr};
 
   // timing
   bool timing = false;
@@ -204,7 +219,7 @@ DMesh dmesh(pfem2gpu_handle* h) {
   m.v2c_off = h->v2c_off;
   m.v2c = h->v2c;
   m.v2c_dir = h->v2c_dir;
-  m.cell_inst = nullptr;
+  m.cell_inst = h->cell_inst;
   m.prm = h->prm;
   m.geom = h->geom;
   m.cvo = h->cvo;
@@ -232,6 +247,13 @@ DTmp dtmp(pfem2gpu_handle* h) {
   return t;
 }
 
+void drop_graph(pfem2gpu_handle* h) {
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->graph_exec = nullptr;
+  h->graph = nullptr;
+}
+
 void free_particles(pfem2gpu_handle* h) {
   for (int k = 0; k < 3; ++k) {
     dfree(h->px[k]);
@@ -257,6 +279,7 @@ int grow(pfem2gpu_handle* h, long need, long keep, int capc) {
   }
   capc = std::max(capc, h->capc);
   const long tsize = (long)h->nchunks * capc + 2 * ncap;
+  if (ncap != h->cap || tsize != h->tsize) drop_graph(h);  // a captured step refers to the old buffers
   if (ncap != h->cap) {
     double *nx[3] = {nullptr, nullptr, nullptr}, *nvv[3] = {nullptr, nullptr, nullptr};
     int* ncell = nullptr;
@@ -334,6 +357,7 @@ int read_count(pfem2gpu_handle* h, long* n) {
 // (staging buckets). Reads the count back only when the host-side bound says
 // the capacity might not suffice.
 int reserve(pfem2gpu_handle* h, long extra, long rebin_extra) {
+  if (h->capturing) return PFEM2GPU_OK;  // capacity was secured before the capture began
   if (h->n_ub + extra <= h->cap && capc_for(h, h->n_ub + rebin_extra) <= h->capc) return PFEM2GPU_OK;
   long n = 0;
   CHK(read_count(h, &n));
@@ -397,7 +421,9 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
   const DParts P = dparts(h);
   const DTmp T = dtmp(h);
   const long nmax_p = h->n_ub + n_add;
-  const int g = grid_for((nmax_p + kMoveChunk / 64 - 1) / (kMoveChunk / 64));  // one wavefront per kMoveChunk
+  const int waves_per_block = h->move_block / 64;
+  long gm = (nmax_p + (long)h->move_run * waves_per_block - 1) / ((long)h->move_run * waves_per_block);
+  const int g = (int)std::max(1L, std::min(gm, 65536L));  // one wavefront per run of move_run particles
   const int* nptr = h->cell_start + h->nc;
   const int nch = h->nchunks;
   int* bcount = h->count;
@@ -413,19 +439,21 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
   LAUNCH(h, "k_fill", k_fill_int, grid_for(2L * nch), 0, reinterpret_cast<int*>(h->lb), 2L * nch, 0);
   if (advect)
     LAUNCH(h, "k_cellfield", k_cellfield<D>, grid_for((long)h->nc * h->nvc), 0, h->nc, h->cd, Vf, h->Vc);
-  LAUNCH(h, advect ? "k_move" : "k_rebin", k_move<D>, g, 0, advect, m, h->Vc, nptr, n_add, P, dv, use_dv, h->cshift,
-         h->capc, bcount, spill_base, spill_n, T);
+  t_begin(h, advect ? "k_move" : "k_rebin");
+  hipLaunchKernelGGL(k_move<D>, dim3(g), dim3(h->move_block), 0, h->stream, advect, m, h->Vc, nptr, n_add, P, dv,
+                     use_dv, h->cshift, h->capc, bcount, spill_base, spill_n, T, h->move_run);
+  t_end(h);
+  HIPCHK(hipGetLastError());
   if (use_dv) h->pending_corr = false;
   LAUNCH(h, "k_spill", k_spill_count, grid_for(nmax_p), 0, spill_n, spill_base, h->tcell, h->cshift, scount);
   CHK(scan(h, scount, nch, h->sstart));
   LAUNCH(h, "k_spill", k_spill_scatter<D>, grid_for(nmax_p), 0, spill_n, spill_base, spill2_base, h->cshift,
          h->sstart, sfill, T);
-  const int gs = std::min(nch, 2048);
+  const int gs = std::min(nch, h->settle_grid);
   const int cap_on = advect, refill_on = advect;
   LAUNCH(h, "k_settle", k_settle<D>, gs, h->settle_smem, cap_on, refill_on, m, h->nc, h->cshift, h->pshift, h->nmax,
-         h->nslots, h->capc, bcount, h->sstart, spill2_base, T, h->count + 3 * nch + 1, h->lb, h->d_err, h->fstart, P,
-         Vf, h->contrib, h->dirty);
-  std::swap(h->cell_start, h->fstart);
+         h->nslots, h->capc, bcount, h->sstart, spill2_base, T, h->count + 3 * nch + 1, h->lb, h->d_err, h->cell_start,
+         P, Vf, h->contrib, h->dirty);
   h->n_ub = nmax_p + (advect ? h->refill_max : 0);
   h->contrib_valid = true;
   return PFEM2GPU_OK;
@@ -466,14 +494,7 @@ int append_staged(pfem2gpu_handle* h, long n, const double* const* sx, const dou
 
 // =================================================================== C ABI
 
-extern "C" {
-
-const char* pfem2gpu_last_error(void) { return g_err.c_str(); }
-
-int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, void* hip_stream,
-                    pfem2gpu_handle** out) {
-  if (!mesh || !params || !out) return fail(PFEM2GPU_ERR_ARG, "null argument");
-  *out = nullptr;
+static int validate_mesh(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params) {
   if (mesh->dim != 2 && mesh->dim != 3) return fail(PFEM2GPU_ERR_ARG, "dim must be 2 or 3");
   if (mesh->n_cells <= 0 || mesh->n_vertices <= 0 || mesh->n_dofs <= 0 || !mesh->vertices || !mesh->cell_vertices)
     return fail(PFEM2GPU_ERR_ARG, "empty mesh");
@@ -489,7 +510,39 @@ int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, vo
       return fail(PFEM2GPU_ERR_ARG, "cell_vertices entry out of range");
     if (cdofs[i] < 0 || cdofs[i] >= mesh->n_dofs) return fail(PFEM2GPU_ERR_ARG, "cell_dofs entry out of range");
   }
+  return PFEM2GPU_OK;
+}
 
+// parameters of one instance (src/pfem2particlehandler.cpp:34-39,47,362-364)
+static Prm make_prm(int dim, const pfem2gpu_params* params) {
+  Prm p{};
+  p.n_sub = params->particle_integration_steps == 0 ? 1 : (int)params->particle_integration_steps;
+  p.dt_sub = params->time_step / (double)p.n_sub;
+  p.maxpp = (int)params->max_particles_per_cell_part;
+  p.P = 1;
+  p.Pvalid = 1;
+  for (int k = 0; k < 3; ++k) {
+    p.q[k] = k < dim ? (int)params->particles_per_cell[k] : 1;
+    p.Q[k] = k < dim ? p.q[k] + 1 : 1;
+    p.h[k] = 1.0 / (double)(unsigned)p.q[k];
+  }
+  p.qpow2 = 1;
+  for (int k = 0; k < 3; ++k) p.qf[k] = (double)p.q[k];
+  for (int k = 0; k < dim; ++k) {
+    p.P *= p.Q[k];
+    p.Pvalid *= p.q[k];
+    if (p.q[k] & (p.q[k] - 1)) p.qpow2 = 0;
+  }
+  return p;
+}
+
+// Builds a handle over `mesh` (a concatenation of instances for a batch) with
+// one parameter set per instance and the instance of every cell.
+static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, const std::vector<int>& cell_inst,
+                       const std::vector<long>& cell_off, const std::vector<long>& dof_off, void* hip_stream,
+                       pfem2gpu_handle** out) {
+  const int dim = mesh->dim, nvc = 1 << dim;
+  const int32_t* cdofs = mesh->cell_dofs ? mesh->cell_dofs : mesh->cell_vertices;
   pfem2gpu_handle* h = new pfem2gpu_handle();
   h->dim = dim;
   h->nvc = nvc;
@@ -510,26 +563,21 @@ int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, vo
     return rc;
   };
 
-  // parameters (src/pfem2particlehandler.cpp:34-39)
-  Prm& p = h->prm_host;
-  p.n_sub = params->particle_integration_steps == 0 ? 1 : (int)params->particle_integration_steps;
-  p.dt_sub = params->time_step / (double)p.n_sub;
-  p.maxpp = (int)params->max_particles_per_cell_part;
-  p.P = 1;
-  p.Pvalid = 1;
-  for (int k = 0; k < 3; ++k) {
-    p.q[k] = k < dim ? (int)params->particles_per_cell[k] : 1;
-    p.Q[k] = k < dim ? p.q[k] + 1 : 1;
-    p.h[k] = 1.0 / (double)(unsigned)p.q[k];
+  h->ninst = (int)prms.size();
+  h->prm_inst = prms;
+  h->prm_host = prms[0];
+  h->inst_cell_off = cell_off;
+  h->inst_dof_off = dof_off;
+  int Pmax = 1, Pvmax = 1;
+  h->refill_max = 0;
+  for (int c = 0; c < h->nc; ++c) h->refill_max += prms[cell_inst.empty() ? 0 : cell_inst[c]].Pvalid;
+  for (const Prm& q : prms) {
+    Pmax = std::max(Pmax, q.P);
+    Pvmax = std::max(Pvmax, q.Pvalid);
   }
-  p.qpow2 = 1;
-  for (int k = 0; k < 3; ++k) p.qf[k] = (double)p.q[k];
-  for (int k = 0; k < dim; ++k) {
-    p.P *= p.Q[k];
-    p.Pvalid *= p.q[k];
-    if (p.q[k] & (p.q[k] - 1)) p.qpow2 = 0;
-  }
-  h->refill_max = h->nc * p.Pvalid;
+  Prm p = h->prm_host;
+  p.P = Pmax;
+  p.Pvalid = Pvmax;  // sizing only
   h->pshift = 0;
   while ((1 << h->pshift) < p.P) ++h->pshift;
   h->pmax = 1 << h->pshift;
@@ -604,23 +652,31 @@ int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, vo
     if (hipMemcpy(h->geom, geom.data(), geom.size() * 8, hipMemcpyHostToDevice))
       return bail(fail(PFEM2GPU_ERR_HIP, "upload geometry"));
   }
-  // part decode table: refillable part -> INT_MIN | pi_0 | pi_1 << 8 | pi_2 << 16
+  // part decode table per instance: refillable part -> INT_MIN | pi_0 | pi_1 << 8 | pi_2 << 16
   {
-    std::vector<int> tab(h->pmax, 0);
-    for (int pp = 0; pp < p.P; ++pp) {
-      int rem = pp, code = 0;
-      bool valid = true;
-      for (int d = dim - 1; d >= 0; --d) {
-        const int pk = rem % p.Q[d];
-        rem /= p.Q[d];
-        valid = valid && pk < p.q[d];
-        code |= pk << (8 * d);
+    std::vector<int> tab((size_t)h->ninst * h->pmax, 0);
+    for (int in = 0; in < h->ninst; ++in) {
+      const Prm& q = prms[in];
+      for (int pp = 0; pp < q.P; ++pp) {
+        int rem = pp, code = 0;
+        bool valid = true;
+        for (int d = dim - 1; d >= 0; --d) {
+          const int pk = rem % q.Q[d];
+          rem /= q.Q[d];
+          valid = valid && pk < q.q[d];
+          code |= pk << (8 * d);
+        }
+        tab[(size_t)in * h->pmax + pp] = valid ? (int)(0x80000000u | (unsigned)code) : 0;
       }
-      tab[pp] = valid ? (int)(0x80000000u | (unsigned)code) : 0;
     }
     if ((rc = dalloc(&h->ptab, tab.size()))) return bail(rc);
     if (hipMemcpy(h->ptab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice))
       return bail(fail(PFEM2GPU_ERR_HIP, "upload part table"));
+  }
+  if (!cell_inst.empty()) {
+    if ((rc = dalloc(&h->cell_inst, cell_inst.size()))) return bail(rc);
+    if (hipMemcpy(h->cell_inst, cell_inst.data(), cell_inst.size() * 4, hipMemcpyHostToDevice))
+      return bail(fail(PFEM2GPU_ERR_HIP, "upload cell instances"));
   }
   // dof -> (cell*nvc + i), ascending cell
   {
@@ -634,12 +690,13 @@ int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, vo
         hipMemcpy(h->d2c, lst.data(), lst.size() * 4, hipMemcpyHostToDevice))
       return bail(fail(PFEM2GPU_ERR_HIP, "upload dof map"));
   }
-  if ((rc = dalloc(&h->prm, 1))) return bail(rc);
-  if (hipMemcpy(h->prm, &h->prm_host, sizeof(Prm), hipMemcpyHostToDevice)) return bail(fail(PFEM2GPU_ERR_HIP, "prm"));
+  if ((rc = dalloc(&h->prm, prms.size()))) return bail(rc);
+  if (hipMemcpy(h->prm, prms.data(), prms.size() * sizeof(Prm), hipMemcpyHostToDevice))
+    return bail(fail(PFEM2GPU_ERR_HIP, "prm"));
 
   // per-cell work arrays
   h->scan_nb = (h->nc + kScanChunk - 1) / kScanChunk;
-  if ((rc = dalloc(&h->cell_start, h->nc + 1)) || (rc = dalloc(&h->fstart, h->nc + 1)) ||
+  if ((rc = dalloc(&h->cell_start, h->nc + 1)) ||
       (rc = dalloc(&h->count, 3 * (size_t)h->nchunks + 2)) || (rc = dalloc(&h->sstart, h->nchunks + 1)) ||
       (rc = dalloc(&h->lb, h->nchunks)) || (rc = dalloc(&h->d_err, 1)) ||
       (rc = dalloc(&h->scan_blk, h->scan_nb + 1)) ||
@@ -658,6 +715,69 @@ int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, vo
     return bail(fail(PFEM2GPU_ERR_HIP, "init"));
   if ((rc = grow(h, 1024, 0, 256))) return bail(rc);
   *out = h;
+  return PFEM2GPU_OK;
+}
+
+extern "C" {
+
+const char* pfem2gpu_last_error(void) { return g_err.c_str(); }
+
+int pfem2gpu_create(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params, void* hip_stream,
+                    pfem2gpu_handle** out) {
+  if (!mesh || !params || !out) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  *out = nullptr;
+  CHK(validate_mesh(mesh, params));
+  std::vector<Prm> prms(1, make_prm(mesh->dim, params));
+  return create_impl(mesh, prms, {}, {0, mesh->n_cells}, {0, mesh->n_dofs}, hip_stream, out);
+}
+
+int pfem2gpu_batch_create(int32_t n_instances, const pfem2gpu_mesh* meshes, const pfem2gpu_params* params,
+                          void* hip_stream, pfem2gpu_handle** out) {
+  if (!meshes || !params || !out || n_instances <= 0) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  *out = nullptr;
+  const int dim = meshes[0].dim, nvc = 1 << dim;
+  long nv = 0, nc = 0, nd = 0;
+  std::vector<long> coff(1, 0), doff(1, 0), voff(1, 0);
+  for (int i = 0; i < n_instances; ++i) {
+    CHK(validate_mesh(&meshes[i], &params[i]));
+    if (meshes[i].dim != dim) return fail(PFEM2GPU_ERR_ARG, "all instances must have the same dim");
+    nv += meshes[i].n_vertices;
+    nc += meshes[i].n_cells;
+    nd += meshes[i].n_dofs;
+    voff.push_back(nv);
+    coff.push_back(nc);
+    doff.push_back(nd);
+  }
+  if (nc > 0x7fffffff / 8 || nv > 0x7fffffff || nd > 0x7fffffff)
+    return fail(PFEM2GPU_ERR_ARG, "batch too large for int32 indexing");
+  std::vector<double> X((size_t)dim * nv);
+  std::vector<int32_t> cv((size_t)nc * nvc), cd((size_t)nc * nvc);
+  std::vector<int> inst(nc);
+  std::vector<Prm> prms;
+  for (int i = 0; i < n_instances; ++i) {
+    const pfem2gpu_mesh& m = meshes[i];
+    for (int k = 0; k < dim; ++k)
+      std::copy(m.vertices + (size_t)k * m.n_vertices, m.vertices + (size_t)(k + 1) * m.n_vertices,
+                X.begin() + (size_t)k * nv + voff[i]);
+    const int32_t* cdi = m.cell_dofs ? m.cell_dofs : m.cell_vertices;
+    for (long e = 0; e < m.n_cells * nvc; ++e) {
+      cv[(size_t)coff[i] * nvc + e] = m.cell_vertices[e] + (int32_t)voff[i];
+      cd[(size_t)coff[i] * nvc + e] = cdi[e] + (int32_t)doff[i];
+    }
+    for (long c = 0; c < m.n_cells; ++c) inst[coff[i] + c] = i;
+    prms.push_back(make_prm(dim, &params[i]));
+  }
+  pfem2gpu_mesh all{dim, nv, X.data(), nc, cv.data(), cd.data(), nd};
+  return create_impl(&all, prms, inst, coff, doff, hip_stream, out);
+}
+
+int pfem2gpu_batch_layout(pfem2gpu_handle* h, int64_t* cell_offsets, int64_t* dof_offsets, int32_t* n_instances) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  if (n_instances) *n_instances = h->ninst;
+  for (int i = 0; i <= h->ninst; ++i) {
+    if (cell_offsets) cell_offsets[i] = h->inst_cell_off[i];
+    if (dof_offsets) dof_offsets[i] = h->inst_dof_off[i];
+  }
   return PFEM2GPU_OK;
 }
 
@@ -680,12 +800,12 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
   dfree(h->d2c_off);
   dfree(h->d2c);
   dfree(h->prm);
+  dfree(h->cell_inst);
   dfree(h->geom);
   dfree(h->cvo);
   dfree(h->ptab);
   dfree(h->Vc);
   dfree(h->cell_start);
-  dfree(h->fstart);
   dfree(h->count);
   dfree(h->sstart);
   dfree(h->lb);
@@ -700,6 +820,8 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
     (void)hipEventDestroy(t.b);
   }
   for (auto e : h->pool) (void)hipEventDestroy(e);
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return PFEM2GPU_OK;
@@ -811,6 +933,10 @@ int pfem2gpu_seed_particles(pfem2gpu_handle* h, const double* const* V, int wher
   DField F;
   CHK(field_in(h, V, where, 0, &F));
   CHK(flush(h));
+  for (const Prm& q : h->prm_inst)
+    for (int k = 0; k < h->dim; ++k)
+      if (q.q[k] != h->prm_host.q[k])
+        return fail(PFEM2GPU_ERR_ARG, "seed_particles on a batch needs the same particles_per_cell in every instance");
   const long n = (long)h->nc * h->prm_host.Pvalid;
   if (n > 0x7fffffffL) return fail(PFEM2GPU_ERR_ARG, "too many particles");
   CHK(reserve(h, n, n));
@@ -828,6 +954,7 @@ int pfem2gpu_seed_particles(pfem2gpu_handle* h, const double* const* V, int wher
 
 int pfem2gpu_clear_particles(pfem2gpu_handle* h) {
   if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  drop_graph(h);
   HIPCHK(hipMemsetAsync(h->cell_start, 0, (h->nc + 1) * sizeof(int), h->stream));
   h->pending_corr = false;
   h->contrib_valid = false;
@@ -953,6 +1080,81 @@ int pfem2gpu_cell_counts(pfem2gpu_handle* h, int32_t* counts) {
   HIPCHK(hipMemcpyAsync(cs.data(), h->cell_start, (h->nc + 1) * sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   for (int c = 0; c < h->nc; ++c) counts[c] = cs[c + 1] - cs[c];
+  return PFEM2GPU_OK;
+}
+
+static int step_eager(pfem2gpu_handle* h, const double* const* V, double* const* Vp) {
+  CHK(pfem2gpu_move_particles(h, V, PFEM2GPU_DEVICE));
+  CHK(pfem2gpu_project_particle_fields(h, Vp, PFEM2GPU_DEVICE));
+  CHK(pfem2gpu_correct_particle_velocity(h, V, Vp, PFEM2GPU_DEVICE));
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_step(pfem2gpu_handle* h, const double* const* V, double* const* V_proj, int use_graph) {
+  if (!h || !V || !V_proj) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  for (int k = 0; k < h->dim; ++k)
+    if (!V[k] || !V_proj[k]) return fail(PFEM2GPU_ERR_ARG, "null field component");
+  if (!h->maps_ready) return fail(PFEM2GPU_ERR_STATE, "initialize_maps() must be called before pfem2gpu_step()");
+  // the captured sequence starts from a pending correction; the first step
+  // of a run (none pending) and timed runs go eagerly
+  if (!use_graph || !h->pending_corr || h->timing) return step_eager(h, V, V_proj);
+  bool same = h->graph_exec != nullptr;
+  for (int k = 0; k < h->dim; ++k) same = same && h->graph_V[k] == V[k] && h->graph_Vp[k] == V_proj[k];
+  // capacity for this replay (a grow moves buffers and forces a new capture)
+  if (h->n_ub + h->refill_max > h->cap || capc_for(h, h->n_ub) > h->capc) {
+    double* before = h->px[0];
+    const long tbefore = h->tsize;
+    CHK(reserve(h, h->refill_max, 0));
+    if (h->px[0] != before || h->tsize != tbefore) same = false;
+  }
+  if (!same) {
+    drop_graph(h);
+    const long n_ub0 = h->n_ub;
+    h->capturing = true;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = step_eager(h, V, V_proj);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    h->capturing = false;
+    if (rc != PFEM2GPU_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(PFEM2GPU_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    h->graph = g;
+    HIPCHK(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+    h->n_ub = n_ub0;  // the capture only recorded the step
+    for (int k = 0; k < 3; ++k) {
+      h->graph_V[k] = k < h->dim ? V[k] : nullptr;
+      h->graph_Vp[k] = k < h->dim ? V_proj[k] : nullptr;
+    }
+  }
+  HIPCHK(hipGraphLaunch(h->graph_exec, h->stream));
+  h->n_ub += h->refill_max;
+  h->pending_corr = true;
+  h->contrib_valid = false;
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_set_tuning(pfem2gpu_handle* h, const pfem2gpu_tuning* t) {
+  if (!h || !t) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  if (t->move_block != 64 && t->move_block != 128 && t->move_block != 256)
+    return fail(PFEM2GPU_ERR_ARG, "move_block must be 64, 128 or 256");
+  if (t->move_run < 64 || t->move_run > 8192 || (t->move_run % 64) != 0)
+    return fail(PFEM2GPU_ERR_ARG, "move_run must be a multiple of 64 in [64, 8192]");
+  if (t->settle_grid < 1 || t->settle_grid > 65536) return fail(PFEM2GPU_ERR_ARG, "settle_grid must be in [1, 65536]");
+  h->move_block = t->move_block;
+  h->move_run = t->move_run;
+  h->settle_grid = t->settle_grid;
+  drop_graph(h);
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_get_tuning(pfem2gpu_handle* h, pfem2gpu_tuning* t) {
+  if (!h || !t) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  t->move_block = h->move_block;
+  t->move_run = h->move_run;
+  t->settle_grid = h->settle_grid;
   return PFEM2GPU_OK;
 }
 

@@ -173,27 +173,25 @@ __global__ void k_spill_scatter(const int* __restrict__ spill_n, long spill_base
 // correction is applied from the pre-move cell and reference position, as
 // correct_particle_velocity() precedes move_particles().
 //
-// Each wavefront owns a run of kMoveChunk consecutive particles (sorted by
+// Each wavefront owns a run of `run` consecutive particles (sorted by
 // cell, so lanes share cells and their cached data comes from L1/L2) and
 // processes it as a work queue: every lane runs its particle's Euler
 // substeps until it leaves its cell or finishes; the lanes that left their
 // cell then run the neighbour walk together; lanes whose particle is done
 // take the next particle of the run. The walk thus executes with many lanes
 // active instead of once per substep for a few.
-constexpr int kMoveChunk = 512;
-
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_move(int advect, DMesh m, const double* __restrict__ Vc,
                                                 const int* __restrict__ n_ptr, int n_add, DParts P, DField dV,
                                                 int use_dv, int cshift, int capc, int* __restrict__ bcount,
-                                                long spill_base, int* __restrict__ spill_n, DTmp T) {
+                                                long spill_base, int* __restrict__ spill_n, DTmp T, int run) {
   const int n = *n_ptr + n_add;
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (1ULL << lane) - 1ULL;
   const long gw = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long nw = ((long)gridDim.x * blockDim.x) >> 6;
-  for (long wb = gw * kMoveChunk; wb < n; wb += nw * kMoveChunk) {
-    const int wend = (int)min((long)n, wb + kMoveChunk);
+  for (long wb = gw * run; wb < n; wb += nw * run) {
+    const int wend = (int)min((long)n, wb + run);
     int next = (int)wb + 64;  // first particle of the run not yet taken
     int p = (int)wb + lane;
     bool fresh = true;        // lane must load particle p

@@ -1,0 +1,120 @@
+"""Batched form and graph-captured step on the GPU. Marked gpu.
+
+A batch must reproduce, instance by instance and bit for bit, what a plain
+handle over that instance alone computes (which the parity tests tie to the
+oracle); a replayed hipGraph step must reproduce the eager step."""
+import sys
+
+import numpy as np
+import pytest
+
+from pfem2dealii_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def _concat(instances, offsets_c, offsets_d):
+    x = np.concatenate([i["x"] for i in instances], axis=1)
+    cells = np.concatenate([i["cells"] + offsets_c[k] for k, i in enumerate(instances)]).astype(np.int32)
+    V = np.concatenate([i["V"] for i in instances], axis=1)
+    return x, cells, V
+
+
+def _same(a, b):
+    return np.array_equal(np.nan_to_num(a, nan=7.0), np.nan_to_num(b, nan=7.0)) and \
+        np.array_equal(np.isnan(a), np.isnan(b))
+
+
+def test_batch_matches_plain_instances(gpu_available):
+    _say("batch of 6 channel instances (mesh scale 1/4) vs 6 plain handles, 3 steps")
+    import torch
+    from pfem2dealii_amd import BatchHandler, ParticleHandler
+    inst = W.batched_instances(6, seed=4, scale=0.25)
+    b = BatchHandler([i["mesh"] for i in inst], [i["params"] for i in inst])
+    b.initialize_maps()
+    co, do = b.cell_offsets, b.dof_offsets
+    x, cells, V = _concat(inst, co, do)
+    assert b.insert_particles(x, cells) == 0
+    b.set_velocity_from_field(V)
+    plains = []
+    for i in inst:
+        h = ParticleHandler(i["mesh"], i["params"])
+        h.initialize_maps()
+        h.insert_particles(i["x"], i["cells"])
+        h.set_velocity_from_field(i["V"])
+        plains.append(h)
+    for s in range(3):
+        b.move_particles(V)
+        Vb = b.project_particle_fields()
+        b.correct_particle_velocity(V, Vb)
+        gb = b.download()
+        for k, (i, h) in enumerate(zip(inst, plains)):
+            h.move_particles(i["V"])
+            Vh = h.project_particle_fields()
+            h.correct_particle_velocity(i["V"], Vh)
+            assert _same(Vb[:, do[k]:do[k + 1]], Vh), "instance %d step %d projection" % (k, s)
+            gh = h.download()
+            sel = (gb["cell"] >= co[k]) & (gb["cell"] < co[k + 1])
+            assert np.array_equal(gb["cell"][sel] - co[k], gh["cell"]), "instance %d cells" % k
+            assert np.array_equal(gb["x"][:, sel], gh["x"]), "instance %d positions" % k
+            assert np.array_equal(gb["v"][:, sel], gh["v"]), "instance %d velocities" % k
+    torch.cuda.synchronize()
+
+
+def test_graph_step_matches_eager(gpu_available):
+    _say("hipGraph step replay vs eager step, channel 64x32, 5 steps")
+    import torch
+    from pfem2dealii_amd import ParticleHandler
+    cfg = W.config("channel2d")
+    hs = []
+    for _ in range(2):
+        h = ParticleHandler(cfg["mesh"], cfg["params"])
+        h.initialize_maps()
+        h.insert_particles(cfg["x"], cfg["cells"])
+        h.set_velocity_from_field(cfg["V"])
+        hs.append(h)
+    V = torch.tensor(cfg["V"], dtype=torch.float64, device="cuda")
+    outs = [torch.empty_like(V), torch.empty_like(V)]
+    torch.cuda.synchronize()
+    for s in range(5):
+        hs[0].step(V, outs[0], use_graph=False)
+        hs[1].step(V, outs[1], use_graph=True)
+        hs[0].synchronize()
+        hs[1].synchronize()
+        a, b = outs[0].cpu().numpy(), outs[1].cpu().numpy()
+        assert _same(a, b), "step %d projection" % s
+        da, db = hs[0].download(), hs[1].download()
+        assert np.array_equal(da["cell"], db["cell"]) and np.array_equal(da["x"], db["x"])
+        assert np.array_equal(da["v"], db["v"]), "step %d velocities" % s
+        # eager work between replays
+        torch.cuda.synchronize()
+        assert hs[1].count() == hs[0].count()
+
+
+def test_tuning_does_not_change_results(gpu_available):
+    _say("launch tunings give bitwise identical steps")
+    from pfem2dealii_amd import ParticleHandler
+    cfg = W.config("taylor_green2d", scale=1.0 / 16)
+    res = []
+    for tune in ({"move_block": 256, "move_run": 512, "settle_grid": 2048},
+                 {"move_block": 64, "move_run": 128, "settle_grid": 7},
+                 {"move_block": 128, "move_run": 2048, "settle_grid": 300}):
+        h = ParticleHandler(cfg["mesh"], cfg["params"])
+        h.set_tuning(**tune)
+        assert h.get_tuning() == tune
+        h.initialize_maps()
+        h.insert_particles(cfg["x"], cfg["cells"])
+        h.set_velocity_from_field(cfg["V"])
+        for _ in range(2):
+            h.move_particles(cfg["V"])
+            Vp = h.project_particle_fields()
+            h.correct_particle_velocity(cfg["V"], Vp)
+        res.append((h.download(), Vp))
+    for d, Vp in res[1:]:
+        assert np.array_equal(d["cell"], res[0][0]["cell"]) and np.array_equal(d["x"], res[0][0]["x"])
+        assert np.array_equal(d["v"], res[0][0]["v"]) and _same(Vp, res[0][1])

@@ -154,6 +154,26 @@ typedef struct {
 int pfem2gpu_set_tuning(pfem2gpu_handle* h, const pfem2gpu_tuning* t);
 int pfem2gpu_get_tuning(pfem2gpu_handle* h, pfem2gpu_tuning* t);
 
+/* ---- Adjoint of advect + project (no counterpart in the reference) ----
+ * Runs n_steps of move_particles(V) from the current particles, projects the
+ * final state (P) and returns L = 1/2 |P - target|^2 with its gradient with
+ * respect to the initial particle velocities (grad_v0, [dim][n0] with n0 the
+ * particle count at the call) and to the nodal field V (grad_V,
+ * [dim][n_dofs]). Cells, survivors and refills are frozen at the forward
+ * pass's choices. The forward state is checkpointed every checkpoint_every
+ * steps and recomputed segment by segment during the reverse sweep. All
+ * arrays are device memory. Box cells only; at most 8 substeps. Leaves the
+ * handle at the final forward state. */
+int pfem2gpu_adjoint(pfem2gpu_handle* h, int32_t n_steps, int32_t checkpoint_every, const double* const* V,
+                     const double* const* target, double* const* grad_v0, double* const* grad_V, double* loss);
+
+/* move_particles() that also returns, for every particle after the move,
+ * its index before the move, or -1 - part for a particle the redistribution
+ * created in sub-cell part `part` (src: host array of src_capacity entries;
+ * *n_out = particle count after the move). */
+int pfem2gpu_move_particles_traced(pfem2gpu_handle* h, const double* const* V, int where, int32_t* src,
+                                   int64_t src_capacity, int64_t* n_out);
+
 /* The handle's HIP stream (hipStream_t). */
 void* pfem2gpu_stream(pfem2gpu_handle* h);
 

@@ -57,6 +57,7 @@ EXPORTS = [
     "pfem2gpu_download", "pfem2gpu_cell_counts",
     "pfem2gpu_stream", "pfem2gpu_synchronize", "pfem2gpu_set_timing", "pfem2gpu_timings",
     "pfem2gpu_batch_create", "pfem2gpu_batch_layout", "pfem2gpu_step", "pfem2gpu_set_tuning", "pfem2gpu_get_tuning",
+    "pfem2gpu_adjoint", "pfem2gpu_move_particles_traced",
 ]
 
 
@@ -96,6 +97,9 @@ def load_library(path=None):
     lib.pfem2gpu_step.argtypes = [_vp, _vp, _vp, ctypes.c_int]
     lib.pfem2gpu_set_tuning.argtypes = [_vp, ctypes.POINTER(Tuning)]
     lib.pfem2gpu_get_tuning.argtypes = [_vp, ctypes.POINTER(Tuning)]
+    lib.pfem2gpu_adjoint.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, _vp,
+                                     ctypes.POINTER(ctypes.c_double)]
+    lib.pfem2gpu_move_particles_traced.argtypes = [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int64, _i64p]
     if path is None:
         _lib = lib
     return lib
@@ -288,6 +292,37 @@ class ParticleHandler:
         if a.where != DEVICE or b.where != DEVICE:
             raise ValueError("step() needs device (torch CUDA) fields")
         _check(self.lib, self.lib.pfem2gpu_step(self.h, a.arr, b.arr, 1 if use_graph else 0), "step")
+
+    def move_particles_traced(self, V):
+        """move_particles(V) that returns, per particle after the move, its
+        index before the move (or -1 - part for refills)."""
+        f = _FieldPtrs(V, self.dim, self.n_dofs)
+        cap = self.count() + self.n_cells * 128 + 1024
+        src = np.empty(cap, dtype=np.int32)
+        n = ctypes.c_int64(0)
+        _check(self.lib, self.lib.pfem2gpu_move_particles_traced(self.h, f.arr, f.where, src.ctypes.data, cap,
+                                                                 ctypes.byref(n)), "move_particles_traced")
+        return src[:n.value].copy()
+
+    def adjoint(self, n_steps, checkpoint_every, V, target):
+        """(loss, dL/dv0 [dim][n0], dL/dV [dim][n_dofs]) of n_steps of move and
+        a final projection against `target` (device tensors)."""
+        import torch
+        n0 = self.count()
+        a = _FieldPtrs(V, self.dim, self.n_dofs)
+        b = _FieldPtrs(target, self.dim, self.n_dofs)
+        if a.where != DEVICE or b.where != DEVICE:
+            raise ValueError("adjoint() needs device (torch CUDA) fields")
+        gv0 = torch.zeros((self.dim, max(n0, 1)), dtype=torch.float64, device=V.device)
+        gV = torch.zeros((self.dim, self.n_dofs), dtype=torch.float64, device=V.device)
+        pv = (_vp * 3)(*[gv0.data_ptr() + 8 * gv0.shape[1] * k for k in range(self.dim)] + [None] * (3 - self.dim))
+        pV = (_vp * 3)(*[gV.data_ptr() + 8 * self.n_dofs * k for k in range(self.dim)] + [None] * (3 - self.dim))
+        loss = ctypes.c_double(0.0)
+        torch.cuda.synchronize()
+        _check(self.lib, self.lib.pfem2gpu_adjoint(self.h, int(n_steps), int(checkpoint_every), a.arr, b.arr, pv, pV,
+                                                   ctypes.byref(loss)), "adjoint")
+        self.synchronize()
+        return loss.value, gv0[:, :n0], gV
 
     def set_tuning(self, move_block=256, move_run=512, settle_grid=2048):
         t = Tuning(int(move_block), int(move_run), int(settle_grid))

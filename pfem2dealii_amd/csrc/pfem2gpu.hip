@@ -23,7 +23,7 @@
 #include <vector>
 
 #include "../../include/pfem2gpu.h"
-#include "pfem2gpu_kernels.h"
+#include "pfem2gpu_adjoint.h"
 
 using namespace pfem2gpu;
 
@@ -70,6 +70,7 @@ struct pfem2gpu_handle {
   int* d2c = nullptr;
   Prm* prm = nullptr;
   int* cell_inst = nullptr;          // [nc] instance of each cell (batches only)
+  bool all_box = false;              // every cell is an axis-aligned box
   int ninst = 1;
   std::vector<Prm> prm_inst;         // per-instance parameters (host copy)
   std::vector<long> inst_cell_off;   // [ninst+1]
@@ -111,6 +112,10 @@ struct pfem2gpu_handle {
   int cpw = 16, pmax = 1, nmax = 512, nslots = 0, settle_smem = 0;  // cpw = 1 << cshift
   unsigned char* dirty = nullptr;  // per settle chunk: sums left to k_contrib_dirty
   bool contrib_valid = false;      // per (cell, vertex) sums match the particles
+
+  // identity recording for the adjoint (null when off)
+  int* rec_src = nullptr;  // where the next move writes each slot's source particle
+  int* tsrc = nullptr;     // [tsize] staging source indices
 
   // tuning (set by pfem2gpu_set_tuning; chosen by the autotuner)
   int move_block = 256;     // workgroup size of the move kernel
@@ -234,6 +239,7 @@ DParts dparts(pfem2gpu_handle* h) {
     p.v[k] = h->pv[k];
   }
   p.cell = h->pcell;
+  p.src = h->rec_src;
   return p;
 }
 
@@ -244,6 +250,7 @@ DTmp dtmp(pfem2gpu_handle* h) {
     t.v[k] = h->tv[k];
   }
   t.cell = h->tcell;
+  t.src = h->rec_src ? h->tsrc : nullptr;
   return t;
 }
 
@@ -263,6 +270,7 @@ void free_particles(pfem2gpu_handle* h) {
   }
   dfree(h->pcell);
   dfree(h->tcell);
+  dfree(h->tsrc);
   h->cap = 0;
   h->tsize = 0;
 }
@@ -320,6 +328,10 @@ int grow(pfem2gpu_handle* h, long need, long keep, int capc) {
       CHK(dalloc(&h->tv[k], tsize));
     }
     CHK(dalloc(&h->tcell, tsize));
+    if (h->tsrc) {
+      dfree(h->tsrc);
+      CHK(dalloc(&h->tsrc, tsize));
+    }
     h->tsize = tsize;
     h->capc = capc;
   }
@@ -648,6 +660,7 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
       }
       if (!boxh[c]) g[0] = std::nan("");
     }
+    h->all_box = std::all_of(boxh.begin(), boxh.end(), [](unsigned char b) { return b != 0; });
     if ((rc = dalloc(&h->geom, geom.size())) || (rc = dalloc(&h->Vc, (size_t)h->nc * nvc * dim))) return bail(rc);
     if (hipMemcpy(h->geom, geom.data(), geom.size() * 8, hipMemcpyHostToDevice))
       return bail(fail(PFEM2GPU_ERR_HIP, "upload geometry"));
@@ -1134,6 +1147,206 @@ int pfem2gpu_step(pfem2gpu_handle* h, const double* const* V, double* const* V_p
   h->pending_corr = true;
   h->contrib_valid = false;
   return PFEM2GPU_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ adjoint
+
+namespace {
+
+struct DevBuf {
+  std::vector<void*> ptrs;
+  ~DevBuf() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  int get(T** p, size_t n) {
+    CHK(dalloc(p, n));
+    ptrs.push_back(*p);
+    return PFEM2GPU_OK;
+  }
+};
+
+struct Snapshot {
+  long n = 0;
+  double* x[3] = {nullptr, nullptr, nullptr};
+  double* v[3] = {nullptr, nullptr, nullptr};
+  int* cell = nullptr;
+  int* cs = nullptr;
+};
+
+int snap_save(pfem2gpu_handle* h, DevBuf& mem, Snapshot& s) {
+  CHK(read_count(h, &s.n));
+  const size_t n = std::max<long>(s.n, 1);
+  for (int k = 0; k < h->dim; ++k) {
+    CHK(mem.get(&s.x[k], n));
+    CHK(mem.get(&s.v[k], n));
+    HIPCHK(hipMemcpyAsync(s.x[k], h->px[k], s.n * 8, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(s.v[k], h->pv[k], s.n * 8, hipMemcpyDeviceToDevice, h->stream));
+  }
+  CHK(mem.get(&s.cell, n));
+  CHK(mem.get(&s.cs, h->nc + 1));
+  HIPCHK(hipMemcpyAsync(s.cell, h->pcell, s.n * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(s.cs, h->cell_start, (h->nc + 1) * 4, hipMemcpyDeviceToDevice, h->stream));
+  return PFEM2GPU_OK;
+}
+
+int snap_restore(pfem2gpu_handle* h, const Snapshot& s) {
+  for (int k = 0; k < h->dim; ++k) {
+    HIPCHK(hipMemcpyAsync(h->px[k], s.x[k], s.n * 8, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->pv[k], s.v[k], s.n * 8, hipMemcpyDeviceToDevice, h->stream));
+  }
+  HIPCHK(hipMemcpyAsync(h->pcell, s.cell, s.n * 4, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->cell_start, s.cs, (h->nc + 1) * 4, hipMemcpyDeviceToDevice, h->stream));
+  h->n_ub = s.n;
+  h->pending_corr = false;
+  h->contrib_valid = false;
+  return PFEM2GPU_OK;
+}
+
+template <int D>
+int adjoint_impl(pfem2gpu_handle* h, int n_steps, int K, const double* const* Vu, const double* const* Tu,
+                 double* const* gv0, double* const* gV, double* loss) {
+  DevBuf mem;
+  const int nc = h->nc, nd = h->ndofs;
+  DField V, Tg;
+  for (int k = 0; k < 3; ++k) {
+    V.c[k] = k < D ? Vu[k] : nullptr;
+    Tg.c[k] = k < D ? Tu[k] : nullptr;
+  }
+  CHK(flush(h));
+  drop_graph(h);
+  if (!h->tsrc) CHK(dalloc(&h->tsrc, h->tsize));
+  // forward: checkpoints every K steps, identity maps and end cells of every step
+  const int nseg = (n_steps + K - 1) / K;
+  std::vector<Snapshot> ck(nseg);
+  std::vector<int*> srcs(n_steps + 1, nullptr), ecell(n_steps + 1, nullptr);
+  std::vector<long> cnt(n_steps + 1, 0);
+  for (int s = 1; s <= n_steps; ++s) {
+    if ((s - 1) % K == 0) CHK(snap_save(h, mem, ck[(s - 1) / K]));
+    CHK(reserve(h, h->refill_max, 0));
+    if (!h->tsrc || h->tsize == 0) return fail(PFEM2GPU_ERR_STATE, "staging arrays missing");
+    CHK(mem.get(&srcs[s], h->cap));
+    h->rec_src = srcs[s];
+    const int rc = rebin_or_move<D>(h, 1, 0, V);
+    h->rec_src = nullptr;
+    CHK(rc);
+    CHK(read_count(h, &cnt[s]));
+    CHK(mem.get(&ecell[s], std::max<long>(cnt[s], 1)));
+    HIPCHK(hipMemcpyAsync(ecell[s], h->pcell, cnt[s] * 4, hipMemcpyDeviceToDevice, h->stream));
+  }
+  cnt[0] = ck[0].n;
+  Snapshot fin;
+  CHK(snap_save(h, mem, fin));
+  // final projection, loss and the projection adjoint
+  double *Pn[3] = {nullptr, nullptr, nullptr}, *W = nullptr, *dloss = nullptr;
+  for (int k = 0; k < D; ++k) CHK(mem.get(&Pn[k], nd));
+  CHK(mem.get(&W, nd));
+  CHK(mem.get(&dloss, 1));
+  const DMesh m = dmesh(h);
+  LAUNCH(h, "k_contrib_dirty", k_contrib_dirty<D>, std::min(h->nchunks, 16384), 0, m, nc, h->cpw, h->dirty,
+         h->cell_start, dparts(h), h->contrib);
+  LAUNCH(h, "k_node_w", k_node_w<D>, grid_for(nd), 0, nd, h->d2c_off, h->d2c, h->contrib, Pn[0], Pn[1], Pn[2], W);
+  LAUNCH(h, "k_fill", k_fill_double, 1, 0, dloss, 1L);
+  DField Pf;
+  for (int k = 0; k < 3; ++k) Pf.c[k] = Pn[k];
+  LAUNCH(h, "k_loss", k_loss<D>, std::min(grid_for(nd), 1024), 0, nd, Pf, Tg, dloss);
+  long nmax = 1;
+  for (long c : cnt) nmax = std::max(nmax, c);
+  double *xa[3] = {nullptr, nullptr, nullptr}, *va[3] = {nullptr, nullptr, nullptr};
+  double *xb[3] = {nullptr, nullptr, nullptr}, *vb[3] = {nullptr, nullptr, nullptr};
+  for (int k = 0; k < D; ++k) {
+    CHK(mem.get(&xa[k], nmax));
+    CHK(mem.get(&va[k], nmax));
+    CHK(mem.get(&xb[k], nmax));
+    CHK(mem.get(&vb[k], nmax));
+  }
+  LAUNCH(h, "k_adj_init", k_adj_init<D>, grid_for(cnt[n_steps]), 0, m, h->cell_start + nc, dparts(h), Pf, W, Tg,
+         xa[0], xa[1], xa[2], va[0], va[1], va[2]);
+  for (int k = 0; k < D; ++k) LAUNCH(h, "k_fill", k_fill_double, grid_for(nd), 0, gV[k], (long)nd);
+  // backward, one checkpoint segment at a time (latest first): recompute the
+  // segment's start states, then reverse its steps
+  std::vector<double*> sx(K * D, nullptr);
+  std::vector<int*> scell(K, nullptr);
+  for (int j = 0; j < K; ++j) {
+    for (int k = 0; k < D; ++k) CHK(mem.get(&sx[j * D + k], nmax));
+    CHK(mem.get(&scell[j], nmax));
+  }
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int s0 = seg * K + 1, s1 = std::min(n_steps, seg * K + K);
+    CHK(snap_restore(h, ck[seg]));
+    for (int s = s0; s <= s1; ++s) {
+      const int j = s - s0;
+      for (int k = 0; k < D; ++k)
+        HIPCHK(hipMemcpyAsync(sx[j * D + k], h->px[k], cnt[s - 1] * 8, hipMemcpyDeviceToDevice, h->stream));
+      HIPCHK(hipMemcpyAsync(scell[j], h->pcell, cnt[s - 1] * 4, hipMemcpyDeviceToDevice, h->stream));
+      if (s < s1) CHK(rebin_or_move<D>(h, 1, 0, V));
+    }
+    for (int s = s1; s >= s0; --s) {
+      const int j = s - s0;
+      for (int k = 0; k < D; ++k) {
+        LAUNCH(h, "k_fill", k_fill_double, grid_for(cnt[s - 1]), 0, xb[k], cnt[s - 1]);
+        LAUNCH(h, "k_fill", k_fill_double, grid_for(cnt[s - 1]), 0, vb[k], cnt[s - 1]);
+      }
+      LAUNCH(h, "k_adj_unmap", k_adj_unmap<D>, grid_for(cnt[s]), 0, m, h->pshift, (int)cnt[s], srcs[s], ecell[s],
+             xa[0], xa[1], xa[2], va[0], va[1], va[2], xb[0], xb[1], xb[2], vb[0], vb[1], vb[2], gV[0], gV[1], gV[2]);
+      LAUNCH(h, "k_adj_move", k_adj_move<D>, grid_for(cnt[s - 1]), 0, m, h->Vc, (int)cnt[s - 1], sx[j * D],
+             sx[j * D + 1], D > 2 ? sx[j * D + 2] : nullptr, scell[j], xb[0], xb[1], xb[2], gV[0], gV[1], gV[2]);
+      for (int k = 0; k < D; ++k) {
+        std::swap(xa[k], xb[k]);
+        std::swap(va[k], vb[k]);
+      }
+    }
+  }
+  for (int k = 0; k < D; ++k)
+    HIPCHK(hipMemcpyAsync(gv0[k], va[k], cnt[0] * 8, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->h_pinned + 2, dloss, 8, hipMemcpyDeviceToHost, h->stream));
+  CHK(check_device(h));
+  std::memcpy(loss, h->h_pinned + 2, 8);
+  CHK(snap_restore(h, fin));
+  CHK(check_device(h));
+  return PFEM2GPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfem2gpu_adjoint(pfem2gpu_handle* h, int32_t n_steps, int32_t checkpoint_every, const double* const* V,
+                     const double* const* target, double* const* grad_v0, double* const* grad_V, double* loss) {
+  if (!h || !V || !target || !grad_v0 || !grad_V || !loss) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  if (n_steps < 1 || checkpoint_every < 1) return fail(PFEM2GPU_ERR_ARG, "n_steps and checkpoint_every must be >= 1");
+  if (!h->maps_ready) return fail(PFEM2GPU_ERR_STATE, "initialize_maps() must be called before the adjoint");
+  if (!h->all_box) return fail(PFEM2GPU_ERR_ARG, "the adjoint supports axis-aligned box cells only");
+  for (const Prm& q : h->prm_inst)
+    if (q.n_sub > kAdjMaxSub) return fail(PFEM2GPU_ERR_ARG, "the adjoint supports at most 8 substeps");
+  return h->dim == 2 ? adjoint_impl<2>(h, n_steps, checkpoint_every, V, target, grad_v0, grad_V, loss)
+                     : adjoint_impl<3>(h, n_steps, checkpoint_every, V, target, grad_v0, grad_V, loss);
+}
+
+int pfem2gpu_move_particles_traced(pfem2gpu_handle* h, const double* const* V, int where, int32_t* src,
+                                   int64_t src_capacity, int64_t* n_out) {
+  if (!h || !src || !n_out) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  if (!h->maps_ready) return fail(PFEM2GPU_ERR_STATE, "initialize_maps() must be called before move_particles()");
+  DField F;
+  CHK(field_in(h, V, where, 0, &F));
+  CHK(reserve(h, h->refill_max, 0));
+  drop_graph(h);
+  if (!h->tsrc) CHK(dalloc(&h->tsrc, h->tsize));
+  int* d = nullptr;
+  CHK(dalloc(&d, h->cap));
+  h->rec_src = d;
+  int rc = do_rebin(h, 1, 0, F);
+  h->rec_src = nullptr;
+  long n = 0;
+  if (rc == PFEM2GPU_OK) rc = read_count(h, &n);
+  if (rc == PFEM2GPU_OK && n > src_capacity) rc = fail(PFEM2GPU_ERR_ARG, "src capacity too small");
+  if (rc == PFEM2GPU_OK && hipMemcpy(src, d, n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(PFEM2GPU_ERR_HIP, "copy of the identity map failed");
+  (void)hipFree(d);
+  *n_out = n;
+  return rc;
 }
 
 int pfem2gpu_set_tuning(pfem2gpu_handle* h, const pfem2gpu_tuning* t) {

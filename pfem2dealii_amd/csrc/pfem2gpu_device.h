@@ -55,12 +55,14 @@ struct DParts {
   double* x[3];
   double* v[3];
   int* cell;
+  int* src;  // optional: source particle of each slot after a move (-1 - part for refills)
 };
 
 struct DTmp {
   double* x[3];
   double* v[3];
   int* cell;
+  int* src;  // optional: particle index before the move
 };
 
 __device__ __forceinline__ int inst_of(const DMesh& m, int c) { return m.cell_inst ? m.cell_inst[c] : 0; }

@@ -163,6 +163,7 @@ __global__ void k_spill_scatter(const int* __restrict__ spill_n, long spill_base
       T.v[k][dst] = T.v[k][src];
     }
     T.cell[dst] = c;
+    if (T.src) T.src[dst] = T.src[src];
   }
 }
 
@@ -286,6 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_move(int advect, DMesh m, const doub
           T.v[k][dst] = v[k];
         }
         T.cell[dst] = cc.c;
+        if (T.src) T.src[dst] = p;
       }
       const unsigned long long dm = __ballot(done);
       if (done) {
@@ -612,6 +614,7 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
           P.v[k][o0 + fpos] = T.v[k][i];
         }
         P.cell[o0 + fpos] = c;
+        if (P.src) P.src[o0 + fpos] = T.src[i];
       }
     }
     // 4. refills: one particle at the centre of each empty part of a
@@ -654,6 +657,7 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
             P.v[k][o0 + fpos] = v[k];
           }
           P.cell[o0 + fpos] = c;
+          if (P.src) P.src[o0 + fpos] = -1 - pp;
         }
       }
     }
@@ -669,6 +673,7 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
           P.v[k][o0 + t] = s_v[k * nslots + slot];
         }
         P.cell[o0 + t] = c0 + (s_pl[slot] >> 16);
+        if (P.src) P.src[o0 + t] = slot < n ? T.src[chunk_index(slot, nb, bucket0, spill0)] : -1 - (s_pl[slot] & 0xffff);
       }
       // 6. per (cell, vertex) sums, sequential in final order
       for (int t = threadIdx.x; t < ncell * NV; t += blockDim.x) {

@@ -1,0 +1,127 @@
+"""Adjoint of advect + project on the GPU against an FP64 CPU reference
+gradient of the same frozen-decision steps and against central finite
+differences. Marked gpu.
+
+Workload: the adjoint channel (Q1 mesh of [0,2]x[0,1] at 1/8 of 256x128 per
+direction, 16 seeded uniform particles per cell), Poiseuille amplitude 1.0,
+20 steps of move, final projection, loss 1/2 |P - P(amplitude 1.05)|^2,
+checkpoints every 5 steps; 32 seeded random directions, epsilon 1e-6."""
+import sys
+
+import numpy as np
+import pytest
+
+from pfem2dealii_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 20
+CKPT = 5
+
+
+def _say(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def _handle(cfg, V, x=None, cells=None, v=None):
+    from pfem2dealii_amd import ParticleHandler
+    h = ParticleHandler(cfg["mesh"], cfg["params"])
+    h.initialize_maps()
+    x = cfg["x"] if x is None else x
+    cells = cfg["cells"] if cells is None else cells
+    assert h.insert_particles(x, cells, v) == 0
+    if v is None:
+        h.set_velocity_from_field(V)
+    return h
+
+
+@pytest.fixture(scope="module")
+def adjoint_case():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from oracle.adjoint_ref import FrozenModel
+    cfg = W.config("adjoint2d", scale=1.0 / 8)
+    mesh = cfg["mesh"]
+    V1 = W.poiseuille(mesh, 1.0)
+    V105 = W.poiseuille(mesh, 1.05)
+    ht = _handle(cfg, V105)
+    for _ in range(STEPS):
+        ht.move_particles(V105)
+    target = ht.project_particle_fields()
+    h = _handle(cfg, V1)
+    d0 = h.download(xi=False)
+    Vd = torch.tensor(V1, dtype=torch.float64, device="cuda")
+    Td = torch.tensor(target, dtype=torch.float64, device="cuda")
+    loss, gv0, gV = h.adjoint(STEPS, CKPT, Vd, Td)
+    hr = _handle(cfg, V1)
+    d0r = hr.download(xi=False)
+    assert all(np.array_equal(d0[k], d0r[k]) for k in ("x", "v", "cell"))
+    srcs, ecs = [], []
+    for _ in range(STEPS):
+        srcs.append(hr.move_particles_traced(V1))
+        ecs.append(hr.download(xi=False)["cell"])
+    fin = h.download(xi=False)
+    assert np.array_equal(fin["cell"], ecs[-1]) and np.array_equal(fin["x"], hr.download(xi=False)["x"])
+    ref = FrozenModel(mesh, cfg["params"], d0["x"], d0["cell"], srcs, ecs)
+    return {"cfg": cfg, "V": V1, "target": target, "v0": d0["v"], "x0": d0["x"], "c0": d0["cell"], "loss": loss,
+            "gv0": gv0.cpu().numpy(), "gV": gV.cpu().numpy(), "ref": ref}
+
+
+def test_adjoint_matches_cpu_reference_gradient(adjoint_case):
+    _say("adjoint vs FP64 CPU reference gradient (32x16 channel, 20 steps)")
+    c = adjoint_case
+    L, gv0, gV = c["ref"].grad(c["V"], c["v0"], c["target"])
+    assert abs(c["loss"] - L) <= 1e-11 * abs(L)
+    assert np.abs(c["gv0"] - gv0).max() <= 1e-10 * np.abs(gv0).max()
+    assert np.abs(c["gV"] - gV).max() <= 1e-10 * np.abs(gV).max()
+    assert np.abs(gV).max() > 0 and np.abs(gv0).max() > 0
+
+
+def test_adjoint_matches_finite_differences(adjoint_case):
+    _say("adjoint vs central finite differences, 32 directions, eps 1e-6")
+    c = adjoint_case
+    ref = c["ref"]
+    ref.grad(c["V"], c["v0"], c["target"])  # records the frozen path
+    rng = np.random.default_rng(7)
+    eps = 1e-6
+    worst = 0.0
+    for _ in range(32):
+        dV = rng.standard_normal(c["V"].shape)
+        dv = rng.standard_normal(c["v0"].shape)
+        gdot = float((c["gV"] * dV).sum() + (c["gv0"] * dv).sum())
+        Lp = ref.loss(c["V"] + eps * dV, c["v0"] + eps * dv, c["target"])
+        Lm = ref.loss(c["V"] - eps * dV, c["v0"] - eps * dv, c["target"])
+        fd = (Lp - Lm) / (2 * eps)
+        worst = max(worst, abs(fd - gdot) / abs(gdot))
+    assert worst <= 1e-6, worst
+
+
+def test_adjoint_through_autograd(adjoint_case):
+    _say("adjoint wrapped in torch.autograd.Function")
+    import torch
+    c = adjoint_case
+    cfg = c["cfg"]
+
+    class AdvectProject(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, V, v0):
+            h = _handle(cfg, None, x=c["x0"], cells=c["c0"], v=v0.detach().cpu().numpy())
+            loss, gv0, gV = h.adjoint(STEPS, CKPT, V.detach().contiguous(), torch.tensor(c["target"], device="cuda"))
+            ctx.save_for_backward(gv0, gV)
+            h.close()
+            return torch.tensor(loss, dtype=torch.float64, device=V.device)
+
+        @staticmethod
+        def backward(ctx, g):
+            gv0, gV = ctx.saved_tensors
+            return g * gV, g * gv0
+
+    V = torch.tensor(c["V"], dtype=torch.float64, device="cuda", requires_grad=True)
+    v0 = torch.tensor(c["v0"], dtype=torch.float64, device="cuda", requires_grad=True)
+    loss = AdvectProject.apply(V, v0)
+    (3.0 * loss).backward()
+    assert abs(loss.item() - c["loss"]) <= 1e-12 * abs(c["loss"])
+    assert np.abs(V.grad.cpu().numpy() - 3.0 * c["gV"]).max() <= 1e-10 * np.abs(c["gV"]).max()
+    assert np.abs(v0.grad.cpu().numpy() - 3.0 * c["gv0"]).max() <= 1e-10 * np.abs(c["gv0"]).max()

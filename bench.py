@@ -34,6 +34,37 @@ def alg_bytes_per_particle_step(dim, n_nodes, n_particles):
     return 3 * s_p + 24 * dim * (n_nodes / float(n_particles))
 
 
+def kernel_alg_bytes(dim, n_nodes, n_in, n_out):
+    """Algorithmic bytes of one step split over the kernels that carry them:
+    the move reads and writes the particle state once (2 S_p per particle at
+    the start of the step) and reads V and V_old (16 d per node); the
+    redistribution/projection kernel reads the final state once (S_p per
+    particle after the step); the nodal division writes the projected field
+    (8 d per node). Redistribution and sorting traffic is overhead, not
+    algorithmic, and counts for no kernel."""
+    s_p = 16 * dim + 4
+    return {"k_move": 2.0 * s_p * n_in + 16.0 * dim * n_nodes, "k_settle": float(s_p) * n_out,
+            "k_node": 8.0 * dim * n_nodes}
+
+
+def hbm_traffic(workload, scale, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the same
+    counter reads for a plain copy of known size), or None."""
+    if scale != 1.0:
+        return None
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s.txt" % workload)
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        f = ln.split()
+        if len(f) >= 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
 def cpu_baseline(workload, budget_s=12.0):
     """Oracle in reference-order mode (AoS vector, std::map redistribution,
     vector-order projection), 1 thread, on a 1/4-per-direction sample of the
@@ -68,7 +99,7 @@ def cpu_baseline(workload, budget_s=12.0):
                                                            len(cfg["cells"]), done, dt)}
 
 
-def batched_comparison(steps, warmup, n_inst=256, scale=1.0, log=None):
+def batched_comparison(steps, warmup, n_inst=256, scale=1.0, log=None, force_tune=False):
     """256 independent 2D channel instances (32x16..512x256 Q1, Poisson(12)
     particles per cell, Poiseuille amplitude U[0.5, 2]): one batch handle
     (one launch per phase, hipGraph per step, autotuned launch shape) against
@@ -93,7 +124,8 @@ def batched_comparison(steps, warmup, n_inst=256, scale=1.0, log=None):
     torch.cuda.synchronize()
     bstep = lambda: b.step(V, Vp, use_graph=True)  # noqa: E731
     bstep()
-    tuning, _ = tune(b, bstep, signature={"workload": "batched", "n": n_inst, "scale": scale}, log=log)
+    tuning, _ = tune(b, bstep, signature={"workload": "batched", "n": n_inst, "scale": scale}, log=log,
+                     force=force_tune)
     for _ in range(warmup):
         bstep()
     stream = torch.cuda.ExternalStream(b.stream())
@@ -147,6 +179,80 @@ def batched_comparison(steps, warmup, n_inst=256, scale=1.0, log=None):
             "speedup": b_rate / p_rate, "tuning": tuning}
 
 
+def adjoint_cost(reps=3, scale=1.0):
+    """Cost of the adjoint against its forward on the adjoint channel (256x128
+    Q1 on [0,2]x[0,1], 16 seeded uniform particles per cell, Poiseuille
+    amplitude 1, 20 steps of 5 substeps, final projection, checkpoints every 5
+    steps). Forward: the 20 moves and the projection. Adjoint: one
+    pfem2gpu_adjoint call (its own checkpointed forward, the loss, the reverse
+    sweep with segment recompute). Wall clock with the stream drained on both
+    sides, fresh handles from the same initial state, median of `reps`."""
+    import torch
+    from pfem2dealii_amd import ParticleHandler, workloads as W
+    cfg = W.config("adjoint2d", scale=scale)
+    steps, K = cfg["steps"], 5
+
+    def fresh(amp):
+        h = ParticleHandler(cfg["mesh"], cfg["params"])
+        h.initialize_maps()
+        h.insert_particles(cfg["x"], cfg["cells"])
+        Vd = torch.tensor(W.poiseuille(cfg["mesh"], amp), dtype=torch.float64, device="cuda")
+        h.set_velocity_from_field(Vd)
+        h.synchronize()
+        return h, Vd
+
+    ht, Vt = fresh(1.05)
+    target = torch.empty_like(Vt)
+    for _ in range(steps):
+        ht.move_particles(Vt)
+    ht.project_particle_fields(target)
+    ht.synchronize()
+    ht.close()
+    fwd, adj = [], []
+    n0 = 0
+    loss = None
+    for r in range(reps + 1):
+        h, V = fresh(1.0)
+        n0 = h.count()
+        Vp = torch.empty_like(V)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            h.move_particles(V)
+        h.project_particle_fields(Vp)
+        h.synchronize()
+        t1 = time.perf_counter()
+        h.close()
+        h, V = fresh(1.0)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        loss, _, _ = h.adjoint(steps, K, V, target)
+        h.synchronize()
+        t3 = time.perf_counter()
+        h.close()
+        if r > 0:
+            fwd.append(t1 - t0)
+            adj.append(t3 - t2)
+    f, a = float(np.median(fwd)), float(np.median(adj))
+    return {"workload": "adjoint2d", "mesh": "x".join(str(a_) for a_ in cfg["mesh"]["n"]), "particles_at_start": n0,
+            "steps": steps, "checkpoint_every": K, "forward_ms": round(f * 1e3, 3), "adjoint_ms": round(a * 1e3, 3),
+            "ratio": round(a / f, 3), "loss": loss}
+
+
+def calibrate_copy(n=1 << 27, reps=5):
+    """A plain device copy of n doubles (8n bytes read, 8n written), run
+    `reps` times: the known-size reference for the FETCH_SIZE/WRITE_SIZE
+    counters."""
+    import torch
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,11 +263,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3, help="steps re-run with per-kernel events")
     ap.add_argument("--batched-steps", type=int, default=5, help="steps of the 256-instance comparison (0: skip)")
+    ap.add_argument("--adjoint-reps", type=int, default=3, help="repetitions of the adjoint cost measurement (0: skip)")
+    ap.add_argument("--autotune", action="store_true", help="tune launch shapes instead of the shipped table")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU design is documented, not built); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     import torch
     from pfem2dealii_amd import ParticleHandler, workloads as W
@@ -190,7 +303,7 @@ def main():
 
     step()
     from pfem2dealii_amd.autotune import tune
-    tuning, _ = tune(h, step, signature={"workload": args.workload, "scale": args.scale})
+    tuning, _ = tune(h, step, signature={"workload": args.workload, "scale": args.scale}, force=args.autotune)
     for _ in range(args.warmup):
         step()
     h.synchronize()
@@ -213,28 +326,36 @@ def main():
     ms_per_step = gpu_ms / args.steps
     value = psteps / (gpu_ms * 1e-3)
 
-    # per-kernel durations over a few more steps (events around each launch)
+    # per-kernel durations over a few more steps (HIP events around each
+    # launch on the handle's stream), with the particle count before each step
     h.set_timing(True)
-    pc = h.count()
-    for _ in range(args.profile_steps):
+    pcounts = torch.zeros(args.profile_steps + 1, dtype=torch.int32, device="cuda")
+    for s in range(args.profile_steps):
+        h.count_async(pcounts.data_ptr() + 4 * s)
         step()
+    h.count_async(pcounts.data_ptr() + 4 * args.profile_steps)
     tim = h.timings()
     h.set_timing(False)
+    pcl = pcounts.cpu().tolist()
     nodes = mesh["X"].shape[1]
     b_alg = alg_bytes_per_particle_step(dim, nodes, psteps / args.steps)
-    step_ms = sum(ms for (_, ms) in tim.values()) / max(1, args.profile_steps)
     achieved_step = b_alg * psteps / args.steps / (ms_per_step * 1e-3) / 1e9
-    kern = {k: {"launches": n, "avg_ms": ms / n, "ms_per_step": ms / max(1, args.profile_steps)} for k, (n, ms) in tim.items()}
-    dom = max(kern.items(), key=lambda kv: kv[1]["ms_per_step"]) if kern else ("none", {"avg_ms": 0.0, "ms_per_step": 0.0})
-    # the dominant kernel's share of the step's algorithmic bytes is not
-    # separable; its achieved rate is the algorithmic bytes of the whole step
-    # over its own duration, an upper bound on its share.
-    roofline = {"bound": "hbm", "achieved": round(achieved_step, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved_step / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "whole step (%d kernels)" % len(kern),
-                "alg_bytes_per_particle_step": round(b_alg, 2), "particles_per_step": psteps / args.steps,
-                "dominant_kernel": dom[0], "dominant_avg_ms": round(dom[1]["avg_ms"], 4),
-                "dominant_ms_per_step": round(dom[1]["ms_per_step"], 4)}
+    nprof = max(1, args.profile_steps)
+    kern = {k: {"launches": n, "avg_ms": ms / n, "ms_per_step": ms / nprof} for k, (n, ms) in tim.items()}
+    dom = max(kern.items(), key=lambda kv: kv[1]["ms_per_step"]) if kern else ("none", {"avg_ms": 0.0, "launches": 0})
+    n_in = sum(pcl[:-1]) / float(nprof)
+    n_out = sum(pcl[1:]) / float(nprof)
+    alg = kernel_alg_bytes(dim, nodes, n_in, n_out)
+    dname = dom[0]
+    dbytes = alg.get(dname)
+    per_launch = dbytes / (dom[1]["launches"] / float(nprof)) if dbytes is not None else None
+    achieved = per_launch / (dom[1]["avg_ms"] * 1e-3) / 1e9 if per_launch else None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                "traffic": hbm_traffic(args.workload, args.scale, dname), "kernel": dname,
+                "alg_bytes_per_launch": per_launch, "avg_launch_ms": round(dom[1]["avg_ms"], 4),
+                "step": {"achieved": round(achieved_step, 1), "frac": round(achieved_step / HBM_PEAK_GBS, 4),
+                         "alg_bytes_per_particle_step": round(b_alg, 2), "particles_per_step": psteps / args.steps}}
     out = {
         "metric": "particle-steps/s plain vs batched-256 on 1 GPU at 1e-11 rel err; %HBM peak",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -246,11 +367,13 @@ def main():
                    "form": "plain", "tuning": tuning},
         "roofline": roofline,
         "kernels_ms_per_step": {k: round(v["ms_per_step"], 4) for k, v in sorted(kern.items())},
-        "wall_s": round(wall, 3), "setup_s": round(setup_s, 1), "profiled_particles": pc,
+        "wall_s": round(wall, 3), "setup_s": round(setup_s, 1),
     }
     h.close()
     if args.batched_steps > 0:
-        out["batched"] = batched_comparison(args.batched_steps, 1)
+        out["batched"] = batched_comparison(args.batched_steps, 1, force_tune=args.autotune)
+    if args.adjoint_reps > 0:
+        out["adjoint"] = adjoint_cost(args.adjoint_reps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload)
         out["cpu_baseline"]["host"] = {"nproc": os.cpu_count()}

@@ -143,13 +143,11 @@ int pfem2gpu_batch_layout(pfem2gpu_handle* h, int64_t* cell_offsets, int64_t* do
  * pointers (a first step with no pending correction runs eagerly). */
 int pfem2gpu_step(pfem2gpu_handle* h, const double* const* V, double* const* V_proj, int use_graph);
 
-/* Launch shape of the two main kernels (chosen by the autotuner in
- * pfem2dealii_amd/autotune.py): move workgroup size, particles per
- * wavefront run of the move work queue, settle workgroups. */
+/* Launch shape of the walk rounds and the settle kernel (chosen by the
+ * autotuner in pfem2dealii_amd/autotune.py). Neither changes results. */
 typedef struct {
-  int32_t move_block;   /* 64, 128 or 256 */
-  int32_t move_run;     /* multiple of 64 in [64, 8192] */
-  int32_t settle_grid;  /* [1, 65536] */
+  int32_t walk_grid;    /* workgroups of a walk round: multiple of 1024 in [1024, 65536] */
+  int32_t settle_grid;  /* workgroups of the settle kernel: [1, 65536] */
 } pfem2gpu_tuning;
 int pfem2gpu_set_tuning(pfem2gpu_handle* h, const pfem2gpu_tuning* t);
 int pfem2gpu_get_tuning(pfem2gpu_handle* h, pfem2gpu_tuning* t);

@@ -42,11 +42,11 @@ class _Params(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    """Launch shape of the move and settle kernels (pfem2gpu_tuning)."""
-    _fields_ = [("move_block", ctypes.c_int32), ("move_run", ctypes.c_int32), ("settle_grid", ctypes.c_int32)]
+    """Launch shape of the walk rounds and the settle kernel (pfem2gpu_tuning)."""
+    _fields_ = [("walk_grid", ctypes.c_int32), ("settle_grid", ctypes.c_int32)]
 
     def as_dict(self):
-        return {"move_block": self.move_block, "move_run": self.move_run, "settle_grid": self.settle_grid}
+        return {"walk_grid": self.walk_grid, "settle_grid": self.settle_grid}
 
 
 EXPORTS = [
@@ -324,8 +324,8 @@ class ParticleHandler:
         self.synchronize()
         return loss.value, gv0[:, :n0], gV
 
-    def set_tuning(self, move_block=256, move_run=512, settle_grid=2048):
-        t = Tuning(int(move_block), int(move_run), int(settle_grid))
+    def set_tuning(self, walk_grid=4096, settle_grid=2048):
+        t = Tuning(int(walk_grid), int(settle_grid))
         _check(self.lib, self.lib.pfem2gpu_set_tuning(self.h, ctypes.byref(t)), "set_tuning")
 
     def get_tuning(self):

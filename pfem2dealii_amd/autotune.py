@@ -1,19 +1,21 @@
 """Autotuner for the launch shape of the particle step.
 
-Tunes the three knobs of pfem2gpu_set_tuning by coordinate descent on the
-live handle (the measured steps advance the simulation; the launch shape does
-not change results, only timing):
-  move_block  workgroup size of the move kernel (64, 128, 256)
-  move_run    consecutive particles per wavefront run of its work queue
+Tunes the knobs of pfem2gpu_set_tuning by coordinate descent on the live
+handle (the measured steps advance the simulation; the launch shape does not
+change results, only timing):
+  walk_grid   workgroups of a walk round of the move
   settle_grid workgroups of the redistribution/projection kernel
 Each candidate is timed over `steps` steps with HIP events on the handle's
-stream after one untimed step. Results are cached per workload signature.
+stream after one untimed step. Results are cached per workload signature;
+`tuning_mi355x.json` beside this file holds the shapes found on MI355X for the
+benchmark workloads and is consulted first unless `force` is set.
 """
 import json
 import os
 
-MOVE_BLOCKS = (64, 128, 256)
-MOVE_RUNS = (128, 256, 512, 1024, 2048)
+TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_mi355x.json")
+
+WALK_GRIDS = (1024, 2048, 4096, 8192, 16384)
 SETTLE_GRIDS = (512, 1024, 2048, 4096, 8192)
 
 _cache = {}
@@ -34,10 +36,27 @@ def _time_steps(handle, step, steps):
     return a.elapsed_time(b) / steps
 
 
-def tune(handle, step, signature=None, steps=2, cache_path=None, log=None):
-    """Chooses and applies the fastest launch shape; returns (tuning, ms/step)."""
+def shipped(signature):
+    """The launch shape recorded for `signature` in the MI355X table, or None."""
+    try:
+        with open(TABLE) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = table.get(json.dumps(signature, sort_keys=True))
+    return dict(e["tuning"]) if e else None
+
+
+def tune(handle, step, signature=None, steps=2, cache_path=None, log=None, force=False):
+    """Chooses and applies the fastest launch shape; returns (tuning, ms/step).
+    ms/step is None when the shape came from the shipped table."""
     key = json.dumps(signature, sort_keys=True) if signature is not None else None
-    if key is not None:
+    if key is not None and not force:
+        t = shipped(signature)
+        if t is not None:
+            handle.set_tuning(**t)
+            return t, None
+    if key is not None and not force:
         if cache_path and not _cache and os.path.exists(cache_path):
             try:
                 _cache.update(json.load(open(cache_path)))
@@ -49,8 +68,8 @@ def tune(handle, step, signature=None, steps=2, cache_path=None, log=None):
     best = dict(handle.get_tuning())
     handle.set_tuning(**best)
     best_ms = _time_steps(handle, step, steps)
-    trials = [dict(best, move_block=b, move_run=r) for b in MOVE_BLOCKS for r in MOVE_RUNS]
-    for stage in ("move", "settle"):
+    trials = [dict(best, walk_grid=g) for g in WALK_GRIDS]
+    for stage in ("walk", "settle"):
         if stage == "settle":
             trials = [dict(best, settle_grid=g) for g in SETTLE_GRIDS]
         for t in trials:

@@ -113,13 +113,25 @@ struct pfem2gpu_handle {
   unsigned char* dirty = nullptr;  // per settle chunk: sums left to k_contrib_dirty
   bool contrib_valid = false;      // per (cell, vertex) sums match the particles
 
+  // walk queues of the split move (two, used in turn by the walk rounds)
+  double* qx[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  double* qxo[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  double* qv[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  double* qu[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  int* qcell[2] = {nullptr, nullptr};
+  int* qs[2] = {nullptr, nullptr};
+  int* qsrc[2] = {nullptr, nullptr};
+  long qcap = 0;
+  int* qn = nullptr;  // [qn_len] queue lengths of the walk rounds (max n_sub + 1 used)
+  int qn_len = 0;
+
   // identity recording for the adjoint (null when off)
   int* rec_src = nullptr;  // where the next move writes each slot's source particle
   int* tsrc = nullptr;     // [tsize] staging source indices
+  int* gord = nullptr;     // [tsize] grouping scratch of settle chunks that exceed the LDS stage
 
   // tuning (set by pfem2gpu_set_tuning; chosen by the autotuner)
-  int move_block = 256;     // workgroup size of the move kernel
-  int move_run = 512;       // consecutive particles per wavefront run
+  int walk_grid = 4096;     // workgroups of a walk round (multiple of kQStripes)
   int settle_grid = 2048;   // workgroups of the settle kernel
   // graph-captured harness step
   bool capturing = false;
@@ -137,6 +149,8 @@ struct pfem2gpu_handle {
 };
 
 namespace {
+
+constexpr int kSettleLds = 52 * 1024;  // LDS budget of a settle workgroup
 
 int grid_for(long n) {
   long g = (n + kBlock - 1) / kBlock;
@@ -261,6 +275,67 @@ void drop_graph(pfem2gpu_handle* h) {
   h->graph = nullptr;
 }
 
+// Workgroups of the in-cell move pass over n particles, and the walk-queue
+// stripe size for it: the particles of the ceil(grid / kQStripes)
+// workgroups that write to one stripe.
+long move_cell_grid(long n) { return std::max(1L, std::min((n + kBlock - 1) / kBlock, 1L << 20)); }
+
+long walk_qreg(long n) {
+  const long gc = move_cell_grid(n);
+  const long per_block = (long)kBlock * ((n + gc * kBlock - 1) / (gc * kBlock));
+  return ((gc + kQStripes - 1) / kQStripes) * per_block;
+}
+
+void free_queues(pfem2gpu_handle* h) {
+  for (int b = 0; b < 2; ++b) {
+    for (int k = 0; k < 3; ++k) {
+      dfree(h->qx[b][k]);
+      dfree(h->qxo[b][k]);
+      dfree(h->qv[b][k]);
+      dfree(h->qu[b][k]);
+    }
+    dfree(h->qcell[b]);
+    dfree(h->qs[b]);
+    dfree(h->qsrc[b]);
+  }
+  h->qcap = 0;
+}
+
+// Walk queues for `need` particles (every particle may leave its cell in
+// the first substep).
+int grow_queues(pfem2gpu_handle* h, long need) {
+  if (need <= h->qcap) return PFEM2GPU_OK;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  free_queues(h);
+  for (int b = 0; b < 2; ++b) {
+    for (int k = 0; k < h->dim; ++k) {
+      CHK(dalloc(&h->qx[b][k], need));
+      CHK(dalloc(&h->qxo[b][k], need));
+      CHK(dalloc(&h->qv[b][k], need));
+      CHK(dalloc(&h->qu[b][k], need));
+    }
+    CHK(dalloc(&h->qcell[b], need));
+    CHK(dalloc(&h->qs[b], need));
+    CHK(dalloc(&h->qsrc[b], need));
+  }
+  h->qcap = need;
+  return PFEM2GPU_OK;
+}
+
+DWalk dwalk(pfem2gpu_handle* h, int b) {
+  DWalk q;
+  for (int k = 0; k < 3; ++k) {
+    q.x[k] = h->qx[b][k];
+    q.xo[k] = h->qxo[b][k];
+    q.v[k] = h->qv[b][k];
+    q.u[k] = h->qu[b][k];
+  }
+  q.cell = h->qcell[b];
+  q.s = h->qs[b];
+  q.src = h->qsrc[b];
+  return q;
+}
+
 void free_particles(pfem2gpu_handle* h) {
   for (int k = 0; k < 3; ++k) {
     dfree(h->px[k]);
@@ -271,6 +346,8 @@ void free_particles(pfem2gpu_handle* h) {
   dfree(h->pcell);
   dfree(h->tcell);
   dfree(h->tsrc);
+  dfree(h->gord);
+  free_queues(h);
   h->cap = 0;
   h->tsize = 0;
 }
@@ -316,6 +393,7 @@ int grow(pfem2gpu_handle* h, long need, long keep, int capc) {
     h->pcell = ncell;
     h->cap = ncap;
   }
+  CHK(grow_queues(h, walk_qreg(h->cap) * kQStripes));
   if (tsize != h->tsize) {
     HIPCHK(hipStreamSynchronize(h->stream));
     for (int k = 0; k < 3; ++k) {
@@ -328,6 +406,8 @@ int grow(pfem2gpu_handle* h, long need, long keep, int capc) {
       CHK(dalloc(&h->tv[k], tsize));
     }
     CHK(dalloc(&h->tcell, tsize));
+    dfree(h->gord);
+    CHK(dalloc(&h->gord, tsize));
     if (h->tsrc) {
       dfree(h->tsrc);
       CHK(dalloc(&h->tsrc, tsize));
@@ -433,9 +513,6 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
   const DParts P = dparts(h);
   const DTmp T = dtmp(h);
   const long nmax_p = h->n_ub + n_add;
-  const int waves_per_block = h->move_block / 64;
-  long gm = (nmax_p + (long)h->move_run * waves_per_block - 1) / ((long)h->move_run * waves_per_block);
-  const int g = (int)std::max(1L, std::min(gm, 65536L));  // one wavefront per run of move_run particles
   const int* nptr = h->cell_start + h->nc;
   const int nch = h->nchunks;
   int* bcount = h->count;
@@ -451,11 +528,23 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
   LAUNCH(h, "k_fill", k_fill_int, grid_for(2L * nch), 0, reinterpret_cast<int*>(h->lb), 2L * nch, 0);
   if (advect)
     LAUNCH(h, "k_cellfield", k_cellfield<D>, grid_for((long)h->nc * h->nvc), 0, h->nc, h->cd, Vf, h->Vc);
-  t_begin(h, advect ? "k_move" : "k_rebin");
-  hipLaunchKernelGGL(k_move<D>, dim3(g), dim3(h->move_block), 0, h->stream, advect, m, h->Vc, nptr, n_add, P, dv,
-                     use_dv, h->cshift, h->capc, bcount, spill_base, spill_n, T, h->move_run);
-  t_end(h);
-  HIPCHK(hipGetLastError());
+  int rounds = 0;
+  if (advect)
+    for (const Prm& q : h->prm_inst) rounds = std::max(rounds, q.n_sub);
+  LAUNCH(h, "k_fill", k_fill_int, grid_for((long)h->qn_len * kQStripes), 0, h->qn, (long)h->qn_len * kQStripes, 0);
+  const long gc = move_cell_grid(nmax_p);
+  const long qreg = walk_qreg(nmax_p);
+  if (qreg * kQStripes > h->qcap) {
+    if (h->capturing) return fail(PFEM2GPU_ERR_STATE, "walk queues too small during graph capture");
+    CHK(grow_queues(h, qreg * kQStripes));
+  }
+  LAUNCH(h, advect ? "k_move" : "k_rebin", k_move_cell<D>, gc, 0, advect, m, h->Vc, nptr, n_add, P, dv, use_dv,
+         h->cshift, h->capc, bcount, spill_base, spill_n, T, dwalk(h, 0), h->qn, qreg);
+  const int gw = kQStripes * std::max(1, h->walk_grid / kQStripes);
+  for (int r = 0; r < rounds; ++r)
+    LAUNCH(h, "k_walk", k_move_walk<D>, gw, 0, m, h->Vc, dwalk(h, r & 1), h->qn + (long)r * kQStripes, qreg,
+           h->cshift, h->capc, bcount, spill_base, spill_n, T, dwalk(h, (r + 1) & 1),
+           h->qn + (long)(r + 1) * kQStripes);
   if (use_dv) h->pending_corr = false;
   LAUNCH(h, "k_spill", k_spill_count, grid_for(nmax_p), 0, spill_n, spill_base, h->tcell, h->cshift, scount);
   CHK(scan(h, scount, nch, h->sstart));
@@ -465,7 +554,7 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
   const int cap_on = advect, refill_on = advect;
   LAUNCH(h, "k_settle", k_settle<D>, gs, h->settle_smem, cap_on, refill_on, m, h->nc, h->cshift, h->pshift, h->nmax,
          h->nslots, h->capc, bcount, h->sstart, spill2_base, T, h->count + 3 * nch + 1, h->lb, h->d_err, h->cell_start,
-         P, Vf, h->contrib, h->dirty);
+         P, h->Vc, h->gord, h->contrib, h->dirty);
   h->n_ub = nmax_p + (advect ? h->refill_max : 0);
   h->contrib_valid = true;
   return PFEM2GPU_OK;
@@ -594,14 +683,15 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
   while ((1 << h->pshift) < p.P) ++h->pshift;
   h->pmax = 1 << h->pshift;
   // settle stage: 16 cells per workgroup (one chunk), as many staged
-  // particles as fit in 64 KiB of LDS beside one refill slot per part
+  // particles as fit in kSettleLds bytes of LDS beside one refill slot per
+  // part (three workgroups per CU)
   h->cshift = 4;
   h->cpw = 1 << h->cshift;
   h->nchunks = (h->nc + h->cpw - 1) >> h->cshift;
   h->nmax = 64;
   for (int nm = 64; nm <= 4096; nm += 64) {
     const int slots = nm + h->cpw * p.Pvalid;
-    if (settle_layout(dim, h->cpw, h->pmax, nm, slots).bytes > 64 * 1024) break;
+    if (settle_layout(dim, h->cpw, h->pmax, nm, slots).bytes > kSettleLds) break;
     h->nmax = nm;
   }
   h->nslots = h->nmax + h->cpw * p.Pvalid;
@@ -716,6 +806,8 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
       (rc = dalloc(&h->contrib, (size_t)h->nc * nvc * (dim + 1))) || (rc = dalloc(&h->d_rejected, 1)) ||
       (rc = dalloc(&h->dirty, h->nchunks)))
     return bail(rc);
+  for (const Prm& q : prms) h->qn_len = std::max(h->qn_len, q.n_sub + 2);
+  if ((rc = dalloc(&h->qn, (size_t)h->qn_len * kQStripes))) return bail(rc);
   for (int k = 0; k < dim; ++k)
     if ((rc = dalloc(&h->dV[k], h->ndofs)) || (rc = dalloc(&h->stageF[0][k], h->ndofs)) ||
         (rc = dalloc(&h->stageF[1][k], h->ndofs)))
@@ -823,6 +915,7 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
   dfree(h->sstart);
   dfree(h->lb);
   dfree(h->d_err);
+  dfree(h->qn);
   dfree(h->scan_blk);
   dfree(h->contrib);
   dfree(h->d_rejected);
@@ -1351,13 +1444,10 @@ int pfem2gpu_move_particles_traced(pfem2gpu_handle* h, const double* const* V, i
 
 int pfem2gpu_set_tuning(pfem2gpu_handle* h, const pfem2gpu_tuning* t) {
   if (!h || !t) return fail(PFEM2GPU_ERR_ARG, "null argument");
-  if (t->move_block != 64 && t->move_block != 128 && t->move_block != 256)
-    return fail(PFEM2GPU_ERR_ARG, "move_block must be 64, 128 or 256");
-  if (t->move_run < 64 || t->move_run > 8192 || (t->move_run % 64) != 0)
-    return fail(PFEM2GPU_ERR_ARG, "move_run must be a multiple of 64 in [64, 8192]");
+  if (t->walk_grid < kQStripes || t->walk_grid > 64 * kQStripes || (t->walk_grid % kQStripes) != 0)
+    return fail(PFEM2GPU_ERR_ARG, "walk_grid must be a multiple of 1024 in [1024, 65536]");
   if (t->settle_grid < 1 || t->settle_grid > 65536) return fail(PFEM2GPU_ERR_ARG, "settle_grid must be in [1, 65536]");
-  h->move_block = t->move_block;
-  h->move_run = t->move_run;
+  h->walk_grid = t->walk_grid;
   h->settle_grid = t->settle_grid;
   drop_graph(h);
   return PFEM2GPU_OK;
@@ -1365,8 +1455,7 @@ int pfem2gpu_set_tuning(pfem2gpu_handle* h, const pfem2gpu_tuning* t) {
 
 int pfem2gpu_get_tuning(pfem2gpu_handle* h, pfem2gpu_tuning* t) {
   if (!h || !t) return fail(PFEM2GPU_ERR_ARG, "null argument");
-  t->move_block = h->move_block;
-  t->move_run = h->move_run;
+  t->walk_grid = h->walk_grid;
   t->settle_grid = h->settle_grid;
   return PFEM2GPU_OK;
 }

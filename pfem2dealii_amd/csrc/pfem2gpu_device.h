@@ -65,6 +65,20 @@ struct DTmp {
   int* src;  // optional: particle index before the move
 };
 
+// Walk queue: particles that left their cell in a substep, with the state
+// the neighbour walk needs (position after the substep, reference position
+// before it, the substep's velocity, the substep index) and the carried
+// velocity and source index.
+struct DWalk {
+  double* x[3];
+  double* xo[3];
+  double* v[3];
+  double* u[3];
+  int* cell;
+  int* s;
+  int* src;
+};
+
 __device__ __forceinline__ int inst_of(const DMesh& m, int c) { return m.cell_inst ? m.cell_inst[c] : 0; }
 
 template <int D>
@@ -622,19 +636,49 @@ __device__ int walk(const DMesh& m, int c, const int* vid, const double* xi, con
   return -1;
 }
 
-// Neighbour walk from a box cell: the same steps and arithmetic as walk(),
-// with the current cell's vertex coordinates taken from its corners, the
-// vertex's candidate list from the per-cell table and box candidates tested
-// from the geometry table (one load each).
+// Closest vertex of a box cell along u, seen from xold: the reference's
+// argmax_v of a_v = sum_k vn_k (d_vk inv_m_v), vn = u / |u|, d_v = X_v - xold,
+// inv_m_v = 1 / |d_v|, strict > scan from v = 0
+// (src/pfem2particle.cpp:186-211).
+// The argmax is first decided from estimates e_v = (u . d_v) rsq(|d_v|^2)
+// rsq(|u|^2) with single-precision reciprocal square roots (relative error
+// below 1e-6, so |e_v - cos_v| < 3e-6): when the best estimate leads every
+// other one by more than kVertexGap the exact cosines, which lie within 1e-14
+// of cos_v, have the same unique maximum. Otherwise (near ties, zero or
+// non-finite velocity, a vertex at xold, magnitudes outside single
+// precision's range) the exact cosines are computed as in the reference.
+constexpr double kVertexGap = 1e-4;
+
 template <int D>
-__device__ int walk_box(const DMesh& m, const CellCache<D>& cc, const double* xi, const double* x, const double* u,
-                        double* xi_new) {
+__device__ __forceinline__ int closest_vertex_box(const CellCache<D>& cc, const double* xold, const double* u) {
   constexpr int NV = 1 << D;
-  double xold[D];
-  fwd_box<D>(cc, xi, xold);
   double n2 = 0.0;
 #pragma unroll
   for (int k = 0; k < D; ++k) n2 = n2 + u[k] * u[k];
+  bool sure = (n2 >= 1e-30) && (n2 <= 1e30);
+  const double ru = (double)__builtin_amdgcn_rsqf((float)n2);
+  double e1 = -4.0, e2 = -4.0;
+  int b1 = 0;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    double dot = 0.0, m2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double d = (((v >> k) & 1) ? cc.X1[k] : cc.X0[k]) - xold[k];
+      dot = dot + u[k] * d;
+      m2 = m2 + d * d;
+    }
+    sure = sure && (m2 >= 1e-30) && (m2 <= 1e30);
+    const double e = dot * (double)__builtin_amdgcn_rsqf((float)m2) * ru;
+    if (e > e1) {
+      e2 = e1;
+      e1 = e;
+      b1 = v;
+    } else if (e > e2) {
+      e2 = e;
+    }
+  }
+  if (sure && e1 - e2 > kVertexGap) return b1;
   const double inv_un = 1.0 / sqrt(n2);
   double vn[D];
 #pragma unroll
@@ -660,6 +704,63 @@ __device__ int walk_box(const DMesh& m, const CellCache<D>& cc, const double* xi
       bestv = v;
     }
   }
+  return bestv;
+}
+
+// Reference position of x in candidate cell c2 (box: from the geometry
+// table; otherwise Newton); false when the inverse map fails.
+template <int D>
+__device__ __forceinline__ bool inv_candidate(const DMesh& m, int c2, const double* x, double* xn) {
+  const double* g = m.geom + (size_t)c2 * 3 * D;
+  if (g[0] == g[0]) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) xn[k] = div_by(x[k] - g[k], g[D + k] - g[k], g[2 * D + k]);
+    return true;
+  }
+  int vid2[1 << D];
+  load_ids<D>(m.cv, c2, vid2);
+  return inv_map_newton<D>(m, vid2, x, xn);
+}
+
+// Neighbour walk from a box cell: the same result as walk(), with the current
+// cell's vertex coordinates taken from its corners and the vertex's candidate
+// list from the per-cell table. The candidates (the cells around the closest
+// vertex except the current one) are tested in list order first; the
+// reference's order (descending alignment of the centre direction with the
+// vertex->particle direction, ties by cell id) only decides between two or
+// more containing candidates, so it is computed only then.
+template <int D>
+__device__ int walk_box(const DMesh& m, const CellCache<D>& cc, const double* xi, const double* x, const double* u,
+                        double* xi_new) {
+  double xold[D];
+  fwd_box<D>(cc, xi, xold);
+  const int bestv = closest_vertex_box<D>(cc, xold, u);
+  const int2 oc = m.cvo[(size_t)cc.c * (1 << D) + bestv];
+  const int o0 = oc.x, nn = oc.y;
+  if (nn > 8) return -2;  // caller falls back to the generic walk
+  unsigned hits = 0;
+  int first = -1;
+  double xf[D];
+  for (int j = 0; j < nn; ++j) {
+    const int c2 = m.v2c[o0 + j];
+    if (c2 == cc.c) continue;  // the current cell already failed the same test
+    double xn[D];
+    if (inv_candidate<D>(m, c2, x, xn) && inside<D>(xn)) {
+      hits |= 1u << j;
+      if (first < 0) {
+        first = j;
+#pragma unroll
+        for (int k = 0; k < D; ++k) xf[k] = xn[k];
+      }
+    }
+  }
+  if (hits == 0) return -1;
+  if ((hits & (hits - 1)) == 0) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) xi_new[k] = xf[k];
+    return m.v2c[o0 + first];
+  }
+  // two or more containing candidates: the first in the reference's order
   double vtp[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) vtp[k] = x[k] - (((bestv >> k) & 1) ? cc.X1[k] : cc.X0[k]);
@@ -669,9 +770,6 @@ __device__ int walk_box(const DMesh& m, const CellCache<D>& cc, const double* xi
   const double inv_t = 1.0 / sqrt(t2);
 #pragma unroll
   for (int k = 0; k < D; ++k) vtp[k] = vtp[k] * inv_t;
-  const int2 oc = m.cvo[(size_t)cc.c * NV + bestv];
-  const int o0 = oc.x, nn = oc.y;
-  if (nn > 8) return -2;  // caller falls back to the generic walk
   double dots[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -692,23 +790,9 @@ __device__ int walk_box(const DMesh& m, const CellCache<D>& cc, const double* xi
       }
     }
     used |= 1u << bj;
-    const int c2 = m.v2c[o0 + bj];
-    if (c2 == cc.c) continue;  // the current cell already failed the same test
-    const double* g = m.geom + (size_t)c2 * 3 * D;
-    double xn[D];
-    bool ok;
-    if (g[0] == g[0]) {
-#pragma unroll
-      for (int k = 0; k < D; ++k) xn[k] = div_by(x[k] - g[k], g[D + k] - g[k], g[2 * D + k]);
-      ok = true;
-    } else {
-      int vid2[NV];
-      load_ids<D>(m.cv, c2, vid2);
-      ok = inv_map_newton<D>(m, vid2, x, xn);
-    }
-    if (ok && inside<D>(xn)) {
-#pragma unroll
-      for (int k = 0; k < D; ++k) xi_new[k] = xn[k];
+    if ((hits >> bj) & 1u) {
+      const int c2 = m.v2c[o0 + bj];
+      inv_candidate<D>(m, c2, x, xi_new);
       return c2;
     }
   }

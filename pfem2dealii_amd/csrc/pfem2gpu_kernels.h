@@ -167,176 +167,262 @@ __global__ void k_spill_scatter(const int* __restrict__ spill_n, long spill_base
   }
 }
 
-// Advances every particle through the n_sub Euler substeps with location
-// (move_particles, src/pfem2particlehandler.cpp:141-171) and appends it to the
-// bucket of its destination chunk of 2^cshift cells in T. `advect` = 0 turns
-// the pass into a pure re-binning (used by insertion). A pending velocity
-// correction is applied from the pre-move cell and reference position, as
-// correct_particle_velocity() precedes move_particles().
-//
-// Each wavefront owns a run of `run` consecutive particles (sorted by
-// cell, so lanes share cells and their cached data comes from L1/L2) and
-// processes it as a work queue: every lane runs its particle's Euler
-// substeps until it leaves its cell or finishes; the lanes that left their
-// cell then run the neighbour walk together; lanes whose particle is done
-// take the next particle of the run. The walk thus executes with many lanes
-// active instead of once per substep for a few.
+// -------------------------------------------------------------------- move
+// move_particles (src/pfem2particlehandler.cpp:141-171): per particle the
+// correction of the carried velocity (fused, :111-139), then n_sub explicit
+// Euler substeps x <- x + dt_sub u(x) (:150-163), each followed by the
+// current-cell test and, when the particle left its cell, the neighbour walk
+// (:418-492). Scheduled in lockstep: k_move_cell runs every particle's
+// substeps while it stays in its cell and queues the particles that leave
+// it; each k_move_walk round takes the queue, walks every queued particle to
+// its new cell together, continues its substeps and queues it again if it
+// leaves once more. A particle walks at most once per substep, so n_sub
+// rounds drain the queue. Every particle ends in the bucket of its
+// destination chunk (2^cshift consecutive cells), or in the spill region when
+// the bucket is full; the order inside a bucket is not deterministic and the
+// settle kernel does not depend on it.
+
+// The walk queue is split into kQStripes stripes of `qreg` entries with one
+// counter each; a workgroup writes to stripe blockIdx.x % kQStripes, so the
+// counters see little contention and a stripe never overflows (it receives
+// at most the particles of the workgroups mapped to it).
+constexpr int kQStripes = 1024;
+
+// Appends one particle per lane with key >= 0 to its chunk bucket (or the
+// spill region) and queues the lanes with walk set. Wave-uniform.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_move(int advect, DMesh m, const double* __restrict__ Vc,
-                                                const int* __restrict__ n_ptr, int n_add, DParts P, DField dV,
-                                                int use_dv, int cshift, int capc, int* __restrict__ bcount,
-                                                long spill_base, int* __restrict__ spill_n, DTmp T, int run) {
-  const int n = *n_ptr + n_add;
+__device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, int src, const double* x,
+                                          const double* xo, const double* v, const double* u, int capc,
+                                          int* __restrict__ bcount, long spill_base, int* __restrict__ spill_n,
+                                          DTmp& T, DWalk& Q, int* __restrict__ qn, long qreg) {
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (1ULL << lane) - 1ULL;
-  const long gw = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const long nw = ((long)gridDim.x * blockDim.x) >> 6;
-  for (long wb = gw * run; wb < n; wb += nw * run) {
-    const int wend = (int)min((long)n, wb + run);
-    int next = (int)wb + 64;  // first particle of the run not yet taken
-    int p = (int)wb + lane;
-    bool fresh = true;        // lane must load particle p
+  const unsigned long long wm = __ballot(walk);
+  if (wm) {
+    const int stripe = blockIdx.x % kQStripes;
+    const int leader = __ffsll((unsigned long long)wm) - 1;
+    int qb = 0;
+    if (lane == leader) qb = atomicAdd(&qn[stripe], __popcll(wm));
+    qb = __shfl(qb, leader, 64);
+    if (walk) {
+      const long q = (long)stripe * qreg + qb + __popcll(wm & below);
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        Q.x[k][q] = x[k];
+        Q.xo[k][q] = xo[k];
+        Q.v[k][q] = v[k];
+        Q.u[k][q] = u[k];
+      }
+      Q.cell[q] = cell;
+      Q.s[q] = s;
+      Q.src[q] = src;
+    }
+  }
+  const int slot = wave_group_add(bcount, key);
+  if (key >= 0) {
+    long dst = (long)key * capc + slot;
+    if (slot >= capc) dst = spill_base + atomicAdd(spill_n, 1);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      T.x[k][dst] = x[k];
+      T.v[k][dst] = v[k];
+    }
+    T.cell[dst] = cell;
+    if (T.src) T.src[dst] = src;
+  }
+}
+
+// Substeps s, s+1, ... of a particle in cell cc until n_sub or until it
+// leaves the cell (returns true, x is then the position after the failing
+// substep, xi the reference position before it and u its velocity).
+template <int D>
+__device__ __forceinline__ bool substeps_in_cell(const DMesh& m, const double* __restrict__ Vc, const CellCache<D>& cc,
+                                                 int n_sub, double dt_sub, int& s, double* x, double* xi, double* u) {
+  for (; s < n_sub; ++s) {
+    gather_cached<D>(Vc, cc, xi, u);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double t = u[k] * dt_sub;
+      x[k] = x[k] + t;
+    }
+    double xn[D];
+    if (inv_cached<D>(m, cc, x, xn) && inside<D>(xn)) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) xi[k] = xn[k];
+    } else {
+      return true;
+    }
+  }
+  return false;
+}
+
+// In-cell pass, one particle per thread. `advect` = 0 turns it into a pure
+// re-binning (used by insertion). A pending velocity correction is applied
+// from the pre-move cell and reference position first, as
+// correct_particle_velocity would have done before the move.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const double* __restrict__ Vc,
+                                                     const int* __restrict__ n_ptr, int n_add, DParts P, DField dV,
+                                                     int use_dv, int cshift, int capc, int* __restrict__ bcount,
+                                                     long spill_base, int* __restrict__ spill_n, DTmp T, DWalk Q,
+                                                     int* __restrict__ qn, long qreg) {
+  const int n = *n_ptr + n_add;
+  for (long base = (long)blockIdx.x * blockDim.x; base < n; base += (long)gridDim.x * blockDim.x) {
+    const int p = (int)(base + threadIdx.x);
+    int key = -1, s = 0;
+    bool walk = false;
     CellCache<D> cc;
-    double x[D], xi[D], v[D];
-    double dt_sub = 0.0;
-    int n_sub = 0, s = 0;
-    bool walking = false, dead = false;
-    for (;;) {
-      if (fresh) {
-        fresh = false;
-        dead = false;
-        walking = false;
-        s = 0;
-        if (p < wend) {
-          const int c = P.cell[p];
-          if (c < 0) {
-            dead = true;
-          } else {
-            load_cell<D>(m, Vc, c, cc, advect != 0);
-            const Prm& pr = m.prm[inst_of(m, c)];
-            dt_sub = pr.dt_sub;
-            n_sub = advect ? pr.n_sub : 0;
+    cc.c = -1;
+    double x[D], xi[D], v[D], u[D];
 #pragma unroll
-            for (int k = 0; k < D; ++k) x[k] = P.x[k][p];
-            inv_cached<D>(m, cc, x, xi);
-            snap_cached<D>(m, pr, cc, x, xi);
-#pragma unroll
-            for (int k = 0; k < D; ++k) v[k] = P.v[k][p];
-            if (use_dv) {
-              double corr[D];
-              int dof[1 << D];
-              load_ids<D>(m.cd, c, dof);
-              gather<D>(dV, dof, xi, corr);
-#pragma unroll
-              for (int k = 0; k < D; ++k) v[k] = v[k] + corr[k];
-            }
-          }
-        } else {
-          p = -1;
-        }
-      }
-      if (__ballot(p >= 0) == 0ULL) break;
-      // Euler substeps in the current cell until the particle leaves it
-      if (p >= 0 && !dead) {
-        while (s < n_sub) {
-          double u[D];
-          gather_cached<D>(Vc, cc, xi, u);
-#pragma unroll
-          for (int k = 0; k < D; ++k) {
-            const double t = u[k] * dt_sub;
-            x[k] = x[k] + t;
-          }
-          double xn[D];
-          if (inv_cached<D>(m, cc, x, xn) && inside<D>(xn)) {
-#pragma unroll
-            for (int k = 0; k < D; ++k) xi[k] = xn[k];
-            ++s;
-          } else {
-            walking = true;
-            break;
-          }
-        }
-      }
-      // neighbour walk of the lanes that left their cell, all together
-      if (walking) {
-        double u[D], xn[D];
-        gather_cached<D>(Vc, cc, xi, u);  // the substep's advecting velocity (xi is pre-substep)
-        const int c2 = walk_cached<D>(m, cc, xi, x, u, xn);
-        walking = false;
-        if (c2 < 0) {
-          dead = true;
-        } else {
-          load_cell<D>(m, Vc, c2, cc, true);
-#pragma unroll
-          for (int k = 0; k < D; ++k) xi[k] = xn[k];
-          ++s;
-        }
-      }
-      const bool done = (p >= 0) && !walking && (dead || s >= n_sub);
-      // append the moved particle to its destination chunk's bucket (one
-      // atomic per distinct chunk among the finishing lanes), or to the
-      // spill area when the bucket is full
-      const int key = (done && !dead) ? (cc.c >> cshift) : -1;
-      const int slot = wave_group_add(bcount, key);
-      if (key >= 0) {
-        long dst = (long)key * capc + slot;
-        if (slot >= capc) dst = spill_base + atomicAdd(spill_n, 1);
+    for (int k = 0; k < D; ++k) {
+      x[k] = 0.0;
+      xi[k] = 0.0;
+      v[k] = 0.0;
+      u[k] = 0.0;
+    }
+    if (p < n) {
+      const int c = P.cell[p];
+      if (c >= 0) {
+        load_cell<D>(m, Vc, c, cc, advect != 0);
+        const Prm& pr = m.prm[inst_of(m, c)];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          T.x[k][dst] = x[k];
-          T.v[k][dst] = v[k];
+          x[k] = P.x[k][p];
+          v[k] = P.v[k][p];
         }
-        T.cell[dst] = cc.c;
-        if (T.src) T.src[dst] = p;
+        inv_cached<D>(m, cc, x, xi);
+        snap_cached<D>(m, pr, cc, x, xi);
+        if (use_dv) {
+          double corr[D];
+          int dof[1 << D];
+          load_ids<D>(m.cd, c, dof);
+          gather<D>(dV, dof, xi, corr);
+#pragma unroll
+          for (int k = 0; k < D; ++k) v[k] = v[k] + corr[k];
+        }
+        walk = substeps_in_cell<D>(m, Vc, cc, advect ? pr.n_sub : 0, pr.dt_sub, s, x, xi, u);
+        if (!walk) key = c >> cshift;
       }
-      const unsigned long long dm = __ballot(done);
-      if (done) {
-        p = next + __popcll(dm & below);
-        fresh = true;
-      }
-      next += __popcll(dm);
     }
+    move_emit<D>(key, walk, cc.c, s, p, x, xi, v, u, capc, bcount, spill_base, spill_n, T, Q, qn, qreg);
+  }
+}
+
+// Walk round: workgroup b takes stripe b % kQStripes of the input queue
+// (the grid is a multiple of kQStripes) and writes to the same stripe of the
+// output queue.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __restrict__ Vc, DWalk Qin,
+                                                     const int* __restrict__ qn_in, long qreg, int cshift, int capc,
+                                                     int* __restrict__ bcount, long spill_base,
+                                                     int* __restrict__ spill_n, DTmp T, DWalk Qout,
+                                                     int* __restrict__ qn_out) {
+  const int stripe = blockIdx.x % kQStripes;
+  const int per = gridDim.x / kQStripes;
+  const int nq = qn_in[stripe];
+  for (long base = (long)(blockIdx.x / kQStripes) * blockDim.x; base < nq; base += (long)per * blockDim.x) {
+    const int i = (int)(stripe * qreg + base + threadIdx.x);
+    const bool in = base + threadIdx.x < nq;
+    int key = -1, s = 0, src = 0;
+    bool walk = false;
+    CellCache<D> cc;
+    cc.c = -1;
+    double x[D], xi[D], v[D], u[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      x[k] = 0.0;
+      xi[k] = 0.0;
+      v[k] = 0.0;
+      u[k] = 0.0;
+    }
+    if (in) {
+      const int c = Qin.cell[i];
+      s = Qin.s[i];
+      src = Qin.src[i];
+      double xo[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        x[k] = Qin.x[k][i];
+        xo[k] = Qin.xo[k][i];
+        v[k] = Qin.v[k][i];
+        u[k] = Qin.u[k][i];
+      }
+      load_cell<D>(m, Vc, c, cc, false);
+      const Prm& pr = m.prm[inst_of(m, c)];
+      double xn[D];
+      const int c2 = walk_cached<D>(m, cc, xo, x, u, xn);
+      if (c2 >= 0) {
+        load_cell<D>(m, Vc, c2, cc, true);
+#pragma unroll
+        for (int k = 0; k < D; ++k) xi[k] = xn[k];
+        ++s;
+        walk = substeps_in_cell<D>(m, Vc, cc, pr.n_sub, pr.dt_sub, s, x, xi, u);
+        if (!walk) key = c2 >> cshift;
+      }
+    }
+    move_emit<D>(key, walk, cc.c, s, src, x, xi, v, u, capc, bcount, spill_base, spill_n, T, Qout, qn_out, qreg);
   }
 }
 
 // ---------------------------------------------------------------- settle
 // check_particle_distribution (src/pfem2particlehandler.cpp:355-410) on the
-// chunk buckets written by k_move, fused with the per-(cell, vertex)
+// chunk buckets written by the move, fused with the per-(cell, vertex)
 // projection sums of project_particle_fields (:191-199). A workgroup owns
-// one chunk of 2^cshift consecutive cells; its particles are the chunk's
-// bucket plus its share of the spill region.
-// Chunks are taken in ticket order. The workgroup stages the chunk's
-// particles in LDS with their reference position, part and canonical key and
-// counts them per (cell, part); the final count of each cell is survivors
-// min(count, max+1) per part plus one refill per empty part of a non-empty
-// cell. The chunk's output offset comes from a decoupled look-back over the
-// chunk totals (no separate count pass and scan). Each particle is ranked
-// inside its (cell, part) group by key; survivors and refills get their final
-// slot (cell base + part base + rank, i.e. (part, key) order); the chunk is
-// written out coalesced in that order, with the new cell_start, and the
-// (cell, vertex) sums are accumulated sequentially in the same order.
-// A chunk with more than `nmax` particles takes the same steps from global
-// memory, recomputing parts and keys, and leaves its projection sums to
-// k_contrib_dirty.
+// one chunk of 2^cshift consecutive cells at a time; its particles are the
+// chunk's bucket plus its share of the spill region. Chunks are taken in
+// ticket order (the next ticket and the next chunk's bucket and spill extents
+// are read while the current chunk is processed).
+//  A. the chunk's cell data (box geometry, advecting field, instance, part
+//     codes) and its particles (position, velocity, cell) are read in one
+//     round of loads; each particle's reference position, part and
+//     canonical key are computed and counted per (cell, part) in LDS;
+//  B. per (cell, part): survivors min(count, max+1) plus one refill per
+//     empty valid part of a non-empty cell; one workgroup-wide scan gives
+//     the final and the raw (grouping) offsets of every (cell, part); the
+//     chunk total is published for the decoupled look-back;
+//  D. particles are grouped by (cell, part) and ranked inside their group by
+//     key, which gives every survivor its final slot (cell base + part base
+//     + rank, i.e. (part, key) order); refills are built in their slots;
+//  C. only then is the chunk's output offset resolved from the totals of
+//     the chunks before it (decoupled look-back), so that the wait overlaps
+//     the chunk's own work;
+//  E. the chunk is written out coalesced in final order, with the new
+//     cell_start, and the (cell, vertex) sums are accumulated sequentially
+//     in the same order.
+// A chunk with more than `nmax` particles resolves its offset in B and takes
+// the remaining steps with its particles in global memory (grouping through
+// the `gord` scratch array, ranking over the group only); it leaves its
+// projection sums to k_contrib_dirty.
 struct SettleLayout {
-  int off_x, off_v, off_xi, off_key, off_pl, off_perm, off_ord, off_cnt, off_base, off_roff, off_rfill, off_cell,
-      bytes;
+  int off_xi, off_v, off_perm;    // [nslots]: staged particles, then refills
+  int off_xr;                     // [nslots - nmax]: refill positions
+  int off_pl;                     // [nslots]: cell << 16 | part
+  int off_key, off_ord;           // [nmax]
+  int off_cnt, off_base, off_roff, off_fill, off_code;  // [cpw * pmax] per (cell, part)
+  int off_geo, off_vc;            // [cpw] per cell: 3*D and 2^D*D doubles
+  int off_cell;                   // [cpw] x 4 ints per cell, misc
+  int bytes;
 };
 
 __host__ __device__ inline SettleLayout settle_layout(int D, int cpw, int pmax, int nmax, int nslots) {
   SettleLayout L;
   int o = 0;
-  L.off_x = o;
+  L.off_xi = o;
   o += 8 * D * nslots;
   L.off_v = o;
   o += 8 * D * nslots;
-  L.off_xi = o;
-  o += 8 * D * nslots;
+  L.off_xr = o;
+  o += 8 * D * (nslots - nmax);
   L.off_key = o;
   o += 8 * nmax;
-  L.off_pl = o;
-  o += 4 * nslots;
+  L.off_geo = o;
+  o += 8 * 3 * D * cpw;
+  L.off_vc = o;
+  o += 8 * (1 << D) * D * cpw;
   L.off_perm = o;
+  o += 4 * nslots;
+  L.off_pl = o;
   o += 4 * nslots;
   L.off_ord = o;
   o += 4 * nmax;
@@ -346,10 +432,13 @@ __host__ __device__ inline SettleLayout settle_layout(int D, int cpw, int pmax, 
   o += 4 * cpw * pmax;
   L.off_roff = o;
   o += 4 * cpw * pmax;
-  L.off_rfill = o;
+  L.off_fill = o;
   o += 4 * cpw * pmax;
+  L.off_code = o;
+  o += 4 * cpw * pmax;
+  o = (o + 15) & ~15;
   L.off_cell = o;
-  o += 4 * (4 * cpw + 8);  // per-cell count, raw offset, final offset, total; misc
+  o += 4 * (4 * cpw + 16);  // per-cell count, instance, final offset, spare; misc; scan partials
   L.bytes = o;
   return L;
 }
@@ -413,137 +502,254 @@ __device__ __forceinline__ unsigned lb_prefix(unsigned long long* lb, int ch, in
   return prefix;
 }
 
+// Inclusive-to-exclusive scan of one 64-bit value per thread over the
+// workgroup; `part` holds one value per wavefront. Returns the exclusive
+// prefix and sets *total.
+__device__ __forceinline__ unsigned long long block_scan_u64(unsigned long long v, unsigned long long* part,
+                                                             unsigned long long* total) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  unsigned long long incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned long long t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) part[wave] = incl;
+  __syncthreads();
+  unsigned long long wbase = 0, tot = 0;
+  const int nw = blockDim.x >> 6;
+  for (int w = 0; w < nw; ++w) {
+    const unsigned long long s = part[w];
+    if (w < wave) wbase += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return wbase + incl - v;
+}
+
+// Reference position of x in chunk cell lc from the LDS geometry (box) or
+// the generic path, snapped as in recover_xi_c.
+template <int D>
+__device__ __forceinline__ void settle_xi(const DMesh& m, const Prm& pr, int c, const double* __restrict__ geo,
+                                          const double* x, double* xi) {
+  CellCache<D> cc;
+  cc.c = c;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    cc.X0[k] = geo[k];
+    cc.X1[k] = geo[D + k];
+    cc.RH[k] = geo[2 * D + k];
+  }
+  cc.box = (cc.X0[0] == cc.X0[0]);
+  inv_cached<D>(m, cc, x, xi);
+  snap_cached<D>(m, pr, cc, x, xi);
+}
+
+constexpr int kSettlePre = 3;  // particles per thread loaded with the chunk's cell data
+
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill, DMesh m, int nc, int cshift, int pshift,
                                                   int nmax, int nslots, int capc, const int* __restrict__ bcount,
                                                   const int* __restrict__ sstart, long spill2_base, DTmp T,
                                                   int* __restrict__ ticket, unsigned long long* __restrict__ lb,
-                                                  int* __restrict__ err, int* __restrict__ cs_out, DParts P, DField Vf,
+                                                  int* __restrict__ err, int* __restrict__ cs_out, DParts P,
+                                                  const double* __restrict__ Vc, int* __restrict__ gord,
                                                   double* __restrict__ contrib, unsigned char* __restrict__ dirty) {
   constexpr int NV = 1 << D;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int cpw = 1 << cshift;
   const int pmax = 1 << pshift;  // part buckets per cell, padded to a power of two
   const SettleLayout L = settle_layout(D, cpw, pmax, nmax, nslots);
-  double* s_x = reinterpret_cast<double*>(smem_raw + L.off_x);
-  double* s_v = reinterpret_cast<double*>(smem_raw + L.off_v);
   double* s_xi = reinterpret_cast<double*>(smem_raw + L.off_xi);
+  double* s_v = reinterpret_cast<double*>(smem_raw + L.off_v);
+  double* s_xr = reinterpret_cast<double*>(smem_raw + L.off_xr);
   unsigned long long* s_key = reinterpret_cast<unsigned long long*>(smem_raw + L.off_key);
-  int* s_pl = reinterpret_cast<int*>(smem_raw + L.off_pl);
+  double* s_geo = reinterpret_cast<double*>(smem_raw + L.off_geo);
+  double* s_vc = reinterpret_cast<double*>(smem_raw + L.off_vc);
   int* s_perm = reinterpret_cast<int*>(smem_raw + L.off_perm);
+  int* s_pl = reinterpret_cast<int*>(smem_raw + L.off_pl);
   int* s_ord = reinterpret_cast<int*>(smem_raw + L.off_ord);
   int* s_cnt = reinterpret_cast<int*>(smem_raw + L.off_cnt);
   int* s_base = reinterpret_cast<int*>(smem_raw + L.off_base);
   int* s_roff = reinterpret_cast<int*>(smem_raw + L.off_roff);
-  int* s_rfill = reinterpret_cast<int*>(smem_raw + L.off_rfill);
-  int* s_ccnt = reinterpret_cast<int*>(smem_raw + L.off_cell);
-  int* s_coff = s_ccnt + cpw;  // raw (staged) offset of each cell
-  int* s_cfin = s_coff + cpw;  // final offset of each cell inside the chunk
-  int* s_misc = s_cfin + cpw;  // [0] refill slots, [1] ticket, [2] chunk total, [3] chunk prefix
+  int* s_fill = reinterpret_cast<int*>(smem_raw + L.off_fill);
+  int* s_code = reinterpret_cast<int*>(smem_raw + L.off_code);
+  int* s_ccnt = reinterpret_cast<int*>(smem_raw + L.off_cell);  // particles per cell
+  int* s_inst = s_ccnt + cpw;                                   // instance per cell
+  int* s_cfin = s_inst + cpw;                                   // final offset per cell (chunk-relative)
+  // [0] refill slots, [2] chunk total, [3] chunk prefix, [4] chunk, [5] bucket count, [6] spill start, [7] spill end
+  int* s_misc = s_cfin + 2 * cpw;
+  unsigned long long* s_part = reinterpret_cast<unsigned long long*>(s_misc + 8);  // [4] scan partials
 
   const int nchunks = (nc + cpw - 1) >> cshift;
+  const int per_thread = ((pmax << cshift) + blockDim.x - 1) / blockDim.x;
+  if (threadIdx.x == 0) {
+    const int ch = atomicAdd(ticket, 1);
+    s_misc[4] = ch;
+    if (ch < nchunks) {
+      s_misc[5] = bcount[ch];
+      s_misc[6] = sstart[ch];
+      s_misc[7] = sstart[ch + 1];
+    }
+  }
+  __syncthreads();
   for (;;) {
-    if (threadIdx.x == 0) s_misc[1] = atomicAdd(ticket, 1);
-    __syncthreads();
-    const int ch = s_misc[1];
+    const int ch = s_misc[4];
     if (ch >= nchunks) break;
+    int next = 0;
+    if (threadIdx.x == 0) next = atomicAdd(ticket, 1);  // the next chunk, drawn early
     const int c0 = ch << cshift;
     const int c1 = min(nc, c0 + cpw);
     const int ncell = c1 - c0;
-    const int nb = min(bcount[ch], capc);
-    const int ns = sstart[ch + 1] - sstart[ch];
-    const int n = nb + ns;
+    const int nb = min(s_misc[5], capc);
+    const int n = nb + s_misc[7] - s_misc[6];
     const long bucket0 = (long)ch * capc;
-    const long spill0 = spill2_base + sstart[ch];
+    const long spill0 = spill2_base + s_misc[6];
     const bool staged = n <= nmax;
-    for (int i = threadIdx.x; i < ncell * pmax; i += blockDim.x) {
-      s_cnt[i] = 0;
-      s_rfill[i] = 0;
+    // A. cell data and particles of the chunk, one round of loads
+    for (int i = threadIdx.x; i < ncell * 3 * D; i += blockDim.x) s_geo[i] = m.geom[(size_t)c0 * 3 * D + i];
+    if (do_refill)
+      for (int i = threadIdx.x; i < ncell * NV * D; i += blockDim.x) s_vc[i] = Vc[(size_t)c0 * NV * D + i];
+    for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
+      s_inst[i] = inst_of(m, c0 + i);
+      s_ccnt[i] = 0;
     }
-    for (int i = threadIdx.x; i < cpw; i += blockDim.x) s_ccnt[i] = 0;
+    for (int t = threadIdx.x; t < ncell * pmax; t += blockDim.x) {
+      s_cnt[t] = 0;
+      s_fill[t] = 0;
+      s_code[t] = m.ptab[(inst_of(m, c0 + (t >> pshift)) << pshift) + (t & (pmax - 1))];
+    }
     if (threadIdx.x == 0) s_misc[0] = 0;
-    __syncthreads();
-    // 1. reference position, part and key of every particle of the chunk
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      const long i = chunk_index(j, nb, bucket0, spill0);
-      const int c = T.cell[i];
-      const int lc = c - c0;
-      const Prm& pr = m.prm[inst_of(m, c)];
-      double x[D], xi[D];
+    int pc[kSettlePre];
+    double px[kSettlePre][D], pv[kSettlePre][D];
 #pragma unroll
-      for (int k = 0; k < D; ++k) x[k] = T.x[k][i];
-      recover_xi_c<D>(m, pr, c, x, xi);
+    for (int u = 0; u < kSettlePre; ++u) {
+      const int j = threadIdx.x + u * blockDim.x;
+      pc[u] = -1;
+      if (j < n) {
+        const long i = chunk_index(j, nb, bucket0, spill0);
+        pc[u] = T.cell[i];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          px[u][k] = T.x[k][i];
+          pv[u][k] = T.v[k][i];
+        }
+      }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x, u = 0; j < n; j += blockDim.x, ++u) {
+      int c;
+      double x[D], v[D];
+      if (u < kSettlePre) {
+        c = pc[0];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          x[k] = px[0][k];
+          v[k] = pv[0][k];
+        }
+#pragma unroll
+        for (int w = 0; w + 1 < kSettlePre; ++w) {
+          pc[w] = pc[w + 1];
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            px[w][k] = px[w + 1][k];
+            pv[w][k] = pv[w + 1][k];
+          }
+        }
+      } else {
+        const long i = chunk_index(j, nb, bucket0, spill0);
+        c = T.cell[i];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          x[k] = T.x[k][i];
+          v[k] = T.v[k][i];
+        }
+      }
+      const int lc = c - c0;
+      const Prm& pr = m.prm[s_inst[lc]];
+      double xi[D];
+      settle_xi<D>(m, pr, c, s_geo + lc * 3 * D, x, xi);
       const int part = part_lin<D>(pr, xi, nullptr);
       atomicAdd(&s_cnt[(lc << pshift) + part], 1);
       atomicAdd(&s_ccnt[lc], 1);
       if (staged) {
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          s_x[k * nslots + j] = x[k];
-          s_v[k * nslots + j] = T.v[k][i];
           s_xi[k * nslots + j] = xi[k];
+          s_v[k * nslots + j] = v[k];
         }
         s_pl[j] = (lc << 16) | part;
         s_key[j] = particle_key<D>(x);
       }
     }
     __syncthreads();
-    // 2. per (cell, part): survivors + refill; per cell the exclusive prefixes
-    //    over parts of the final (part bases) and raw (groups) counts
-    for (int t = threadIdx.x; t < ncell * pmax; t += blockDim.x) {
-      const int lc = t >> pshift, pp = t & (pmax - 1);
-      const int c = c0 + lc;
-      const int inst = inst_of(m, c);
+    // B. final (survivors + refill) and raw counts per (cell, part), scanned
+    //    over the chunk in (cell, part) order; the chunk total is published
+    unsigned long long mine = 0;
+    for (int e = 0; e < per_thread; ++e) {
+      const int t = threadIdx.x * per_thread + e;
+      if (t >= ncell * pmax) break;
+      const int lc = t >> pshift;
       const int k = s_cnt[t];
-      const int maxpp = m.prm[inst].maxpp;
-      const int kept = apply_cap ? min(k, maxpp + 1) : k;
-      const bool valid = m.ptab[(inst << pshift) + pp] < 0;
-      const int refill = (do_refill && s_ccnt[lc] > 0 && k == 0 && valid) ? 1 : 0;
-      s_base[t] = kept + refill;  // padding parts have k == 0 and are not valid
+      const int kept = apply_cap ? min(k, m.prm[s_inst[lc]].maxpp + 1) : k;
+      const int refill = (do_refill && s_ccnt[lc] > 0 && k == 0 && s_code[t] < 0) ? 1 : 0;
+      s_base[t] = kept + refill;
+      mine += ((unsigned long long)(kept + refill) << 32) | (unsigned)k;
     }
-    __syncthreads();
-    for (int lc = threadIdx.x; lc < ncell; lc += blockDim.x) {
-      int run = 0, raw = 0;
-      for (int pp = 0; pp < pmax; ++pp) {
-        const int t = (lc << pshift) + pp;
-        const int a = s_base[t];
-        s_base[t] = run;
-        run += a;
-        s_roff[t] = raw;
-        raw += s_cnt[t];
-      }
-      s_cfin[lc] = run;                               // cell total for now
+    unsigned long long tot = 0;
+    unsigned long long run = block_scan_u64(mine, s_part, &tot);
+    for (int e = 0; e < per_thread; ++e) {
+      const int t = threadIdx.x * per_thread + e;
+      if (t >= ncell * pmax) break;
+      const int fin = s_base[t];
+      s_base[t] = (int)(run >> 32);
+      s_roff[t] = (int)(run & 0xffffffffULL);
+      if ((t & (pmax - 1)) == 0) s_cfin[t >> pshift] = (int)(run >> 32);
+      run += ((unsigned long long)fin << 32) | (unsigned)s_cnt[t];
     }
-    __syncthreads();
+    const int nout = (int)(tot >> 32);
     if (threadIdx.x == 0) {
-      int run = 0, raw = 0;
-      for (int lc = 0; lc < ncell; ++lc) {
-        const int tot = s_cfin[lc];
-        s_cfin[lc] = run;
-        run += tot;
-        s_coff[lc] = raw;
-        raw += s_ccnt[lc];
+      lb_publish(lb, ch, ch == 0 ? 2u : 1u, (unsigned)nout);
+      if (ch == 0) s_misc[3] = 0;
+      if (next < nchunks) {  // header of the next chunk, read while this one runs
+        s_misc[5] = bcount[next];
+        s_misc[6] = sstart[next];
+        s_misc[7] = sstart[next + 1];
       }
-      s_misc[2] = run;
-      lb_publish(lb, ch, ch == 0 ? 2u : 1u, (unsigned)run);
-      const unsigned prefix = ch == 0 ? 0u : lb_prefix(lb, ch, err);
-      if (ch > 0) lb_publish(lb, ch, 2u, prefix + (unsigned)run);
-      s_misc[3] = (int)prefix;
+      if (!staged && ch > 0) {  // the global-memory path writes as it goes
+        const unsigned prefix = lb_prefix(lb, ch, err);
+        lb_publish(lb, ch, 2u, prefix + (unsigned)nout);
+        s_misc[3] = (int)prefix;
+      }
     }
     __syncthreads();
-    const int o0 = s_misc[3];
-    for (int lc = threadIdx.x; lc < ncell; lc += blockDim.x) cs_out[c0 + lc] = o0 + s_cfin[lc];
-    if (c1 == nc && threadIdx.x == 0) cs_out[nc] = o0 + s_misc[2];
-    if (staged) {
-      // group the staged particles by (cell, part)
-      for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        const int lc = s_pl[j] >> 16;
-        const int t = (lc << pshift) + (s_pl[j] & 0xffff);
-        s_ord[s_coff[lc] + s_roff[t] + atomicAdd(&s_rfill[t], 1)] = j;
+    // D. group by (cell, part), rank by key, place survivors and refills
+    //    (staged chunks in LDS; the output offset is not needed yet)
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      int t;
+      if (staged) {
+        t = ((s_pl[j] >> 16) << pshift) + (s_pl[j] & 0xffff);
+      } else {
+        const long i = chunk_index(j, nb, bucket0, spill0);
+        const int c = T.cell[i];
+        const int lc = c - c0;
+        const Prm& pr = m.prm[s_inst[lc]];
+        double x[D], xi[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) x[k] = T.x[k][i];
+        settle_xi<D>(m, pr, c, s_geo + lc * 3 * D, x, xi);
+        t = (lc << pshift) + part_lin<D>(pr, xi, nullptr);
       }
-      __syncthreads();
+      const int pos = s_roff[t] + atomicAdd(&s_fill[t], 1);
+      if (staged)
+        s_ord[pos] = j;
+      else
+        gord[chunk_index(pos, nb, bucket0, spill0)] = j;
     }
-    // 3. rank inside (cell, part) and final slot of every particle
+    __syncthreads();
+    const int o0n = staged ? 0 : s_misc[3];  // output offset (global-memory path only)
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
       int lc, part, r = 0;
       if (staged) {
@@ -551,131 +757,166 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
         part = s_pl[j] & 0xffff;
         const unsigned long long key = s_key[j];
         const int t = (lc << pshift) + part;
-        const int a = s_coff[lc] + s_roff[t], b = a + s_cnt[t];
-        for (int u = a; u < b; ++u) {
-          const int q = s_ord[u];
-          if (q == j) continue;
-          const unsigned long long kq = s_key[q];
+        const int a = s_roff[t], b = a + s_cnt[t];
+        for (int q = a; q < b; ++q) {
+          const int o = s_ord[q];
+          if (o == j) continue;
+          const unsigned long long kq = s_key[o];
           if (kq < key) {
             ++r;
           } else if (kq == key) {
             double xq[D], vq[D], xj[D], vj[D];
+            const long iq = chunk_index(o, nb, bucket0, spill0), ij = chunk_index(j, nb, bucket0, spill0);
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-              xq[k] = s_x[k * nslots + q];
-              vq[k] = s_v[k * nslots + q];
-              xj[k] = s_x[k * nslots + j];
+              xq[k] = T.x[k][iq];
+              vq[k] = s_v[k * nslots + o];
+              xj[k] = T.x[k][ij];
               vj[k] = s_v[k * nslots + j];
             }
-            if (tie_less<D>(xq, vq, xj, vj, q, j)) ++r;
+            if (tie_less<D>(xq, vq, xj, vj, o, j)) ++r;
           }
         }
       } else {
-        // more particles than the LDS stage holds: recompute from global
         const long i = chunk_index(j, nb, bucket0, spill0);
         const int c = T.cell[i];
         lc = c - c0;
-        const Prm& pr = m.prm[inst_of(m, c)];
+        const Prm& pr = m.prm[s_inst[lc]];
         double xj[D], vj[D], xi[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
           xj[k] = T.x[k][i];
           vj[k] = T.v[k][i];
         }
-        recover_xi_c<D>(m, pr, c, xj, xi);
+        settle_xi<D>(m, pr, c, s_geo + lc * 3 * D, xj, xi);
         part = part_lin<D>(pr, xi, nullptr);
         const unsigned long long key = particle_key<D>(xj);
-        for (int q = 0; q < n; ++q) {
-          if (q == j) continue;
-          const long iq = chunk_index(q, nb, bucket0, spill0);
-          if (T.cell[iq] != c) continue;
-          double xq[D], vq[D], xiq[D];
+        const int t = (lc << pshift) + part;
+        const int a = s_roff[t], b = a + s_cnt[t];
+        for (int q = a; q < b; ++q) {
+          const int o = gord[chunk_index(q, nb, bucket0, spill0)];
+          if (o == j) continue;
+          const long iq = chunk_index(o, nb, bucket0, spill0);
+          double xq[D], vq[D];
 #pragma unroll
           for (int k = 0; k < D; ++k) {
             xq[k] = T.x[k][iq];
             vq[k] = T.v[k][iq];
           }
-          recover_xi_c<D>(m, pr, c, xq, xiq);
-          if (part_lin<D>(pr, xiq, nullptr) != part) continue;
           const unsigned long long kq = particle_key<D>(xq);
-          if (kq < key || (kq == key && tie_less<D>(xq, vq, xj, vj, q, j))) ++r;
+          if (kq < key || (kq == key && tie_less<D>(xq, vq, xj, vj, o, j))) ++r;
         }
       }
-      const int c = c0 + lc;
-      if (apply_cap && r > m.prm[inst_of(m, c)].maxpp) continue;
-      const int fpos = s_cfin[lc] + s_base[(lc << pshift) + part] + r;
+      if (apply_cap && r > m.prm[s_inst[lc]].maxpp) continue;
+      const int fpos = s_base[(lc << pshift) + part] + r;
       if (staged) {
         s_perm[fpos] = j;
       } else {
         const long i = chunk_index(j, nb, bucket0, spill0);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          P.x[k][o0 + fpos] = T.x[k][i];
-          P.v[k][o0 + fpos] = T.v[k][i];
+          P.x[k][o0n + fpos] = T.x[k][i];
+          P.v[k][o0n + fpos] = T.v[k][i];
         }
-        P.cell[o0 + fpos] = c;
-        if (P.src) P.src[o0 + fpos] = T.src[i];
+        P.cell[o0n + fpos] = c0 + lc;
+        if (P.src) P.src[o0n + fpos] = T.src[i];
       }
     }
-    // 4. refills: one particle at the centre of each empty part of a
-    //    non-empty cell, velocity interpolated from the advecting field
+    // refills: one particle at the centre of each empty part of a non-empty
+    // cell, velocity interpolated from the advecting field
     if (do_refill) {
       for (int t = threadIdx.x; t < ncell * pmax; t += blockDim.x) {
         const int lc = t >> pshift, pp = t & (pmax - 1);
+        const int code = s_code[t];
+        if (s_cnt[t] != 0 || s_ccnt[lc] == 0 || code >= 0) continue;  // occupied, empty cell, or no part
+        const Prm& pr = m.prm[s_inst[lc]];
         const int c = c0 + lc;
-        if (s_cnt[t] != 0 || s_ccnt[lc] == 0) continue;
-        const int inst = inst_of(m, c);
-        const int code = m.ptab[(inst << pshift) + pp];
-        if (code >= 0) continue;  // not a refillable part (or padding)
-        const Prm& pr = m.prm[inst];
-        int pi[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) pi[d] = (code >> (8 * d)) & 0xff;
         double xi[D], x[D], v[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) xi[k] = ((double)pi[k] + 1.0 / 2) * pr.h[k];
-        int vid[NV], dof[NV];
-        load_ids<D>(m.cv, c, vid);
-        load_ids<D>(m.cd, c, dof);
-        fwd<D>(m, vid, xi, x);
-        gather<D>(Vf, dof, xi, v);
-        const int fpos = s_cfin[lc] + s_base[t];
-        if (staged) {
-          const int slot = n + atomicAdd(&s_misc[0], 1);
+        for (int k = 0; k < D; ++k) xi[k] = ((double)((code >> (8 * k)) & 0xff) + 1.0 / 2) * pr.h[k];
+        const double* g = s_geo + lc * 3 * D;
+        if (g[0] == g[0]) {
+          CellCache<D> cc;
 #pragma unroll
           for (int k = 0; k < D; ++k) {
-            s_x[k * nslots + slot] = x[k];
-            s_v[k * nslots + slot] = v[k];
-            s_xi[k * nslots + slot] = xi[k];
+            cc.X0[k] = g[k];
+            cc.X1[k] = g[D + k];
           }
-          s_pl[slot] = (lc << 16) | pp;
+          fwd_box<D>(cc, xi, x);
+        } else {
+          int vid[NV];
+          load_ids<D>(m.cv, c, vid);
+          fwd<D>(m, vid, xi, x);
+        }
+        double N[NV];
+        shape<D>(xi, N);
+        const double* f = s_vc + lc * NV * D;
+#pragma unroll
+        for (int k = 0; k < D; ++k) v[k] = 0.0;
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+          for (int k = 0; k < D; ++k) v[k] = v[k] + N[i] * f[i * D + k];
+        const int fpos = s_base[t];
+        if (staged) {
+          const int r = atomicAdd(&s_misc[0], 1);
+          const int slot = nmax + r;
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            s_xi[k * nslots + slot] = xi[k];
+            s_v[k * nslots + slot] = v[k];
+            s_xr[k * (nslots - nmax) + r] = x[k];
+          }
           s_perm[fpos] = slot;
+          s_pl[slot] = (lc << 16) | pp;
         } else {
 #pragma unroll
           for (int k = 0; k < D; ++k) {
-            P.x[k][o0 + fpos] = x[k];
-            P.v[k][o0 + fpos] = v[k];
+            P.x[k][o0n + fpos] = x[k];
+            P.v[k][o0n + fpos] = v[k];
           }
-          P.cell[o0 + fpos] = c;
-          if (P.src) P.src[o0 + fpos] = -1 - pp;
+          P.cell[o0n + fpos] = c;
+          if (P.src) P.src[o0n + fpos] = -1 - pp;
         }
       }
     }
+    // C. output offset of a staged chunk by decoupled look-back, resolved
+    //    after its own grouping and ranking so that the predecessors have
+    //    most likely published their prefixes by now
+    if (staged && threadIdx.x == 0 && ch > 0) {
+      const unsigned prefix = lb_prefix(lb, ch, err);
+      lb_publish(lb, ch, 2u, prefix + (unsigned)nout);
+      s_misc[3] = (int)prefix;
+    }
     __syncthreads();
+    const int o0 = s_misc[3];
+    for (int lc = threadIdx.x; lc < ncell; lc += blockDim.x) cs_out[c0 + lc] = o0 + s_cfin[lc];
+    if (c1 == nc && threadIdx.x == 0) cs_out[nc] = o0 + nout;
     if (staged) {
-      // 5. coalesced write-out in final order
-      const int nout = s_misc[2];
+      // E. coalesced write-out in final order
       for (int t = threadIdx.x; t < nout; t += blockDim.x) {
         const int slot = s_perm[t];
+        if (slot < nmax) {
+          const long i = chunk_index(slot, nb, bucket0, spill0);
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-          P.x[k][o0 + t] = s_x[k * nslots + slot];
-          P.v[k][o0 + t] = s_v[k * nslots + slot];
+          for (int k = 0; k < D; ++k) {
+            P.x[k][o0 + t] = T.x[k][i];
+            P.v[k][o0 + t] = s_v[k * nslots + slot];
+          }
+          P.cell[o0 + t] = c0 + (s_pl[slot] >> 16);
+          if (P.src) P.src[o0 + t] = T.src[i];
+        } else {
+          const int r = slot - nmax;  // refill
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            P.x[k][o0 + t] = s_xr[k * (nslots - nmax) + r];
+            P.v[k][o0 + t] = s_v[k * nslots + slot];
+          }
+          P.cell[o0 + t] = c0 + (s_pl[slot] >> 16);
+          if (P.src) P.src[o0 + t] = -1 - (s_pl[slot] & 0xffff);
         }
-        P.cell[o0 + t] = c0 + (s_pl[slot] >> 16);
-        if (P.src) P.src[o0 + t] = slot < n ? T.src[chunk_index(slot, nb, bucket0, spill0)] : -1 - (s_pl[slot] & 0xffff);
       }
-      // 6. per (cell, vertex) sums, sequential in final order
+      // per (cell, vertex) sums, sequential in final order
       for (int t = threadIdx.x; t < ncell * NV; t += blockDim.x) {
         const int lc = t / NV, iv = t - lc * NV;
         const int c = c0 + lc;
@@ -702,7 +943,10 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
         e[D] = W;
       }
     }
-    if (threadIdx.x == 0) dirty[ch] = staged ? 0 : 1;
+    if (threadIdx.x == 0) {
+      dirty[ch] = staged ? 0 : 1;
+      s_misc[4] = next;
+    }
     __syncthreads();
   }
 }

@@ -5,7 +5,7 @@ differences. Marked gpu.
 Workload: the adjoint channel (Q1 mesh of [0,2]x[0,1] at 1/8 of 256x128 per
 direction, 16 seeded uniform particles per cell), Poiseuille amplitude 1.0,
 20 steps of move, final projection, loss 1/2 |P - P(amplitude 1.05)|^2,
-checkpoints every 5 steps; 32 seeded random directions, epsilon 1e-6."""
+checkpoints every 5 steps; 64 seeded random directions, epsilon 1e-6."""
 import sys
 
 import numpy as np
@@ -80,14 +80,14 @@ def test_adjoint_matches_cpu_reference_gradient(adjoint_case):
 
 
 def test_adjoint_matches_finite_differences(adjoint_case):
-    _say("adjoint vs central finite differences, 32 directions, eps 1e-6")
+    _say("adjoint vs central finite differences, 64 directions, eps 1e-6")
     c = adjoint_case
     ref = c["ref"]
     ref.grad(c["V"], c["v0"], c["target"])  # records the frozen path
     rng = np.random.default_rng(7)
     eps = 1e-6
     worst = 0.0
-    for _ in range(32):
+    for _ in range(64):
         dV = rng.standard_normal(c["V"].shape)
         dv = rng.standard_normal(c["v0"].shape)
         gdot = float((c["gV"] * dV).sum() + (c["gv0"] * dv).sum())

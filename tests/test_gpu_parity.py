@@ -141,3 +141,48 @@ def test_device_pointers(gpu_available):
         g.h.synchronize()
         assert_fields_match(out.cpu().numpy(), Vo)
         assert_particles_match(g.download(), o.download())
+
+
+def _lattice_on_faces(mesh, m):
+    """(m+1)^2 particles per cell at xi = (i/m, j/m), i, j = 0..m: on faces,
+    at vertices and at sub-cell centres, all exactly representable."""
+    nc = mesh["cell_vertices"].shape[0]
+    t = np.arange(m + 1) / float(m)
+    a, b = np.meshgrid(t, t, indexing="ij")
+    xi = np.stack([np.tile(a.ravel(), nc), np.tile(b.ravel(), nc)])
+    cells = np.repeat(np.arange(nc, dtype=np.int32), (m + 1) ** 2)
+    return W.seed_from_unit(mesh, cells, xi), cells
+
+
+@pytest.mark.parametrize("u", [(1.0, 1.0), (1.0, 0.0), (-1.0, 0.5), (0.0, -1.0), (0.0, 0.0)])
+def test_walk_ties_on_faces_and_vertices(gpu_available, u):
+    """Constant fields with power-of-two substeps on a 16x8 mesh: particles
+    start and land exactly on faces and vertices (two or more containing
+    candidates), and axis-aligned or diagonal velocities give exact ties in
+    the closest-vertex cosines, so every exact branch of the walk runs."""
+    _say("walk ties, constant field %s, 16x8 mesh, 3 steps" % (u,))
+    mesh = W.structured_mesh([16, 8], [0.0, 0.0], [2.0, 1.0])
+    nd = mesh["X"].shape[1]
+    V = np.stack([np.full(nd, u[0]), np.full(nd, u[1])])
+    cfg = {"mesh": mesh, "V": V, "params": dict(n_sub=4, dt=0.125, max_per_part=5, particles_per_cell=[4, 4])}
+    x, cells = _lattice_on_faces(mesh, 4)
+    _run_pair(cfg, 3, x=x, cells=cells)
+
+
+def test_dense_chunks_exceed_lds_stage(gpu_available):
+    """150 particles in each of the first 40 cells of a 16x8 channel: their
+    chunks hold more particles than the settle kernel stages in LDS, so the
+    insertion re-binning, the capped redistribution and the projection of
+    those chunks run through the global-memory path."""
+    _say("dense chunks beyond the LDS stage, 16x8 channel, 2 steps")
+    cfg = W.config("channel2d", scale=0.25)
+    mesh = cfg["mesh"]
+    nc = mesh["cell_vertices"].shape[0]
+    rng = np.random.default_rng(21)
+    counts = np.zeros(nc, dtype=np.int64)
+    counts[:40] = 150
+    counts[40:] = rng.integers(0, 12, nc - 40)
+    cells = np.repeat(np.arange(nc, dtype=np.int32), counts)
+    xi = rng.random((2, cells.shape[0]))
+    x = W.seed_from_unit(mesh, cells, xi)
+    _run_pair(cfg, 2, x=x, cells=cells)

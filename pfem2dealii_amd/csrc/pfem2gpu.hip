@@ -682,19 +682,19 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
   h->pshift = 0;
   while ((1 << h->pshift) < p.P) ++h->pshift;
   h->pmax = 1 << h->pshift;
-  // settle stage: 16 cells per workgroup (one chunk), as many staged
-  // particles as fit in kSettleLds bytes of LDS beside one refill slot per
-  // part (three workgroups per CU)
+  // settle stage: 16 cells per workgroup (one chunk); as many final slots
+  // (survivors and refills of a chunk) as fit in kSettleLds bytes of LDS
+  // (three workgroups per CU), and at most as many incoming particles as the
+  // workgroup holds in registers
   h->cshift = 4;
   h->cpw = 1 << h->cshift;
   h->nchunks = (h->nc + h->cpw - 1) >> h->cshift;
-  h->nmax = 64;
-  for (int nm = 64; nm <= 4096; nm += 64) {
-    const int slots = nm + h->cpw * p.Pvalid;
-    if (settle_layout(dim, h->cpw, h->pmax, nm, slots).bytes > kSettleLds) break;
-    h->nmax = nm;
+  h->nslots = 64;
+  for (int ns = 64; ns <= 8192; ns += 64) {
+    if (settle_layout(dim, h->cpw, h->pmax, 0, ns).bytes > kSettleLds) break;
+    h->nslots = ns;
   }
-  h->nslots = h->nmax + h->cpw * p.Pvalid;
+  h->nmax = std::min(h->nslots, kBlock * (dim == 2 ? SettlePre<2>::n : SettlePre<3>::n));
   h->settle_smem = settle_layout(dim, h->cpw, h->pmax, h->nmax, h->nslots).bytes;
   if (h->settle_smem > 64 * 1024) {
     delete h;

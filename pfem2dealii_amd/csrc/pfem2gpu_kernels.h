@@ -395,50 +395,46 @@ __global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __r
 // the `gord` scratch array, ranking over the group only); it leaves its
 // projection sums to k_contrib_dirty.
 struct SettleLayout {
-  int off_xi, off_v, off_perm;    // [nslots]: staged particles, then refills
-  int off_xr;                     // [nslots - nmax]: refill positions
-  int off_pl;                     // [nslots]: cell << 16 | part
-  int off_key, off_ord;           // [nmax]
-  int off_cnt, off_base, off_roff, off_fill, off_code;  // [cpw * pmax] per (cell, part)
-  int off_geo, off_vc;            // [cpw] per cell: 3*D and 2^D*D doubles
-  int off_cell;                   // [cpw] x 4 ints per cell, misc
+  int off_fin;                  // [nslots] final order: x[D], v[D], xi[D] (doubles), src (int); before
+                                // the final placement the same bytes hold key (u64), ord (int), xi[D]
+                                // and v[D] (doubles) in arrival order
+  int off_cnt, off_base, off_roff;  // [cpw * pmax] per (cell, part)
+  int off_cbuf[2];              // cell data of the current / next chunk
+  int off_geo, off_vc, off_prm, off_inst;  // offsets inside a cell data buffer
+  int off_cell;                 // per-cell count and final offset; misc; scan partials
   int bytes;
 };
 
 __host__ __device__ inline SettleLayout settle_layout(int D, int cpw, int pmax, int nmax, int nslots) {
+  (void)nmax;
   SettleLayout L;
   int o = 0;
-  L.off_xi = o;
-  o += 8 * D * nslots;
-  L.off_v = o;
-  o += 8 * D * nslots;
-  L.off_xr = o;
-  o += 8 * D * (nslots - nmax);
-  L.off_key = o;
-  o += 8 * nmax;
-  L.off_geo = o;
-  o += 8 * 3 * D * cpw;
-  L.off_vc = o;
-  o += 8 * (1 << D) * D * cpw;
-  L.off_perm = o;
-  o += 4 * nslots;
-  L.off_pl = o;
-  o += 4 * nslots;
-  L.off_ord = o;
-  o += 4 * nmax;
+  L.off_fin = o;
+  o += (8 * 3 * D + 4) * nslots;
+  o = (o + 15) & ~15;
   L.off_cnt = o;
   o += 4 * cpw * pmax;
   L.off_base = o;
   o += 4 * cpw * pmax;
   L.off_roff = o;
   o += 4 * cpw * pmax;
-  L.off_fill = o;
-  o += 4 * cpw * pmax;
-  L.off_code = o;
-  o += 4 * cpw * pmax;
+  int c = 0;
+  L.off_geo = c;
+  c += 8 * 3 * D * cpw;
+  L.off_vc = c;
+  c += 8 * (1 << D) * D * cpw;
+  L.off_prm = c;
+  c += (int)sizeof(Prm) * cpw;
+  L.off_inst = c;
+  c += 4 * cpw;
+  c = (c + 15) & ~15;
   o = (o + 15) & ~15;
+  L.off_cbuf[0] = o;
+  o += c;
+  L.off_cbuf[1] = o;
+  o += c;
   L.off_cell = o;
-  o += 4 * (4 * cpw + 16);  // per-cell count, instance, final offset, spare; misc; scan partials
+  o += 4 * (2 * cpw + 32);  // per-cell count, final offset; misc; scan partials
   L.bytes = o;
   return L;
 }
@@ -531,6 +527,43 @@ __device__ __forceinline__ unsigned long long block_scan_u64(unsigned long long 
 
 // Reference position of x in chunk cell lc from the LDS geometry (box) or
 // the generic path, snapped as in recover_xi_c.
+// The same exclusive prefix looked up by one whole wavefront: lane i reads
+// the status word of chunk ch-1-i, so one round of loads covers 64
+// predecessors; the prefix is the sum over the lanes up to the nearest chunk
+// with an inclusive prefix (status 2), once none of them is still unpublished.
+// Every lane of the calling wavefront must call it; all get the prefix.
+__device__ __forceinline__ unsigned lb_prefix_wave(unsigned long long* lb, int ch, int* err) {
+  const int lane = threadIdx.x & 63;
+  unsigned prefix = 0;
+  int base = ch - 1;
+  long spins = 0;
+  while (base >= 0) {
+    const int k = base - lane;
+    const unsigned long long w =
+        k >= 0 ? __hip_atomic_load(&lb[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (2ULL << 32);
+    const unsigned st = (unsigned)(w >> 32);
+    const unsigned long long m2 = __ballot(st == 2);
+    const unsigned long long m0 = __ballot(st == 0);
+    const int first2 = m2 ? __ffsll((unsigned long long)m2) - 1 : 64;
+    const unsigned long long need = first2 >= 63 ? ~0ULL : ((1ULL << (first2 + 1)) - 1ULL);
+    if (m0 & need) {
+      if (++spins > (1L << 26)) {
+        if (lane == 0) atomicOr(err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    unsigned part = lane <= first2 ? (unsigned)(w & 0xffffffffULL) : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+    prefix += part;
+    if (first2 < 64) break;
+    base -= 64;
+  }
+  return prefix;
+}
+
 template <int D>
 __device__ __forceinline__ void settle_xi(const DMesh& m, const Prm& pr, int c, const double* __restrict__ geo,
                                           const double* x, double* xi) {
@@ -549,8 +582,72 @@ __device__ __forceinline__ void settle_xi(const DMesh& m, const Prm& pr, int c, 
 
 constexpr int kSettlePre = 3;  // particles per thread loaded with the chunk's cell data
 
+// Particles per thread a staged chunk holds in registers between the
+// classification and the final placement.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill, DMesh m, int nc, int cshift, int pshift,
+struct SettlePre {
+  static constexpr int n = D == 2 ? 3 : 2;
+};
+
+// Cell data of a chunk (at most 16 cells: box geometry, advecting field,
+// instance and its parameters), carried in registers from its loads to its
+// stores into an LDS cell data buffer, one value per thread and array.
+constexpr int kPrmWords = (int)(sizeof(Prm) / 8);
+static_assert(sizeof(Prm) % 8 == 0 && 16 * kPrmWords <= kBlock, "Prm words per chunk exceed the workgroup");
+
+template <int D>
+struct CellRegs {
+  double geo;
+  double vc[2];
+  double prm;
+  int inst;
+};
+
+template <int D>
+__device__ __forceinline__ void cells_load(const DMesh& m, const double* __restrict__ Vc, int do_refill, int c0,
+                                           int ncell, CellRegs<D>& r) {
+  constexpr int NV = 1 << D;
+  const int t = threadIdx.x;
+  r.geo = t < ncell * 3 * D ? m.geom[(size_t)c0 * 3 * D + t] : 0.0;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int i = t + q * kBlock;
+    r.vc[q] = (do_refill && i < ncell * NV * D) ? Vc[(size_t)c0 * NV * D + i] : 0.0;
+  }
+  r.prm = t < ncell * kPrmWords ? reinterpret_cast<const double*>(&m.prm[inst_of(m, c0 + t / kPrmWords)])[t % kPrmWords]
+                                : 0.0;
+  r.inst = t < ncell ? inst_of(m, c0 + t) : 0;
+}
+
+template <int D>
+__device__ __forceinline__ void cells_store(const CellRegs<D>& r, int ncell, double* geo, double* vc, Prm* prm,
+                                            int* inst) {
+  constexpr int NV = 1 << D;
+  const int t = threadIdx.x;
+  if (t < ncell * 3 * D) geo[t] = r.geo;
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+    if (t + q * kBlock < ncell * NV * D) vc[t + q * kBlock] = r.vc[q];
+  if (t < ncell * kPrmWords) reinterpret_cast<double*>(prm)[t] = r.prm;
+  if (t < ncell) inst[t] = r.inst;
+}
+
+// Part index lin of a part code: pi_k = (lin / stride_k) % Q_k with x
+// slowest; valid when every pi_k < q_k.
+template <int D>
+__device__ __forceinline__ bool part_code(const Prm& pr, int lin, int* pi) {
+  bool ok = true;
+#pragma unroll
+  for (int k = D - 1; k >= 0; --k) {
+    pi[k] = lin % pr.Q[k];
+    lin /= pr.Q[k];
+    ok = ok && pi[k] < pr.q[k];
+  }
+  return ok;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap, int do_refill, DMesh m, int nc, int cshift, int pshift,
                                                   int nmax, int nslots, int capc, const int* __restrict__ bcount,
                                                   const int* __restrict__ sstart, long spill2_base, DTmp T,
                                                   int* __restrict__ ticket, unsigned long long* __restrict__ lb,
@@ -558,76 +655,47 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
                                                   const double* __restrict__ Vc, int* __restrict__ gord,
                                                   double* __restrict__ contrib, unsigned char* __restrict__ dirty) {
   constexpr int NV = 1 << D;
+  constexpr int PRE = SettlePre<D>::n;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int cpw = 1 << cshift;
   const int pmax = 1 << pshift;  // part buckets per cell, padded to a power of two
   const SettleLayout L = settle_layout(D, cpw, pmax, nmax, nslots);
-  double* s_xi = reinterpret_cast<double*>(smem_raw + L.off_xi);
-  double* s_v = reinterpret_cast<double*>(smem_raw + L.off_v);
-  double* s_xr = reinterpret_cast<double*>(smem_raw + L.off_xr);
-  unsigned long long* s_key = reinterpret_cast<unsigned long long*>(smem_raw + L.off_key);
-  double* s_geo = reinterpret_cast<double*>(smem_raw + L.off_geo);
-  double* s_vc = reinterpret_cast<double*>(smem_raw + L.off_vc);
-  int* s_perm = reinterpret_cast<int*>(smem_raw + L.off_perm);
-  int* s_pl = reinterpret_cast<int*>(smem_raw + L.off_pl);
-  int* s_ord = reinterpret_cast<int*>(smem_raw + L.off_ord);
+  double* s_fx = reinterpret_cast<double*>(smem_raw + L.off_fin);  // [D][nslots]
+  double* s_fv = s_fx + D * nslots;                                // [D][nslots]
+  double* s_fxi = s_fv + D * nslots;                               // [D][nslots]
+  int* s_fsrc = reinterpret_cast<int*>(s_fxi + D * nslots);        // [nslots]
+  // before the final placement: key, group order, reference position and
+  // velocity of the staged particles, in arrival order (same bytes)
+  unsigned long long* s_key = reinterpret_cast<unsigned long long*>(smem_raw + L.off_fin);  // [nmax]
+  int* s_ord = reinterpret_cast<int*>(s_key + nslots);                                    // [nmax]
+  double* s_sxi = reinterpret_cast<double*>(s_ord + nslots);                              // [D][nslots]
+  double* s_sv = s_sxi + D * nslots;                                                      // [D][nslots]
   int* s_cnt = reinterpret_cast<int*>(smem_raw + L.off_cnt);
   int* s_base = reinterpret_cast<int*>(smem_raw + L.off_base);
   int* s_roff = reinterpret_cast<int*>(smem_raw + L.off_roff);
-  int* s_fill = reinterpret_cast<int*>(smem_raw + L.off_fill);
-  int* s_code = reinterpret_cast<int*>(smem_raw + L.off_code);
   int* s_ccnt = reinterpret_cast<int*>(smem_raw + L.off_cell);  // particles per cell
-  int* s_inst = s_ccnt + cpw;                                   // instance per cell
-  int* s_cfin = s_inst + cpw;                                   // final offset per cell (chunk-relative)
-  // [0] refill slots, [2] chunk total, [3] chunk prefix, [4] chunk, [5] bucket count, [6] spill start, [7] spill end
-  int* s_misc = s_cfin + 2 * cpw;
-  unsigned long long* s_part = reinterpret_cast<unsigned long long*>(s_misc + 8);  // [4] scan partials
+  int* s_cfin = s_ccnt + cpw;                                   // final offset per cell (chunk-relative)
+  // [0] chunk, [1..3] bucket count, spill start, spill end of the chunk;
+  // [4] next chunk, [5..7] its header; [8] chunk prefix; [9] staged
+  int* s_misc = s_cfin + cpw;
+  unsigned long long* s_part = reinterpret_cast<unsigned long long*>(s_misc + 16);  // [4] scan partials
 
   const int nchunks = (nc + cpw - 1) >> cshift;
   const int per_thread = ((pmax << cshift) + blockDim.x - 1) / blockDim.x;
-  if (threadIdx.x == 0) {
-    const int ch = atomicAdd(ticket, 1);
-    s_misc[4] = ch;
-    if (ch < nchunks) {
-      s_misc[5] = bcount[ch];
-      s_misc[6] = sstart[ch];
-      s_misc[7] = sstart[ch + 1];
-    }
-  }
-  __syncthreads();
-  for (;;) {
-    const int ch = s_misc[4];
-    if (ch >= nchunks) break;
-    int next = 0;
-    if (threadIdx.x == 0) next = atomicAdd(ticket, 1);  // the next chunk, drawn early
-    const int c0 = ch << cshift;
-    const int c1 = min(nc, c0 + cpw);
-    const int ncell = c1 - c0;
-    const int nb = min(s_misc[5], capc);
-    const int n = nb + s_misc[7] - s_misc[6];
-    const long bucket0 = (long)ch * capc;
-    const long spill0 = spill2_base + s_misc[6];
-    const bool staged = n <= nmax;
-    // A. cell data and particles of the chunk, one round of loads
-    for (int i = threadIdx.x; i < ncell * 3 * D; i += blockDim.x) s_geo[i] = m.geom[(size_t)c0 * 3 * D + i];
-    if (do_refill)
-      for (int i = threadIdx.x; i < ncell * NV * D; i += blockDim.x) s_vc[i] = Vc[(size_t)c0 * NV * D + i];
-    for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
-      s_inst[i] = inst_of(m, c0 + i);
-      s_ccnt[i] = 0;
-    }
-    for (int t = threadIdx.x; t < ncell * pmax; t += blockDim.x) {
-      s_cnt[t] = 0;
-      s_fill[t] = 0;
-      s_code[t] = m.ptab[(inst_of(m, c0 + (t >> pshift)) << pshift) + (t & (pmax - 1))];
-    }
-    if (threadIdx.x == 0) s_misc[0] = 0;
-    int pc[kSettlePre];
-    double px[kSettlePre][D], pv[kSettlePre][D];
+
+  // registers: the particles of this thread in the current chunk
+  int pc[PRE];
+  double px[PRE][D], pv[PRE][D];
+  auto load_parts = [&](int n, int nb, long bucket0, long spill0) {
 #pragma unroll
-    for (int u = 0; u < kSettlePre; ++u) {
+    for (int u = 0; u < PRE; ++u) {
       const int j = threadIdx.x + u * blockDim.x;
       pc[u] = -1;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        px[u][k] = 0.0;
+        pv[u][k] = 0.0;
+      }
       if (j < n) {
         const long i = chunk_index(j, nb, bucket0, spill0);
         pc[u] = T.cell[i];
@@ -638,51 +706,99 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
         }
       }
     }
+  };
+  const int cb0 = L.off_cbuf[0], cb1 = L.off_cbuf[1];
+  auto cbuf_geo = [&](int b) { return reinterpret_cast<double*>(smem_raw + (b ? cb1 : cb0) + L.off_geo); };
+  auto cbuf_vc = [&](int b) { return reinterpret_cast<double*>(smem_raw + (b ? cb1 : cb0) + L.off_vc); };
+  auto cbuf_prm = [&](int b) { return reinterpret_cast<Prm*>(smem_raw + (b ? cb1 : cb0) + L.off_prm); };
+  auto cbuf_inst = [&](int b) { return reinterpret_cast<int*>(smem_raw + (b ? cb1 : cb0) + L.off_inst); };
+
+  // prologue: the first chunk's header, cell data and particles
+  if (threadIdx.x == 0) {
+    const int ch = atomicAdd(ticket, 1);
+    s_misc[0] = ch;
+    if (ch < nchunks) {
+      s_misc[1] = bcount[ch];
+      s_misc[2] = sstart[ch];
+      s_misc[3] = sstart[ch + 1];
+    }
+  }
+  __syncthreads();
+  int buf = 0;
+  {
+    const int ch = s_misc[0];
+    if (ch < nchunks) {
+      const int c0 = ch << cshift;
+      CellRegs<D> cr;
+      cells_load<D>(m, Vc, do_refill, c0, min(nc, c0 + cpw) - c0, cr);
+      cells_store<D>(cr, min(nc, c0 + cpw) - c0, cbuf_geo(0), cbuf_vc(0), cbuf_prm(0), cbuf_inst(0));
+      const int nb = min(s_misc[1], capc);
+      load_parts(nb + s_misc[3] - s_misc[2], nb, (long)ch * capc, spill2_base + s_misc[2]);
+    }
+  }
+  for (;;) {
+    const int ch = s_misc[0];
+    if (ch >= nchunks) break;
+    int next = 0;
+    if (threadIdx.x == 0) next = atomicAdd(ticket, 1);  // the next chunk, drawn early
+    const int c0 = ch << cshift;
+    const int c1 = min(nc, c0 + cpw);
+    const int ncell = c1 - c0;
+    const int nb = min(s_misc[1], capc);
+    const int n = nb + s_misc[3] - s_misc[2];
+    const long bucket0 = (long)ch * capc;
+    const long spill0 = spill2_base + s_misc[2];
+    const bool staged_in = n <= nmax;  // particles fit the registers and the key/group arrays
+    const double* s_geo = cbuf_geo(buf);
+    const double* s_vc = cbuf_vc(buf);
+    const Prm* s_prm = cbuf_prm(buf);
+    for (int i = threadIdx.x; i < ncell * pmax; i += blockDim.x) s_cnt[i] = 0;
+    for (int i = threadIdx.x; i < ncell; i += blockDim.x) s_ccnt[i] = 0;
     __syncthreads();
-    for (int j = threadIdx.x, u = 0; j < n; j += blockDim.x, ++u) {
-      int c;
-      double x[D], v[D];
-      if (u < kSettlePre) {
-        c = pc[0];
+    // A. reference position, part and key of every particle; counts per
+    //    (cell, part)
+    int ppl[PRE];
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-          x[k] = px[0][k];
-          v[k] = pv[0][k];
-        }
-#pragma unroll
-        for (int w = 0; w + 1 < kSettlePre; ++w) {
-          pc[w] = pc[w + 1];
+    for (int u = 0; u < PRE; ++u) {
+      const int j = threadIdx.x + u * blockDim.x;
+      ppl[u] = 0;
+      if (j < n) {
+        const int lc = pc[u] - c0;
+        const Prm& pr = s_prm[lc];
+        double xi[D];
+        settle_xi<D>(m, pr, pc[u], s_geo + lc * 3 * D, px[u], xi);
+        const int part = part_lin<D>(pr, xi, nullptr);
+        ppl[u] = (lc << 16) | part;
+        atomicAdd(&s_cnt[(lc << pshift) + part], 1);
+        atomicAdd(&s_ccnt[lc], 1);
+        if (staged_in) {
+          s_key[j] = particle_key<D>(px[u]);
 #pragma unroll
           for (int k = 0; k < D; ++k) {
-            px[w][k] = px[w + 1][k];
-            pv[w][k] = pv[w + 1][k];
+            s_sxi[k * nslots + j] = xi[k];
+            s_sv[k * nslots + j] = pv[u][k];
           }
         }
-      } else {
-        const long i = chunk_index(j, nb, bucket0, spill0);
-        c = T.cell[i];
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-          x[k] = T.x[k][i];
-          v[k] = T.v[k][i];
-        }
       }
+    }
+    for (int j = threadIdx.x + PRE * blockDim.x; j < n; j += blockDim.x) {  // beyond the registers: not staged
+      const long i = chunk_index(j, nb, bucket0, spill0);
+      const int c = T.cell[i];
       const int lc = c - c0;
-      const Prm& pr = m.prm[s_inst[lc]];
-      double xi[D];
+      const Prm& pr = s_prm[lc];
+      double x[D], xi[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) x[k] = T.x[k][i];
       settle_xi<D>(m, pr, c, s_geo + lc * 3 * D, x, xi);
       const int part = part_lin<D>(pr, xi, nullptr);
       atomicAdd(&s_cnt[(lc << pshift) + part], 1);
       atomicAdd(&s_ccnt[lc], 1);
-      if (staged) {
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-          s_xi[k * nslots + j] = xi[k];
-          s_v[k * nslots + j] = v[k];
-        }
-        s_pl[j] = (lc << 16) | part;
-        s_key[j] = particle_key<D>(x);
-      }
+    }
+    int nh0 = 0, nh1 = 0, nh2 = 0;
+    if (threadIdx.x == 0 && next < nchunks) {  // header of the next chunk, read while this one runs
+      nh0 = bcount[next];
+      nh1 = sstart[next];
+      nh2 = sstart[next + 1];
     }
     __syncthreads();
     // B. final (survivors + refill) and raw counts per (cell, part), scanned
@@ -693,8 +809,10 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
       if (t >= ncell * pmax) break;
       const int lc = t >> pshift;
       const int k = s_cnt[t];
-      const int kept = apply_cap ? min(k, m.prm[s_inst[lc]].maxpp + 1) : k;
-      const int refill = (do_refill && s_ccnt[lc] > 0 && k == 0 && s_code[t] < 0) ? 1 : 0;
+      int pi[D];
+      const bool valid = ((t & (pmax - 1)) < s_prm[lc].P) && part_code<D>(s_prm[lc], t & (pmax - 1), pi);
+      const int kept = apply_cap ? min(k, s_prm[lc].maxpp + 1) : k;
+      const int refill = (do_refill && s_ccnt[lc] > 0 && k == 0 && valid) ? 1 : 0;
       s_base[t] = kept + refill;
       mine += ((unsigned long long)(kept + refill) << 32) | (unsigned)k;
     }
@@ -710,89 +828,98 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
       run += ((unsigned long long)fin << 32) | (unsigned)s_cnt[t];
     }
     const int nout = (int)(tot >> 32);
+    const bool staged = staged_in && nout <= nslots;
     if (threadIdx.x == 0) {
       lb_publish(lb, ch, ch == 0 ? 2u : 1u, (unsigned)nout);
-      if (ch == 0) s_misc[3] = 0;
-      if (next < nchunks) {  // header of the next chunk, read while this one runs
-        s_misc[5] = bcount[next];
-        s_misc[6] = sstart[next];
-        s_misc[7] = sstart[next + 1];
-      }
+      if (ch == 0) s_misc[8] = 0;
       if (!staged && ch > 0) {  // the global-memory path writes as it goes
         const unsigned prefix = lb_prefix(lb, ch, err);
         lb_publish(lb, ch, 2u, prefix + (unsigned)nout);
-        s_misc[3] = (int)prefix;
+        s_misc[8] = (int)prefix;
       }
+      s_misc[4] = next;
+      s_misc[5] = nh0;
+      s_misc[6] = nh1;
+      s_misc[7] = nh2;
     }
     __syncthreads();
-    // D. group by (cell, part), rank by key, place survivors and refills
-    //    (staged chunks in LDS; the output offset is not needed yet)
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      int t;
-      if (staged) {
-        t = ((s_pl[j] >> 16) << pshift) + (s_pl[j] & 0xffff);
-      } else {
+    // D. group by (cell, part) (s_roff becomes the end of each group), rank
+    //    by key inside the group
+    if (staged) {
+#pragma unroll
+      for (int u = 0; u < PRE; ++u) {
+        const int j = threadIdx.x + u * blockDim.x;
+        if (j < n) {
+          const int t = ((ppl[u] >> 16) << pshift) + (ppl[u] & 0xffff);
+          s_ord[atomicAdd(&s_roff[t], 1)] = j;
+        }
+      }
+    } else {
+      for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const long i = chunk_index(j, nb, bucket0, spill0);
         const int c = T.cell[i];
         const int lc = c - c0;
-        const Prm& pr = m.prm[s_inst[lc]];
         double x[D], xi[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) x[k] = T.x[k][i];
-        settle_xi<D>(m, pr, c, s_geo + lc * 3 * D, x, xi);
-        t = (lc << pshift) + part_lin<D>(pr, xi, nullptr);
+        settle_xi<D>(m, s_prm[lc], c, s_geo + lc * 3 * D, x, xi);
+        const int t = (lc << pshift) + part_lin<D>(s_prm[lc], xi, nullptr);
+        gord[chunk_index(atomicAdd(&s_roff[t], 1), nb, bucket0, spill0)] = j;
       }
-      const int pos = s_roff[t] + atomicAdd(&s_fill[t], 1);
-      if (staged)
-        s_ord[pos] = j;
-      else
-        gord[chunk_index(pos, nb, bucket0, spill0)] = j;
     }
     __syncthreads();
-    const int o0n = staged ? 0 : s_misc[3];  // output offset (global-memory path only)
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      int lc, part, r = 0;
-      if (staged) {
-        lc = s_pl[j] >> 16;
-        part = s_pl[j] & 0xffff;
-        const unsigned long long key = s_key[j];
-        const int t = (lc << pshift) + part;
-        const int a = s_roff[t], b = a + s_cnt[t];
-        for (int q = a; q < b; ++q) {
-          const int o = s_ord[q];
-          if (o == j) continue;
-          const unsigned long long kq = s_key[o];
-          if (kq < key) {
-            ++r;
-          } else if (kq == key) {
-            double xq[D], vq[D], xj[D], vj[D];
-            const long iq = chunk_index(o, nb, bucket0, spill0), ij = chunk_index(j, nb, bucket0, spill0);
+    const int o0n = staged ? 0 : s_misc[8];
+    int pf[PRE];  // final slot of each register particle (-1: capped away)
+    if (staged) {
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-              xq[k] = T.x[k][iq];
-              vq[k] = s_v[k * nslots + o];
-              xj[k] = T.x[k][ij];
-              vj[k] = s_v[k * nslots + j];
+      for (int u = 0; u < PRE; ++u) {
+        const int j = threadIdx.x + u * blockDim.x;
+        pf[u] = -1;
+        if (j < n) {
+          const int lc = ppl[u] >> 16;
+          const int t = (lc << pshift) + (ppl[u] & 0xffff);
+          const unsigned long long key = s_key[j];
+          const int b = s_roff[t], a = b - s_cnt[t];
+          int r = 0;
+          for (int q = a; q < b; ++q) {
+            const int o = s_ord[q];
+            if (o == j) continue;
+            const unsigned long long kq = s_key[o];
+            if (kq < key) {
+              ++r;
+            } else if (kq == key) {
+              double xq[D], vq[D];
+              const long iq = chunk_index(o, nb, bucket0, spill0);
+#pragma unroll
+              for (int k = 0; k < D; ++k) {
+                xq[k] = T.x[k][iq];
+                vq[k] = T.v[k][iq];
+              }
+              double vj[D];
+#pragma unroll
+              for (int k = 0; k < D; ++k) vj[k] = s_sv[k * nslots + j];
+              if (tie_less<D>(xq, vq, px[u], vj, o, j)) ++r;
             }
-            if (tie_less<D>(xq, vq, xj, vj, o, j)) ++r;
           }
+          if (!(apply_cap && r > s_prm[lc].maxpp)) pf[u] = s_base[t] + r;
         }
-      } else {
+      }
+    } else {
+      for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const long i = chunk_index(j, nb, bucket0, spill0);
         const int c = T.cell[i];
-        lc = c - c0;
-        const Prm& pr = m.prm[s_inst[lc]];
+        const int lc = c - c0;
         double xj[D], vj[D], xi[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
           xj[k] = T.x[k][i];
           vj[k] = T.v[k][i];
         }
-        settle_xi<D>(m, pr, c, s_geo + lc * 3 * D, xj, xi);
-        part = part_lin<D>(pr, xi, nullptr);
+        settle_xi<D>(m, s_prm[lc], c, s_geo + lc * 3 * D, xj, xi);
+        const int t = (lc << pshift) + part_lin<D>(s_prm[lc], xi, nullptr);
         const unsigned long long key = particle_key<D>(xj);
-        const int t = (lc << pshift) + part;
-        const int a = s_roff[t], b = a + s_cnt[t];
+        const int b = s_roff[t], a = b - s_cnt[t];
+        int r = 0;
         for (int q = a; q < b; ++q) {
           const int o = gord[chunk_index(q, nb, bucket0, spill0)];
           if (o == j) continue;
@@ -806,20 +933,44 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
           const unsigned long long kq = particle_key<D>(xq);
           if (kq < key || (kq == key && tie_less<D>(xq, vq, xj, vj, o, j))) ++r;
         }
-      }
-      if (apply_cap && r > m.prm[s_inst[lc]].maxpp) continue;
-      const int fpos = s_base[(lc << pshift) + part] + r;
-      if (staged) {
-        s_perm[fpos] = j;
-      } else {
-        const long i = chunk_index(j, nb, bucket0, spill0);
+        if (apply_cap && r > s_prm[lc].maxpp) continue;
+        const int fpos = o0n + s_base[t] + r;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          P.x[k][o0n + fpos] = T.x[k][i];
-          P.v[k][o0n + fpos] = T.v[k][i];
+          P.x[k][fpos] = xj[k];
+          P.v[k][fpos] = vj[k];
         }
-        P.cell[o0n + fpos] = c0 + lc;
-        if (P.src) P.src[o0n + fpos] = T.src[i];
+        P.cell[fpos] = c;
+        if (P.src) P.src[fpos] = T.src[i];
+      }
+    }
+    double qxi[PRE][D], qv[PRE][D];
+    if (staged) {
+#pragma unroll
+      for (int u = 0; u < PRE; ++u) {
+        const int j = threadIdx.x + u * blockDim.x;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          qxi[u][k] = pf[u] >= 0 ? s_sxi[k * nslots + j] : 0.0;
+          qv[u][k] = pf[u] >= 0 ? s_sv[k * nslots + j] : 0.0;
+        }
+      }
+    }
+    __syncthreads();  // the arrival-order arrays are dead from here on
+    // final placement of the staged survivors
+    if (staged) {
+#pragma unroll
+      for (int u = 0; u < PRE; ++u) {
+        const int f = pf[u];
+        if (f >= 0) {
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            s_fx[k * nslots + f] = px[u][k];
+            s_fv[k * nslots + f] = qv[u][k];
+            s_fxi[k * nslots + f] = qxi[u][k];
+          }
+          if (P.src) s_fsrc[f] = T.src[chunk_index(threadIdx.x + u * blockDim.x, nb, bucket0, spill0)];
+        }
       }
     }
     // refills: one particle at the centre of each empty part of a non-empty
@@ -827,13 +978,14 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
     if (do_refill) {
       for (int t = threadIdx.x; t < ncell * pmax; t += blockDim.x) {
         const int lc = t >> pshift, pp = t & (pmax - 1);
-        const int code = s_code[t];
-        if (s_cnt[t] != 0 || s_ccnt[lc] == 0 || code >= 0) continue;  // occupied, empty cell, or no part
-        const Prm& pr = m.prm[s_inst[lc]];
+        if (s_cnt[t] != 0 || s_ccnt[lc] == 0) continue;
+        const Prm& pr = s_prm[lc];
+        int pi[D];
+        if (pp >= pr.P || !part_code<D>(pr, pp, pi)) continue;  // padding or no refillable part
         const int c = c0 + lc;
         double xi[D], x[D], v[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) xi[k] = ((double)((code >> (8 * k)) & 0xff) + 1.0 / 2) * pr.h[k];
+        for (int k = 0; k < D; ++k) xi[k] = ((double)pi[k] + 1.0 / 2) * pr.h[k];
         const double* g = s_geo + lc * 3 * D;
         if (g[0] == g[0]) {
           CellCache<D> cc;
@@ -850,73 +1002,77 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
         }
         double N[NV];
         shape<D>(xi, N);
-        const double* f = s_vc + lc * NV * D;
+        const double* fv = s_vc + lc * NV * D;
 #pragma unroll
         for (int k = 0; k < D; ++k) v[k] = 0.0;
 #pragma unroll
         for (int i = 0; i < NV; ++i)
 #pragma unroll
-          for (int k = 0; k < D; ++k) v[k] = v[k] + N[i] * f[i * D + k];
-        const int fpos = s_base[t];
+          for (int k = 0; k < D; ++k) v[k] = v[k] + N[i] * fv[i * D + k];
+        const int f = s_base[t];
         if (staged) {
-          const int r = atomicAdd(&s_misc[0], 1);
-          const int slot = nmax + r;
 #pragma unroll
           for (int k = 0; k < D; ++k) {
-            s_xi[k * nslots + slot] = xi[k];
-            s_v[k * nslots + slot] = v[k];
-            s_xr[k * (nslots - nmax) + r] = x[k];
+            s_fx[k * nslots + f] = x[k];
+            s_fv[k * nslots + f] = v[k];
+            s_fxi[k * nslots + f] = xi[k];
           }
-          s_perm[fpos] = slot;
-          s_pl[slot] = (lc << 16) | pp;
+          s_fsrc[f] = -1 - pp;
         } else {
 #pragma unroll
           for (int k = 0; k < D; ++k) {
-            P.x[k][o0n + fpos] = x[k];
-            P.v[k][o0n + fpos] = v[k];
+            P.x[k][o0n + f] = x[k];
+            P.v[k][o0n + f] = v[k];
           }
-          P.cell[o0n + fpos] = c;
-          if (P.src) P.src[o0n + fpos] = -1 - pp;
+          P.cell[o0n + f] = c;
+          if (P.src) P.src[o0n + f] = -1 - pp;
         }
       }
     }
+    // the next chunk: particles into the registers, cell data into the other
+    // buffer (both used from the next iteration on)
+    const int nch = s_misc[4];
+    const int nc0 = nch << cshift;
+    const int nncell = nch < nchunks ? min(nc, nc0 + cpw) - nc0 : 0;
+    CellRegs<D> cr;
+    if (nch < nchunks) {
+      cells_load<D>(m, Vc, do_refill, nc0, nncell, cr);
+      const int nnb = min(s_misc[5], capc);
+      load_parts(nnb + s_misc[7] - s_misc[6], nnb, (long)nch * capc, spill2_base + s_misc[6]);
+    }
     // C. output offset of a staged chunk by decoupled look-back, resolved
-    //    after its own grouping and ranking so that the predecessors have
-    //    most likely published their prefixes by now
-    if (staged && threadIdx.x == 0 && ch > 0) {
-      const unsigned prefix = lb_prefix(lb, ch, err);
-      lb_publish(lb, ch, 2u, prefix + (unsigned)nout);
-      s_misc[3] = (int)prefix;
+    //    only now so that the wait overlaps the chunk's own work
+    if (staged && threadIdx.x < 64 && ch > 0) {
+      const unsigned prefix = lb_prefix_wave(lb, ch, err);
+      if (threadIdx.x == 0) {
+        lb_publish(lb, ch, 2u, prefix + (unsigned)nout);
+        s_misc[8] = (int)prefix;
+      }
     }
     __syncthreads();
-    const int o0 = s_misc[3];
+    const int o0 = s_misc[8];
     for (int lc = threadIdx.x; lc < ncell; lc += blockDim.x) cs_out[c0 + lc] = o0 + s_cfin[lc];
     if (c1 == nc && threadIdx.x == 0) cs_out[nc] = o0 + nout;
     if (staged) {
-      // E. coalesced write-out in final order
+      // E. coalesced write-out in final order and the per (cell, vertex)
+      //    sums, sequential in the same order
       for (int t = threadIdx.x; t < nout; t += blockDim.x) {
-        const int slot = s_perm[t];
-        if (slot < nmax) {
-          const long i = chunk_index(slot, nb, bucket0, spill0);
-#pragma unroll
-          for (int k = 0; k < D; ++k) {
-            P.x[k][o0 + t] = T.x[k][i];
-            P.v[k][o0 + t] = s_v[k * nslots + slot];
-          }
-          P.cell[o0 + t] = c0 + (s_pl[slot] >> 16);
-          if (P.src) P.src[o0 + t] = T.src[i];
-        } else {
-          const int r = slot - nmax;  // refill
-#pragma unroll
-          for (int k = 0; k < D; ++k) {
-            P.x[k][o0 + t] = s_xr[k * (nslots - nmax) + r];
-            P.v[k][o0 + t] = s_v[k * nslots + slot];
-          }
-          P.cell[o0 + t] = c0 + (s_pl[slot] >> 16);
-          if (P.src) P.src[o0 + t] = -1 - (s_pl[slot] & 0xffff);
+        int lo = 0, hi = ncell - 1;  // cell of final position t: last lc with s_cfin[lc] <= t
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_cfin[mid] <= t)
+            lo = mid;
+          else
+            hi = mid - 1;
         }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          P.x[k][o0 + t] = s_fx[k * nslots + t];
+          P.v[k][o0 + t] = s_fv[k * nslots + t];
+        }
+        P.cell[o0 + t] = c0 + lo;
+        if (P.src) P.src[o0 + t] = s_fsrc[t];
       }
-      // per (cell, vertex) sums, sequential in final order
       for (int t = threadIdx.x; t < ncell * NV; t += blockDim.x) {
         const int lc = t / NV, iv = t - lc * NV;
         const int c = c0 + lc;
@@ -925,16 +1081,15 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
 #pragma unroll
         for (int k = 0; k < D; ++k) S[k] = 0.0;
         for (int f = a; f < b; ++f) {
-          const int slot = s_perm[f];
           double N = 1.0;
 #pragma unroll
           for (int k = 0; k < D; ++k) {
-            const double xk = s_xi[k * nslots + slot];
+            const double xk = s_fxi[k * nslots + f];
             const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
             N = (k == 0) ? l : N * l;
           }
 #pragma unroll
-          for (int k = 0; k < D; ++k) S[k] = S[k] + N * s_v[k * nslots + slot];
+          for (int k = 0; k < D; ++k) S[k] = S[k] + N * s_fv[k * nslots + f];
           W = W + N;
         }
         double* e = contrib + ((size_t)c * NV + iv) * (D + 1);
@@ -943,10 +1098,15 @@ __global__ __launch_bounds__(kBlock) void k_settle(int apply_cap, int do_refill,
         e[D] = W;
       }
     }
+    if (nch < nchunks) cells_store<D>(cr, nncell, cbuf_geo(buf ^ 1), cbuf_vc(buf ^ 1), cbuf_prm(buf ^ 1), cbuf_inst(buf ^ 1));
     if (threadIdx.x == 0) {
       dirty[ch] = staged ? 0 : 1;
-      s_misc[4] = next;
+      s_misc[0] = s_misc[4];
+      s_misc[1] = s_misc[5];
+      s_misc[2] = s_misc[6];
+      s_misc[3] = s_misc[7];
     }
+    buf ^= 1;
     __syncthreads();
   }
 }

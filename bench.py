@@ -8,11 +8,16 @@ velocity, CFL 0.5, 5 Euler substeps, FP64. One step is the harness order
 move (correct fused) -> project -> FEM stand-in (V_old := projected, V :=
 prescribed) -> correct, with every input resident in HBM.
 
-Prints one JSON line: particle-steps/s (particles counted at the start of
-each step), the HBM roofline fraction of the step and of its dominant kernel
-(algorithmic bytes per particle-step from the measurement definitions), and a
-CPU baseline: the oracle in reference-order mode on a bounded sample of the
-same workload, timed on this host.
+Run A times W warmup and K measured steps from the seeded state (HIP events
+on the handle's stream around the K steps; particles counted at the start of
+each step). Run B repeats the same W + K steps from the same state with HIP
+events around every launch, for the per-kernel durations and the roofline of
+the dominant kernel (algorithmic bytes per launch / average launch duration;
+the traffic figure comes from the committed counter summary). Then the
+256-instance batched form against plain launches, the adjoint's cost
+against its forward, and a CPU baseline: the oracle in reference-order mode on
+a bounded sample of the same workload, timed on this host. Prints one JSON
+line.
 """
 import argparse
 import json
@@ -107,25 +112,35 @@ def batched_comparison(steps, warmup, n_inst=256, scale=1.0, log=None, force_tun
     stream. Particles are counted at the start of each step."""
     import torch
     from pfem2dealii_amd import BatchHandler, ParticleHandler, workloads as W
-    from pfem2dealii_amd.autotune import tune
+    from pfem2dealii_amd.autotune import shipped, tune
     inst = W.batched_instances(n_inst, seed=4, scale=scale)
-    b = BatchHandler([i["mesh"] for i in inst], [i["params"] for i in inst])
-    b.initialize_maps()
-    co = b.cell_offsets
-    x = np.concatenate([i["x"] for i in inst], axis=1)
-    cells = np.concatenate([i["cells"] + co[k] for k, i in enumerate(inst)]).astype(np.int32)
     Vh = np.concatenate([i["V"] for i in inst], axis=1)
-    assert b.insert_particles(x, cells) == 0
-    del x, cells
     V = torch.tensor(Vh, dtype=torch.float64, device="cuda")
-    b.set_velocity_from_field(V)
     Vp = torch.empty_like(V)
+
+    def make():
+        b = BatchHandler([i["mesh"] for i in inst], [i["params"] for i in inst])
+        b.initialize_maps()
+        co = b.cell_offsets
+        x = np.concatenate([i["x"] for i in inst], axis=1)
+        cells = np.concatenate([i["cells"] + co[k] for k, i in enumerate(inst)]).astype(np.int32)
+        assert b.insert_particles(x, cells) == 0
+        b.set_velocity_from_field(V)
+        b.reserve_particles(3 * len(cells))
+        return b
+
+    sig = {"workload": "batched", "n": n_inst, "scale": scale}
+    tuning = None if force_tune else shipped(sig)
+    if tuning is None:  # on a scratch batch: tuning steps advance the simulation
+        bt = make()
+        tuning, _ = tune(bt, lambda: bt.step(V, Vp, use_graph=True), signature=sig, log=log, force=True)
+        bt.close()
+    b = make()
+    b.set_tuning(**tuning)
     n0 = b.count()
     torch.cuda.synchronize()
     bstep = lambda: b.step(V, Vp, use_graph=True)  # noqa: E731
     bstep()
-    tuning, _ = tune(b, bstep, signature={"workload": "batched", "n": n_inst, "scale": scale}, log=log,
-                     force=force_tune)
     for _ in range(warmup):
         bstep()
     stream = torch.cuda.ExternalStream(b.stream())
@@ -149,6 +164,7 @@ def batched_comparison(steps, warmup, n_inst=256, scale=1.0, log=None, force_tun
         h = ParticleHandler(i["mesh"], i["params"], stream=pstream.cuda_stream)
         h.initialize_maps()
         h.insert_particles(i["x"], i["cells"])
+        h.reserve_particles(3 * len(i["cells"]))
         Vi = torch.tensor(i["V"], dtype=torch.float64, device="cuda")
         h.set_velocity_from_field(Vi)
         hs.append(h)
@@ -186,7 +202,9 @@ def adjoint_cost(reps=3, scale=1.0):
     steps). Forward: the 20 moves and the projection. Adjoint: one
     pfem2gpu_adjoint call (its own checkpointed forward, the loss, the reverse
     sweep with segment recompute). Wall clock with the stream drained on both
-    sides, fresh handles from the same initial state, median of `reps`."""
+    sides; each repetition re-inserts the same initial particles into the
+    same two handles (so the second and later repetitions reuse the device
+    buffers of the first); median of `reps` after one warm-up repetition."""
     import torch
     from pfem2dealii_amd import ParticleHandler, workloads as W
     cfg = W.config("adjoint2d", scale=scale)
@@ -208,31 +226,42 @@ def adjoint_cost(reps=3, scale=1.0):
     ht.project_particle_fields(target)
     ht.synchronize()
     ht.close()
+    hf, V = fresh(1.0)
+    ha, _ = fresh(1.0)
+    for h in (hf, ha):
+        h.reserve_particles(3 * len(cfg["cells"]))
+
+    def reset(h):
+        h.clear_particles()
+        h.insert_particles(cfg["x"], cfg["cells"])
+        h.set_velocity_from_field(V)
+        h.synchronize()
+
     fwd, adj = [], []
     n0 = 0
     loss = None
-    for r in range(reps + 1):
-        h, V = fresh(1.0)
-        n0 = h.count()
-        Vp = torch.empty_like(V)
+    Vp = torch.empty_like(V)
+    for r in range(reps + 1):  # the first repetition warms the buffers up
+        reset(hf)
+        n0 = hf.count()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
-            h.move_particles(V)
-        h.project_particle_fields(Vp)
-        h.synchronize()
+            hf.move_particles(V)
+        hf.project_particle_fields(Vp)
+        hf.synchronize()
         t1 = time.perf_counter()
-        h.close()
-        h, V = fresh(1.0)
+        reset(ha)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        loss, _, _ = h.adjoint(steps, K, V, target)
-        h.synchronize()
+        loss, _, _ = ha.adjoint(steps, K, V, target)
+        ha.synchronize()
         t3 = time.perf_counter()
-        h.close()
         if r > 0:
             fwd.append(t1 - t0)
             adj.append(t3 - t2)
+    hf.close()
+    ha.close()
     f, a = float(np.median(fwd)), float(np.median(adj))
     return {"workload": "adjoint2d", "mesh": "x".join(str(a_) for a_ in cfg["mesh"]["n"]), "particles_at_start": n0,
             "steps": steps, "checkpoint_every": K, "forward_ms": round(f * 1e3, 3), "adjoint_ms": round(a * 1e3, 3),
@@ -261,7 +290,6 @@ def main():
     ap.add_argument("--workload", default="taylor_green2d")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=3, help="steps re-run with per-kernel events")
     ap.add_argument("--batched-steps", type=int, default=5, help="steps of the 256-instance comparison (0: skip)")
     ap.add_argument("--adjoint-reps", type=int, default=3, help="repetitions of the adjoint cost measurement (0: skip)")
     ap.add_argument("--autotune", action="store_true", help="tune launch shapes instead of the shipped table")
@@ -283,29 +311,41 @@ def main():
     cfg = W.config(args.workload, scale=args.scale)
     mesh = cfg["mesh"]
     dim = mesh["dim"]
-    h = ParticleHandler(mesh, cfg["params"])
-    h.initialize_maps()
-    rej = h.insert_particles(cfg["x"], cfg["cells"])
-    assert rej == 0
     V = torch.tensor(cfg["V"], dtype=torch.float64, device="cuda")
-    h.set_velocity_from_field(V)
     Vp = torch.empty_like(V)
-    n0 = h.count()
-    del cfg["x"], cfg["cells"]
-    stream = torch.cuda.ExternalStream(h.stream())
-    torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t_setup
 
-    def step():
+    def make():
+        h = ParticleHandler(mesh, cfg["params"])
+        h.initialize_maps()
+        assert h.insert_particles(cfg["x"], cfg["cells"]) == 0
+        h.set_velocity_from_field(V)
+        # capacity for the particle growth of the measured steps (refills),
+        # so that no step reallocates device arrays
+        h.reserve_particles(3 * len(cfg["cells"]))
+        return h
+
+    def step(h):
         h.move_particles(V)
         h.project_particle_fields(Vp)
         h.correct_particle_velocity(V, Vp)
 
-    step()
-    from pfem2dealii_amd.autotune import tune
-    tuning, _ = tune(h, step, signature={"workload": args.workload, "scale": args.scale}, force=args.autotune)
+    from pfem2dealii_amd.autotune import shipped, tune
+    sig = {"workload": args.workload, "scale": args.scale}
+    tuning = None if args.autotune else shipped(sig)
+    if tuning is None:  # on a scratch handle: tuning steps advance the simulation
+        ht = make()
+        tuning, _ = tune(ht, lambda: step(ht), signature=sig, force=True)
+        ht.close()
+    h = make()
+    h.set_tuning(**tuning)
+    n0 = h.count()
+    stream = torch.cuda.ExternalStream(h.stream())
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t_setup
+
+    # A. timed run: warmup steps, then `steps` steps between two HIP events
     for _ in range(args.warmup):
-        step()
+        step(h)
     h.synchronize()
     counts = torch.zeros(args.steps, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
@@ -316,7 +356,7 @@ def main():
     ev0.record(stream)
     for s in range(args.steps):
         h.count_async(counts.data_ptr() + 4 * s)
-        step()
+        step(h)
     ev1.record(stream)
     h.synchronize()
     torch.cuda.synchronize()
@@ -325,35 +365,43 @@ def main():
     psteps = int(counts.sum().item())
     ms_per_step = gpu_ms / args.steps
     value = psteps / (gpu_ms * 1e-3)
+    h.close()
 
-    # per-kernel durations over a few more steps (HIP events around each
-    # launch on the handle's stream), with the particle count before each step
+    # B. the same warmup + timed steps again from the same initial state with
+    #    HIP events around every launch on the handle's stream: per-kernel
+    #    durations over exactly the launches of run A (a kernel trace of this
+    #    program sees runs A and B, which launch identical work)
+    h = make()
+    h.set_tuning(**tuning)
+    nb_steps = args.warmup + args.steps
+    pcounts = torch.zeros(nb_steps + 1, dtype=torch.int32, device="cuda")
     h.set_timing(True)
-    pcounts = torch.zeros(args.profile_steps + 1, dtype=torch.int32, device="cuda")
-    for s in range(args.profile_steps):
+    for s in range(nb_steps):
         h.count_async(pcounts.data_ptr() + 4 * s)
-        step()
-    h.count_async(pcounts.data_ptr() + 4 * args.profile_steps)
+        step(h)
+    h.count_async(pcounts.data_ptr() + 4 * nb_steps)
     tim = h.timings()
     h.set_timing(False)
+    h.close()
+    del cfg["x"], cfg["cells"]
     pcl = pcounts.cpu().tolist()
     nodes = mesh["X"].shape[1]
     b_alg = alg_bytes_per_particle_step(dim, nodes, psteps / args.steps)
     achieved_step = b_alg * psteps / args.steps / (ms_per_step * 1e-3) / 1e9
-    nprof = max(1, args.profile_steps)
-    kern = {k: {"launches": n, "avg_ms": ms / n, "ms_per_step": ms / nprof} for k, (n, ms) in tim.items()}
+    kern = {k: {"launches": n, "avg_ms": ms / n, "ms_per_step": ms / nb_steps} for k, (n, ms) in tim.items()}
     dom = max(kern.items(), key=lambda kv: kv[1]["ms_per_step"]) if kern else ("none", {"avg_ms": 0.0, "launches": 0})
-    n_in = sum(pcl[:-1]) / float(nprof)
-    n_out = sum(pcl[1:]) / float(nprof)
+    n_in = sum(pcl[:-1]) / float(nb_steps)
+    n_out = sum(pcl[1:]) / float(nb_steps)
     alg = kernel_alg_bytes(dim, nodes, n_in, n_out)
     dname = dom[0]
     dbytes = alg.get(dname)
-    per_launch = dbytes / (dom[1]["launches"] / float(nprof)) if dbytes is not None else None
+    per_launch = dbytes / (dom[1]["launches"] / float(nb_steps)) if dbytes is not None else None
     achieved = per_launch / (dom[1]["avg_ms"] * 1e-3) / 1e9 if per_launch else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                 "traffic": hbm_traffic(args.workload, args.scale, dname), "kernel": dname,
                 "alg_bytes_per_launch": per_launch, "avg_launch_ms": round(dom[1]["avg_ms"], 4),
+                "launches": dom[1]["launches"],
                 "step": {"achieved": round(achieved_step, 1), "frac": round(achieved_step / HBM_PEAK_GBS, 4),
                          "alg_bytes_per_particle_step": round(b_alg, 2), "particles_per_step": psteps / args.steps}}
     out = {
@@ -369,7 +417,6 @@ def main():
         "kernels_ms_per_step": {k: round(v["ms_per_step"], 4) for k, v in sorted(kern.items())},
         "wall_s": round(wall, 3), "setup_s": round(setup_s, 1),
     }
-    h.close()
     if args.batched_steps > 0:
         out["batched"] = batched_comparison(args.batched_steps, 1, force_tune=args.autotune)
     if args.adjoint_reps > 0:

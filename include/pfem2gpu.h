@@ -89,6 +89,12 @@ int pfem2gpu_insert_particles(pfem2gpu_handle* h, int64_t n, const double* x, co
 /* clear_particles(). src/pfem2particlehandler.cpp:326-330 */
 int pfem2gpu_clear_particles(pfem2gpu_handle* h);
 
+/* Capacity for n particles (the role std::vector::reserve plays for the
+ * reference's particle vector, src/pfem2particlehandler.h:59): device
+ * arrays sized so that moves of up to n particles (plus the refills of one
+ * redistribution) allocate nothing. Growth is otherwise automatic. */
+int pfem2gpu_reserve_particles(pfem2gpu_handle* h, int64_t n);
+
 /* Sets every particle velocity to the interpolant of V (the gather the
  * reference does when it seeds, src/pfem2particlehandler.cpp:94-103). */
 int pfem2gpu_set_velocity_from_field(pfem2gpu_handle* h, const double* const* V, int where);

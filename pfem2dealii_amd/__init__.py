@@ -51,7 +51,7 @@ class Tuning(ctypes.Structure):
 
 EXPORTS = [
     "pfem2gpu_create", "pfem2gpu_destroy", "pfem2gpu_last_error", "pfem2gpu_initialize_maps",
-    "pfem2gpu_seed_particles", "pfem2gpu_insert_particles", "pfem2gpu_clear_particles",
+    "pfem2gpu_seed_particles", "pfem2gpu_insert_particles", "pfem2gpu_clear_particles", "pfem2gpu_reserve_particles",
     "pfem2gpu_set_velocity_from_field", "pfem2gpu_correct_particle_velocity", "pfem2gpu_move_particles",
     "pfem2gpu_project_particle_fields", "pfem2gpu_particle_count", "pfem2gpu_particle_count_async",
     "pfem2gpu_download", "pfem2gpu_cell_counts",
@@ -77,6 +77,7 @@ def load_library(path=None):
     lib.pfem2gpu_seed_particles.argtypes = [_vp, _vp, ctypes.c_int]
     lib.pfem2gpu_insert_particles.argtypes = [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int, _i64p]
     lib.pfem2gpu_clear_particles.argtypes = [_vp]
+    lib.pfem2gpu_reserve_particles.argtypes = [_vp, ctypes.c_int64]
     lib.pfem2gpu_set_velocity_from_field.argtypes = [_vp, _vp, ctypes.c_int]
     lib.pfem2gpu_correct_particle_velocity.argtypes = [_vp, _vp, _vp, ctypes.c_int]
     lib.pfem2gpu_move_particles.argtypes = [_vp, _vp, ctypes.c_int]
@@ -226,6 +227,10 @@ class ParticleHandler:
 
     def clear_particles(self):
         _check(self.lib, self.lib.pfem2gpu_clear_particles(self.h), "clear_particles")
+
+    def reserve_particles(self, n):
+        """Device capacity for n particles (moves of up to n allocate nothing)."""
+        _check(self.lib, self.lib.pfem2gpu_reserve_particles(self.h, int(n)), "reserve_particles")
 
     def set_velocity_from_field(self, V):
         f = _FieldPtrs(V, self.dim, self.n_dofs)

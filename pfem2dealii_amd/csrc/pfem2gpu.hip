@@ -125,8 +125,12 @@ struct pfem2gpu_handle {
   int* qn = nullptr;  // [qn_len] queue lengths of the walk rounds (max n_sub + 1 used)
   int qn_len = 0;
 
+  // device buffers kept between adjoint calls (pointer, bytes)
+  std::vector<std::pair<void*, size_t>> adj_pool;
+
   // identity recording for the adjoint (null when off)
   int* rec_src = nullptr;  // where the next move writes each slot's source particle
+  DPath rec_path = {nullptr, {nullptr, nullptr, nullptr}, 0};  // where the next move records its paths
   int* tsrc = nullptr;     // [tsize] staging source indices
   int* gord = nullptr;     // [tsize] grouping scratch of settle chunks that exceed the LDS stage
 
@@ -358,7 +362,7 @@ void free_particles(pfem2gpu_handle* h) {
 int grow(pfem2gpu_handle* h, long need, long keep, int capc) {
   long ncap = h->cap;
   if (need > h->cap) {
-    ncap = std::max(need, h->cap + h->cap / 2);
+    ncap = std::max(need, 2 * h->cap);
     ncap = std::max(ncap, 1024L);
     if (ncap > 0x7fffffffL) return fail(PFEM2GPU_ERR_MEMORY, "particle capacity exceeds int32 indexing");
   }
@@ -524,14 +528,13 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
   const int use_dv = (advect && h->pending_corr) ? 1 : 0;
   const DField dv = dvfield(h);
   if (!advect) CHK(flush_correction<D>(h));
-  LAUNCH(h, "k_fill", k_fill_int, grid_for(3L * nch + 2), 0, h->count, 3L * nch + 2, 0);
-  LAUNCH(h, "k_fill", k_fill_int, grid_for(2L * nch), 0, reinterpret_cast<int*>(h->lb), 2L * nch, 0);
+  LAUNCH(h, "k_fill", k_zero3, grid_for(5L * nch + 2 + (long)h->qn_len * kQStripes), 0, h->count, 3L * nch + 2,
+         reinterpret_cast<int*>(h->lb), 2L * nch, h->qn, (long)h->qn_len * kQStripes);
   if (advect)
     LAUNCH(h, "k_cellfield", k_cellfield<D>, grid_for((long)h->nc * h->nvc), 0, h->nc, h->cd, Vf, h->Vc);
   int rounds = 0;
   if (advect)
     for (const Prm& q : h->prm_inst) rounds = std::max(rounds, q.n_sub);
-  LAUNCH(h, "k_fill", k_fill_int, grid_for((long)h->qn_len * kQStripes), 0, h->qn, (long)h->qn_len * kQStripes, 0);
   const long gc = move_cell_grid(nmax_p);
   const long qreg = walk_qreg(nmax_p);
   if (qreg * kQStripes > h->qcap) {
@@ -539,12 +542,12 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
     CHK(grow_queues(h, qreg * kQStripes));
   }
   LAUNCH(h, advect ? "k_move" : "k_rebin", k_move_cell<D>, gc, 0, advect, m, h->Vc, nptr, n_add, P, dv, use_dv,
-         h->cshift, h->capc, bcount, spill_base, spill_n, T, dwalk(h, 0), h->qn, qreg);
+         h->cshift, h->capc, bcount, spill_base, spill_n, T, dwalk(h, 0), h->qn, qreg, h->rec_path);
   const int gw = kQStripes * std::max(1, h->walk_grid / kQStripes);
   for (int r = 0; r < rounds; ++r)
     LAUNCH(h, "k_walk", k_move_walk<D>, gw, 0, m, h->Vc, dwalk(h, r & 1), h->qn + (long)r * kQStripes, qreg,
            h->cshift, h->capc, bcount, spill_base, spill_n, T, dwalk(h, (r + 1) & 1),
-           h->qn + (long)(r + 1) * kQStripes);
+           h->qn + (long)(r + 1) * kQStripes, h->rec_path);
   if (use_dv) h->pending_corr = false;
   LAUNCH(h, "k_spill", k_spill_count, grid_for(nmax_p), 0, spill_n, spill_base, h->tcell, h->cshift, scount);
   CHK(scan(h, scount, nch, h->sstart));
@@ -890,6 +893,8 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
   if (!h) return PFEM2GPU_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   free_particles(h);
+  for (auto& b : h->adj_pool) (void)hipFree(b.first);
+  h->adj_pool.clear();
   for (int k = 0; k < 3; ++k) {
     dfree(h->X[k]);
     dfree(h->dV[k]);
@@ -1066,6 +1071,17 @@ int pfem2gpu_clear_particles(pfem2gpu_handle* h) {
   h->contrib_valid = false;
   h->n_ub = 0;
   return PFEM2GPU_OK;
+}
+
+int pfem2gpu_reserve_particles(pfem2gpu_handle* h, int64_t n) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  if (n < 0 || n > 0x7fffffffL) return fail(PFEM2GPU_ERR_ARG, "n out of range");
+  long cur = 0;
+  CHK(read_count(h, &cur));
+  const long need = std::max<long>(n, cur) + h->refill_max;
+  const int capc = capc_for(h, std::max<long>(n, cur));
+  if (need <= h->cap && capc <= h->capc) return PFEM2GPU_OK;
+  return grow(h, std::max(need, h->cap), cur, capc);
 }
 
 int pfem2gpu_set_velocity_from_field(pfem2gpu_handle* h, const double* const* V, int where) {
@@ -1248,15 +1264,30 @@ int pfem2gpu_step(pfem2gpu_handle* h, const double* const* V, double* const* V_p
 
 namespace {
 
+// Device buffers of one adjoint call, taken from the handle's pool of
+// buffers of earlier calls (best fit) and returned to it at the end, so that
+// repeated calls allocate nothing once the pool holds what they need.
 struct DevBuf {
-  std::vector<void*> ptrs;
+  pfem2gpu_handle* h;
+  std::vector<std::pair<void*, size_t>> taken;
+  explicit DevBuf(pfem2gpu_handle* hh) : h(hh) {}
   ~DevBuf() {
-    for (void* p : ptrs) (void)hipFree(p);
+    for (auto& t : taken) h->adj_pool.push_back(t);
   }
   template <typename T>
   int get(T** p, size_t n) {
-    CHK(dalloc(p, n));
-    ptrs.push_back(*p);
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    int best = -1;
+    for (int i = 0; i < (int)h->adj_pool.size(); ++i)
+      if (h->adj_pool[i].second >= bytes && (best < 0 || h->adj_pool[i].second < h->adj_pool[best].second)) best = i;
+    if (best >= 0) {
+      *p = static_cast<T*>(h->adj_pool[best].first);
+      taken.push_back(h->adj_pool[best]);
+      h->adj_pool.erase(h->adj_pool.begin() + best);
+      return PFEM2GPU_OK;
+    }
+    CHK(dalloc(p, std::max<size_t>(n, 1)));
+    taken.push_back({static_cast<void*>(*p), bytes});
     return PFEM2GPU_OK;
   }
 };
@@ -1269,18 +1300,23 @@ struct Snapshot {
   int* cs = nullptr;
 };
 
+// Copies the current particles (count read on the device) and cell_start;
+// s.n is filled in by the caller once the counts are read back.
 int snap_save(pfem2gpu_handle* h, DevBuf& mem, Snapshot& s) {
-  CHK(read_count(h, &s.n));
-  const size_t n = std::max<long>(s.n, 1);
+  const size_t n = std::max<long>(h->cap, 1);
+  for (int k = 0; k < 3; ++k) {
+    s.x[k] = nullptr;
+    s.v[k] = nullptr;
+  }
   for (int k = 0; k < h->dim; ++k) {
     CHK(mem.get(&s.x[k], n));
     CHK(mem.get(&s.v[k], n));
-    HIPCHK(hipMemcpyAsync(s.x[k], h->px[k], s.n * 8, hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(s.v[k], h->pv[k], s.n * 8, hipMemcpyDeviceToDevice, h->stream));
   }
   CHK(mem.get(&s.cell, n));
   CHK(mem.get(&s.cs, h->nc + 1));
-  HIPCHK(hipMemcpyAsync(s.cell, h->pcell, s.n * 4, hipMemcpyDeviceToDevice, h->stream));
+  LAUNCH(h, "k_copy", k_copy_state, grid_for(h->n_ub), 0, h->cell_start + h->nc, h->px[0], h->px[1],
+         h->dim > 2 ? h->px[2] : nullptr, h->pv[0], h->pv[1], h->dim > 2 ? h->pv[2] : nullptr, h->pcell, s.x[0],
+         s.x[1], s.x[2], s.v[0], s.v[1], s.v[2], s.cell);
   HIPCHK(hipMemcpyAsync(s.cs, h->cell_start, (h->nc + 1) * 4, hipMemcpyDeviceToDevice, h->stream));
   return PFEM2GPU_OK;
 }
@@ -1301,7 +1337,7 @@ int snap_restore(pfem2gpu_handle* h, const Snapshot& s) {
 template <int D>
 int adjoint_impl(pfem2gpu_handle* h, int n_steps, int K, const double* const* Vu, const double* const* Tu,
                  double* const* gv0, double* const* gV, double* loss) {
-  DevBuf mem;
+  DevBuf mem(h);
   const int nc = h->nc, nd = h->ndofs;
   DField V, Tg;
   for (int k = 0; k < 3; ++k) {
@@ -1316,22 +1352,53 @@ int adjoint_impl(pfem2gpu_handle* h, int n_steps, int K, const double* const* Vu
   std::vector<Snapshot> ck(nseg);
   std::vector<int*> srcs(n_steps + 1, nullptr), ecell(n_steps + 1, nullptr);
   std::vector<long> cnt(n_steps + 1, 0);
+  int max_sub = 1;
+  for (const Prm& q : h->prm_inst) max_sub = std::max(max_sub, q.n_sub);
+  // paths of one segment's moves (the last segment's are recorded by the
+  // forward itself, the others by their recomputation); the capacity is
+  // secured for a whole segment first, so that it bounds every count the
+  // paths see
+  const int last0 = (nseg - 1) * K + 1;
+  std::vector<DPath> paths(K);
+  int* dcnt = nullptr;  // particle count after each step (device)
+  CHK(mem.get(&dcnt, n_steps + 1));
+  HIPCHK(hipMemcpyAsync(dcnt, h->cell_start + h->nc, 4, hipMemcpyDeviceToDevice, h->stream));
   for (int s = 1; s <= n_steps; ++s) {
     if ((s - 1) % K == 0) CHK(snap_save(h, mem, ck[(s - 1) / K]));
+    if (s == last0) {
+      CHK(reserve(h, (long)(K + 1) * h->refill_max, 0));
+      for (int j = 0; j < K; ++j) {
+        paths[j].stride = h->cap;
+        CHK(mem.get(&paths[j].c, (size_t)max_sub * h->cap));
+        for (int k = 0; k < 3; ++k) paths[j].xi[k] = nullptr;
+        for (int k = 0; k < D; ++k) CHK(mem.get(&paths[j].xi[k], (size_t)max_sub * h->cap));
+      }
+    }
     CHK(reserve(h, h->refill_max, 0));
     if (!h->tsrc || h->tsize == 0) return fail(PFEM2GPU_ERR_STATE, "staging arrays missing");
     CHK(mem.get(&srcs[s], h->cap));
     h->rec_src = srcs[s];
+    if (s >= last0) h->rec_path = paths[s - last0];
     const int rc = rebin_or_move<D>(h, 1, 0, V);
     h->rec_src = nullptr;
+    h->rec_path = DPath{nullptr, {nullptr, nullptr, nullptr}, 0};
     CHK(rc);
-    CHK(read_count(h, &cnt[s]));
-    CHK(mem.get(&ecell[s], std::max<long>(cnt[s], 1)));
-    HIPCHK(hipMemcpyAsync(ecell[s], h->pcell, cnt[s] * 4, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(dcnt + s, h->cell_start + h->nc, 4, hipMemcpyDeviceToDevice, h->stream));
+    CHK(mem.get(&ecell[s], h->cap));
+    LAUNCH(h, "k_copy", k_copy_state, grid_for(h->n_ub), 0, h->cell_start + h->nc, nullptr, nullptr, nullptr,
+           nullptr, nullptr, nullptr, h->pcell, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ecell[s]);
   }
-  cnt[0] = ck[0].n;
   Snapshot fin;
   CHK(snap_save(h, mem, fin));
+  {  // the counts of every step, read back once
+    std::vector<int> hc(n_steps + 1, 0);
+    HIPCHK(hipMemcpyAsync(hc.data(), dcnt, (n_steps + 1) * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    CHK(check_device(h));
+    for (int s = 0; s <= n_steps; ++s) cnt[s] = hc[s];
+    for (int seg = 0; seg < nseg; ++seg) ck[seg].n = cnt[seg * K];
+    fin.n = cnt[n_steps];
+    h->n_ub = cnt[n_steps];
+  }
   // final projection, loss and the projection adjoint
   double *Pn[3] = {nullptr, nullptr, nullptr}, *W = nullptr, *dloss = nullptr;
   for (int k = 0; k < D; ++k) CHK(mem.get(&Pn[k], nd));
@@ -1358,40 +1425,36 @@ int adjoint_impl(pfem2gpu_handle* h, int n_steps, int K, const double* const* Vu
   LAUNCH(h, "k_adj_init", k_adj_init<D>, grid_for(cnt[n_steps]), 0, m, h->cell_start + nc, dparts(h), Pf, W, Tg,
          xa[0], xa[1], xa[2], va[0], va[1], va[2]);
   for (int k = 0; k < D; ++k) LAUNCH(h, "k_fill", k_fill_double, grid_for(nd), 0, gV[k], (long)nd);
+  double* gcv = nullptr;  // dL/dV per (cell, vertex), reduced to the DoFs at the end
+  CHK(mem.get(&gcv, (size_t)nc * h->nvc * D));
+  LAUNCH(h, "k_fill", k_fill_double, grid_for((long)nc * h->nvc * D), 0, gcv, (long)nc * h->nvc * D);
   // backward, one checkpoint segment at a time (latest first): recompute the
-  // segment's start states, then reverse its steps
-  std::vector<double*> sx(K * D, nullptr);
-  std::vector<int*> scell(K, nullptr);
-  for (int j = 0; j < K; ++j) {
-    for (int k = 0; k < D; ++k) CHK(mem.get(&sx[j * D + k], nmax));
-    CHK(mem.get(&scell[j], nmax));
-  }
+  // segment's moves from its checkpoint (all but the last segment, whose
+  // paths the forward recorded), recording every particle's path, then
+  // reverse its steps
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int s0 = seg * K + 1, s1 = std::min(n_steps, seg * K + K);
-    CHK(snap_restore(h, ck[seg]));
-    for (int s = s0; s <= s1; ++s) {
-      const int j = s - s0;
-      for (int k = 0; k < D; ++k)
-        HIPCHK(hipMemcpyAsync(sx[j * D + k], h->px[k], cnt[s - 1] * 8, hipMemcpyDeviceToDevice, h->stream));
-      HIPCHK(hipMemcpyAsync(scell[j], h->pcell, cnt[s - 1] * 4, hipMemcpyDeviceToDevice, h->stream));
-      if (s < s1) CHK(rebin_or_move<D>(h, 1, 0, V));
+    if (seg < nseg - 1) CHK(snap_restore(h, ck[seg]));
+    for (int s = s0; s <= s1 && seg < nseg - 1; ++s) {
+      CHK(reserve(h, h->refill_max, 0));
+      h->rec_path = paths[s - s0];
+      const int rc = rebin_or_move<D>(h, 1, 0, V);
+      h->rec_path = DPath{nullptr, {nullptr, nullptr, nullptr}, 0};
+      CHK(rc);
     }
     for (int s = s1; s >= s0; --s) {
-      const int j = s - s0;
-      for (int k = 0; k < D; ++k) {
-        LAUNCH(h, "k_fill", k_fill_double, grid_for(cnt[s - 1]), 0, xb[k], cnt[s - 1]);
-        LAUNCH(h, "k_fill", k_fill_double, grid_for(cnt[s - 1]), 0, vb[k], cnt[s - 1]);
-      }
+      LAUNCH(h, "k_fill", k_fill_adj, grid_for(cnt[s - 1]), 0, cnt[s - 1], xb[0], xb[1], xb[2], vb[0], vb[1], vb[2]);
       LAUNCH(h, "k_adj_unmap", k_adj_unmap<D>, grid_for(cnt[s]), 0, m, h->pshift, (int)cnt[s], srcs[s], ecell[s],
              xa[0], xa[1], xa[2], va[0], va[1], va[2], xb[0], xb[1], xb[2], vb[0], vb[1], vb[2], gV[0], gV[1], gV[2]);
-      LAUNCH(h, "k_adj_move", k_adj_move<D>, grid_for(cnt[s - 1]), 0, m, h->Vc, (int)cnt[s - 1], sx[j * D],
-             sx[j * D + 1], D > 2 ? sx[j * D + 2] : nullptr, scell[j], xb[0], xb[1], xb[2], gV[0], gV[1], gV[2]);
+      LAUNCH(h, "k_adj_move", k_adj_move<D>, grid_for(cnt[s - 1]), 0, m, h->Vc, (int)cnt[s - 1], max_sub,
+             paths[s - s0], xb[0], xb[1], xb[2], gcv);
       for (int k = 0; k < D; ++k) {
         std::swap(xa[k], xb[k]);
         std::swap(va[k], vb[k]);
       }
     }
   }
+  LAUNCH(h, "k_adj_gather", k_adj_gather<D>, grid_for(nd), 0, nd, h->d2c_off, h->d2c, gcv, gV[0], gV[1], gV[2]);
   for (int k = 0; k < D; ++k)
     HIPCHK(hipMemcpyAsync(gv0[k], va[k], cnt[0] * 8, hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(hipMemcpyAsync(h->h_pinned + 2, dloss, 8, hipMemcpyDeviceToHost, h->stream));

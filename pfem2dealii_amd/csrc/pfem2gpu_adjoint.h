@@ -76,6 +76,16 @@ __device__ __forceinline__ void box_geom(const DMesh& m, int c, double* X0, doub
   }
 }
 
+// Advecting field at vertex i, component k of a cached cell (registers in
+// 2D, the cell-major table otherwise).
+template <int D>
+__device__ __forceinline__ double cell_field(const double* __restrict__ Vc, const CellCache<D>& cc, int i, int k) {
+  if constexpr (D == 2)
+    return cc.Vn[i][k];
+  else
+    return Vc[(size_t)cc.c * (1 << D) * D + i * D + k];
+}
+
 // Shape values and their reference-coordinate gradients.
 template <int D>
 __device__ __forceinline__ void shape_and_grad(const double* xi, double* N, double (*dN)[3]) {
@@ -181,74 +191,142 @@ __global__ __launch_bounds__(kBlock) void k_adj_unmap(DMesh m, int pshift, int n
   }
 }
 
-// Reverse of the substeps of one move for every particle of the step's start
-// state: the forward substeps are recomputed from the checkpointed start
-// position (same arithmetic, so the same cells), then walked backwards:
-//   dL/dV[dof_i]  += dt N_i(xi_k) a            (interpolation reversed: scatter)
-//   a             += dt (du/dx)^T a, du_c/dx_d = sum_i dN_i/dxi_d V_c[i] / h_d
-// `xt` holds dL/dx after the move on entry and before it on exit.
+// dL/dV[k][j] += sum over the (cell, vertex) entries of DoF j of the
+// per-(cell, vertex) accumulator (ascending cell, as the projection sums).
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __restrict__ Vc, int n_start,
-                                                    const double* xs0, const double* xs1, const double* xs2,
-                                                    const int* __restrict__ cs, double* xt0, double* xt1,
-                                                    double* xt2, double* g0, double* g1, double* g2) {
-  constexpr int NV = 1 << D;
-  const double* xs[3] = {xs0, xs1, xs2};
-  double* xt[3] = {xt0, xt1, xt2};
+__global__ __launch_bounds__(kBlock) void k_adj_gather(int ndofs, const int* __restrict__ d2c_off,
+                                                      const int* __restrict__ d2c, const double* __restrict__ gcv,
+                                                      double* g0, double* g1, double* g2) {
   double* g[3] = {g0, g1, g2};
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_start; p += gridDim.x * blockDim.x) {
-    const int c0 = cs[p];
-    CellCache<D> cc;
-    load_cell<D>(m, Vc, c0, cc, true);
-    const Prm& pr = m.prm[inst_of(m, c0)];
-    const int n_sub = pr.n_sub;
-    const double dt = pr.dt_sub;
-    double x[D], xi[D];
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ndofs; j += gridDim.x * blockDim.x) {
+    double S[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) x[k] = xs[k][p];
-    inv_cached<D>(m, cc, x, xi);
-    snap_cached<D>(m, pr, cc, x, xi);
-    double path_xi[kAdjMaxSub][D];
-    int path_c[kAdjMaxSub];
-    bool alive = true;
-    for (int s = 0; s < n_sub; ++s) {
+    for (int k = 0; k < D; ++k) S[k] = 0.0;
+    for (int e = d2c_off[j]; e < d2c_off[j + 1]; ++e)
 #pragma unroll
-      for (int k = 0; k < D; ++k) path_xi[s][k] = xi[k];
-      path_c[s] = cc.c;
-      double u[D];
-      gather_cached<D>(Vc, cc, xi, u);
+      for (int k = 0; k < D; ++k) S[k] = S[k] + gcv[(size_t)d2c[e] * D + k];
 #pragma unroll
-      for (int k = 0; k < D; ++k) {
-        const double t = u[k] * dt;
-        x[k] = x[k] + t;
-      }
-      double xn[D];
-      if (inv_cached<D>(m, cc, x, xn) && inside<D>(xn)) {
+    for (int k = 0; k < D; ++k) g[k][j] = g[k][j] + S[k];
+  }
+}
+
+// Scatter of per-lane contributions add[i][k] into the per-(cell, vertex)
+// accumulator gcv of the lane's cell (key < 0: nothing), aggregated over the
+// lanes of the wavefront that share the cell: up to kAdjGroups distinct
+// cells are summed in LDS and flushed with one global atomic per (cell,
+// vertex, component); lanes beyond that add directly. Wave-uniform.
+constexpr int kAdjGroups = 16;
+
+// Orders this wavefront's LDS accesses around the point (waits for its LDS
+// operations only).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt and expcnt left at their maxima
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int D>
+__device__ __forceinline__ void adj_scatter(int key, const double (*add)[3], double* __restrict__ gcv, int* s_gcell,
+                                            double* s_acc) {
+  constexpr int NV = 1 << D;
+  constexpr int W = NV * D;
+  const int lane = threadIdx.x & 63;
+  int gid = -1, ng = 0;
+  unsigned long long pending = __ballot(key >= 0);
+  while (pending && ng < kAdjGroups) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int lk = __shfl(key, leader, 64);
+    const unsigned long long grp = __ballot(key == lk) & pending;
+    if ((grp >> lane) & 1ULL) gid = ng;
+    if (lane == leader) s_gcell[ng] = lk;
+    pending &= ~grp;
+    ++ng;
+  }
+  for (int l = lane; l < ng * W; l += 64) s_acc[l] = 0.0;
+  wave_lds_sync();
+  if (gid >= 0) {
 #pragma unroll
-        for (int k = 0; k < D; ++k) xi[k] = xn[k];
-      } else {
-        const int c2 = walk_cached<D>(m, cc, xi, x, u, xn);
-        if (c2 < 0) {
-          alive = false;
-          break;
-        }
-        load_cell<D>(m, Vc, c2, cc, true);
+    for (int i = 0; i < NV; ++i)
 #pragma unroll
-        for (int k = 0; k < D; ++k) xi[k] = xn[k];
+      for (int k = 0; k < D; ++k) atomicAdd(&s_acc[gid * W + i * D + k], add[i][k]);
+  } else if (key >= 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int k = 0; k < D; ++k) atomicAdd(&gcv[((size_t)key * NV + i) * D + k], add[i][k]);
+  }
+  wave_lds_sync();
+  for (int l = lane; l < ng * W; l += 64) {
+    const int gi = l / W;
+    atomicAdd(&gcv[(size_t)s_gcell[gi] * W + (l - gi * W)], s_acc[l]);
+  }
+  wave_lds_sync();
+}
+
+// Reverse of the substeps of one move for every particle of the step's start
+// state, from the path the recomputed move recorded (cell and reference
+// position at the start of each substep), walked backwards:
+//   dL/dV[dof_i]  += dt N_i(xi_s) a            (interpolation reversed: scatter
+//                                                into the per-(cell, vertex)
+//                                                accumulator gcv)
+//   a             += dt (du/dx)^T a, du_c/dx_d = sum_i dN_i/dxi_d V_c[i] / h_d
+// `xt` holds dL/dx after the move on entry and before it on exit. A particle
+// the move deleted has no adjoint (it is no survivor), so lanes with a zero
+// adjoint skip their substeps (a zero adjoint contributes nothing). The
+// loops are uniform across each wavefront (max_sub substeps) for the
+// aggregated scatter.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __restrict__ Vc, int n_start, int max_sub,
+                                                    DPath path, double* xt0, double* xt1, double* xt2,
+                                                    double* __restrict__ gcv) {
+  constexpr int NV = 1 << D;
+  __shared__ int s_gcell[kBlock / 64][kAdjGroups];
+  __shared__ double s_acc[kBlock / 64][kAdjGroups * NV * D];
+  const int wave = threadIdx.x >> 6;
+  double* xt[3] = {xt0, xt1, xt2};
+  for (long base = (long)blockIdx.x * blockDim.x; base < n_start; base += (long)gridDim.x * blockDim.x) {
+    const int p = (int)(base + threadIdx.x);
+    double a[D];
+    bool live = false;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      a[k] = p < n_start ? xt[k][p] : 0.0;
+      live = live || a[k] != 0.0;
+    }
+    int n_sub = 0;
+    double dt = 0.0;
+    int pc[kAdjMaxSub];
+    double pxi[kAdjMaxSub][D];
+#pragma unroll
+    for (int s = 0; s < kAdjMaxSub; ++s) {  // the whole path, one round of loads
+      pc[s] = -1;
+#pragma unroll
+      for (int k = 0; k < D; ++k) pxi[s][k] = 0.0;
+      if (live && s < max_sub) {
+        const long o = s * path.stride + p;
+        pc[s] = path.c[o];
+#pragma unroll
+        for (int k = 0; k < D; ++k) pxi[s][k] = path.xi[k][o];
       }
     }
-    double a[D];
+    if (live) {
+      const Prm& pr = m.prm[inst_of(m, pc[0])];
+      n_sub = pr.n_sub;
+      dt = pr.dt_sub;
+    }
+    CellCache<D> cc;
+    cc.c = -1;
 #pragma unroll
-    for (int k = 0; k < D; ++k) a[k] = alive ? xt[k][p] : 0.0;
-    if (alive) {
-      for (int s = n_sub - 1; s >= 0; --s) {
-        const int c = path_c[s];
-        double N[NV], dN[NV][3], X0[D], RH[D];
-        shape_and_grad<D>(path_xi[s], N, dN);
-        box_geom<D>(m, c, X0, RH);
-        int dof[NV];
-        load_ids<D>(m.cd, c, dof);
-        const double* f = Vc + (size_t)c * NV * D;
+    for (int s = kAdjMaxSub - 1; s >= 0; --s) {
+      if (s >= max_sub) continue;
+      const bool act = live && s < n_sub;
+      double add[NV][3];
+      int key = -1;
+      if (act) {
+        const int c = pc[s];
+        if (c != cc.c) load_cell<D>(m, Vc, c, cc, true);  // particles mostly keep their cell
+        key = c;
+        double N[NV], dN[NV][3];
+        shape_and_grad<D>(pxi[s], N, dN);
         double gx[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) gx[d] = 0.0;
@@ -257,18 +335,53 @@ __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __re
           double av = 0.0;
 #pragma unroll
           for (int k = 0; k < D; ++k) {
-            atomicAdd(&g[k][dof[i]], dt * N[i] * a[k]);
-            av += a[k] * f[i * D + k];
+            add[i][k] = dt * N[i] * a[k];
+            av += a[k] * cell_field<D>(Vc, cc, i, k);
           }
 #pragma unroll
           for (int d = 0; d < D; ++d) gx[d] += dN[i][d] * av;
         }
 #pragma unroll
-        for (int d = 0; d < D; ++d) a[d] += dt * gx[d] * RH[d];
+        for (int d = 0; d < D; ++d) a[d] += dt * gx[d] * cc.RH[d];
       }
+      adj_scatter<D>(key, add, gcv, s_gcell[wave], s_acc[wave]);
     }
+    if (p < n_start) {
 #pragma unroll
-    for (int k = 0; k < D; ++k) xt[k][p] = a[k];
+      for (int k = 0; k < D; ++k) xt[k][p] = a[k];
+    }
+  }
+}
+
+// Copies the first *n_ptr entries of up to six double arrays and one int
+// array (the count lives on the device, so no host round trip is needed).
+__global__ void k_copy_state(const int* __restrict__ n_ptr, const double* s0, const double* s1, const double* s2,
+                             const double* s3, const double* s4, const double* s5, const int* sc, double* d0,
+                             double* d1, double* d2, double* d3, double* d4, double* d5, int* dc) {
+  const long n = *n_ptr;
+  const double* src[6] = {s0, s1, s2, s3, s4, s5};
+  double* dst[6] = {d0, d1, d2, d3, d4, d5};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+      if (src[a]) dst[a][i] = src[a][i];
+    if (sc) dc[i] = sc[i];
+  }
+}
+
+// Zeroes the velocity and position adjoints of a step's start state.
+__global__ void k_fill_adj(long n, double* a0, double* a1, double* a2, double* b0, double* b1, double* b2) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    a0[i] = 0.0;
+    b0[i] = 0.0;
+    if (a1) {
+      a1[i] = 0.0;
+      b1[i] = 0.0;
+    }
+    if (a2) {
+      a2[i] = 0.0;
+      b2[i] = 0.0;
+    }
   }
 }
 

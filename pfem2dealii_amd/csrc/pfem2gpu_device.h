@@ -79,6 +79,15 @@ struct DWalk {
   int* src;
 };
 
+// Optional record of every particle's path through a move: the cell and the
+// reference position at the start of each substep, [s * stride + p] for the
+// particle at index p of the move's input (c == nullptr: off).
+struct DPath {
+  int* c;
+  double* xi[3];
+  long stride;
+};
+
 __device__ __forceinline__ int inst_of(const DMesh& m, int c) { return m.cell_inst ? m.cell_inst[c] : 0; }
 
 template <int D>

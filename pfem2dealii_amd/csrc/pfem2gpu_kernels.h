@@ -15,6 +15,19 @@ __global__ void k_fill_int(int* __restrict__ a, long n, int value) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] = value;
 }
 
+// Zeroes three int arrays in one launch (the per-move counters).
+__global__ void k_zero3(int* __restrict__ a, long na, int* __restrict__ b, long nb, int* __restrict__ c, long nc) {
+  const long n = na + nb + nc;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    if (i < na)
+      a[i] = 0;
+    else if (i < na + nb)
+      b[i - na] = 0;
+    else
+      c[i - na - nb] = 0;
+  }
+}
+
 // dV = V - V_old per DoF (src/pfem2particlehandler.cpp:120-124)
 template <int D>
 __global__ void k_sub(int ndofs, DField V, DField Vold, double* __restrict__ d0, double* __restrict__ d1,
@@ -237,8 +250,15 @@ __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, i
 // substep, xi the reference position before it and u its velocity).
 template <int D>
 __device__ __forceinline__ bool substeps_in_cell(const DMesh& m, const double* __restrict__ Vc, const CellCache<D>& cc,
-                                                 int n_sub, double dt_sub, int& s, double* x, double* xi, double* u) {
+                                                 int n_sub, double dt_sub, int& s, double* x, double* xi, double* u,
+                                                 const DPath& path, int pidx) {
   for (; s < n_sub; ++s) {
+    if (path.c) {
+      const long o = s * path.stride + pidx;
+      path.c[o] = cc.c;
+#pragma unroll
+      for (int k = 0; k < D; ++k) path.xi[k][o] = xi[k];
+    }
     gather_cached<D>(Vc, cc, xi, u);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -265,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const
                                                      const int* __restrict__ n_ptr, int n_add, DParts P, DField dV,
                                                      int use_dv, int cshift, int capc, int* __restrict__ bcount,
                                                      long spill_base, int* __restrict__ spill_n, DTmp T, DWalk Q,
-                                                     int* __restrict__ qn, long qreg) {
+                                                     int* __restrict__ qn, long qreg, DPath path) {
   const int n = *n_ptr + n_add;
   for (long base = (long)blockIdx.x * blockDim.x; base < n; base += (long)gridDim.x * blockDim.x) {
     const int p = (int)(base + threadIdx.x);
@@ -301,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const
 #pragma unroll
           for (int k = 0; k < D; ++k) v[k] = v[k] + corr[k];
         }
-        walk = substeps_in_cell<D>(m, Vc, cc, advect ? pr.n_sub : 0, pr.dt_sub, s, x, xi, u);
+        walk = substeps_in_cell<D>(m, Vc, cc, advect ? pr.n_sub : 0, pr.dt_sub, s, x, xi, u, path, p);
         if (!walk) key = c >> cshift;
       }
     }
@@ -317,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __r
                                                      const int* __restrict__ qn_in, long qreg, int cshift, int capc,
                                                      int* __restrict__ bcount, long spill_base,
                                                      int* __restrict__ spill_n, DTmp T, DWalk Qout,
-                                                     int* __restrict__ qn_out) {
+                                                     int* __restrict__ qn_out, DPath path) {
   const int stripe = blockIdx.x % kQStripes;
   const int per = gridDim.x / kQStripes;
   const int nq = qn_in[stripe];
@@ -357,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __r
 #pragma unroll
         for (int k = 0; k < D; ++k) xi[k] = xn[k];
         ++s;
-        walk = substeps_in_cell<D>(m, Vc, cc, pr.n_sub, pr.dt_sub, s, x, xi, u);
+        walk = substeps_in_cell<D>(m, Vc, cc, pr.n_sub, pr.dt_sub, s, x, xi, u, path, src);
         if (!walk) key = c2 >> cshift;
       }
     }

@@ -705,25 +705,19 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
 
   // registers: the particles of this thread in the current chunk
   int pc[PRE];
-  double px[PRE][D], pv[PRE][D];
+  double px[PRE][D];
   auto load_parts = [&](int n, int nb, long bucket0, long spill0) {
 #pragma unroll
     for (int u = 0; u < PRE; ++u) {
       const int j = threadIdx.x + u * blockDim.x;
       pc[u] = -1;
 #pragma unroll
-      for (int k = 0; k < D; ++k) {
-        px[u][k] = 0.0;
-        pv[u][k] = 0.0;
-      }
+      for (int k = 0; k < D; ++k) px[u][k] = 0.0;
       if (j < n) {
         const long i = chunk_index(j, nb, bucket0, spill0);
         pc[u] = T.cell[i];
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-          px[u][k] = T.x[k][i];
-          pv[u][k] = T.v[k][i];
-        }
+        for (int k = 0; k < D; ++k) px[u][k] = T.x[k][i];
       }
     }
   };
@@ -778,6 +772,18 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
     // A. reference position, part and key of every particle; counts per
     //    (cell, part)
     int ppl[PRE];
+    double pv[PRE][D];  // velocities: loaded here, stored with the classification
+#pragma unroll
+    for (int u = 0; u < PRE; ++u) {
+      const int j = threadIdx.x + u * blockDim.x;
+#pragma unroll
+      for (int k = 0; k < D; ++k) pv[u][k] = 0.0;
+      if (j < n && staged_in) {
+        const long i = chunk_index(j, nb, bucket0, spill0);
+#pragma unroll
+        for (int k = 0; k < D; ++k) pv[u][k] = T.v[k][i];
+      }
+    }
 #pragma unroll
     for (int u = 0; u < PRE; ++u) {
       const int j = threadIdx.x + u * blockDim.x;
@@ -1093,13 +1099,14 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
         P.cell[o0 + t] = c0 + lo;
         if (P.src) P.src[o0 + t] = s_fsrc[t];
       }
-      for (int t = threadIdx.x; t < ncell * NV; t += blockDim.x) {
-        const int lc = t / NV, iv = t - lc * NV;
+      // one thread per (cell, vertex, component), component D being W
+      for (int t = threadIdx.x; t < ncell * NV * (D + 1); t += blockDim.x) {
+        const int q = t / (D + 1), comp = t - q * (D + 1);
+        const int lc = q / NV, iv = q - lc * NV;
         const int c = c0 + lc;
         const int a = s_cfin[lc], b = (lc + 1 < ncell) ? s_cfin[lc + 1] : nout;
-        double S[D], W = 0.0;
-#pragma unroll
-        for (int k = 0; k < D; ++k) S[k] = 0.0;
+        const double* vv = s_fv + (comp < D ? comp : 0) * nslots;
+        double S = 0.0;
         for (int f = a; f < b; ++f) {
           double N = 1.0;
 #pragma unroll
@@ -1108,14 +1115,9 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
             const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
             N = (k == 0) ? l : N * l;
           }
-#pragma unroll
-          for (int k = 0; k < D; ++k) S[k] = S[k] + N * s_fv[k * nslots + f];
-          W = W + N;
+          S = comp < D ? S + N * vv[f] : S + N;
         }
-        double* e = contrib + ((size_t)c * NV + iv) * (D + 1);
-#pragma unroll
-        for (int k = 0; k < D; ++k) e[k] = S[k];
-        e[D] = W;
+        contrib[((size_t)c * NV + iv) * (D + 1) + comp] = S;
       }
     }
     if (nch < nchunks) cells_store<D>(cr, nncell, cbuf_geo(buf ^ 1), cbuf_vc(buf ^ 1), cbuf_prm(buf ^ 1), cbuf_inst(buf ^ 1));

@@ -77,6 +77,7 @@ struct pfem2gpu_handle {
   std::vector<long> inst_dof_off;    // [ninst+1]
   double* geom = nullptr;  // [nc][2*dim] box corners (X_0[0] = NaN for other cells)
   int2* cvo = nullptr;     // [nc][nvc] (v2c offset, count) per cell vertex
+  int4* cvc = nullptr;     // 2D: [nc][4] cells around each cell vertex (null in 3D)
   int* ptab = nullptr;     // [pmax] part decode table
   double* Vc = nullptr;    // [nc][nvc][dim] advecting field, cell-major
   int pshift = 0;
@@ -103,7 +104,7 @@ struct pfem2gpu_handle {
   int* scan_blk = nullptr;
   int scan_nb = 0;
   double* contrib = nullptr;  // [nc][nvc][dim+1]
-  double* dV[3] = {nullptr, nullptr, nullptr};
+  double* dVc = nullptr;  // [nc][nvc][dim] pending correction V - V_old, cell-major
   bool pending_corr = false;
   double* stageF[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   unsigned long long* d_rejected = nullptr;
@@ -246,6 +247,7 @@ DMesh dmesh(pfem2gpu_handle* h) {
   m.prm = h->prm;
   m.geom = h->geom;
   m.cvo = h->cvo;
+  m.cvc = h->cvc;
   m.ptab = h->ptab;
   return m;
 }
@@ -490,17 +492,11 @@ int field_in(pfem2gpu_handle* h, const double* const* F, int where, int slot, DF
   return PFEM2GPU_OK;
 }
 
-DField dvfield(pfem2gpu_handle* h) {
-  DField f;
-  for (int k = 0; k < 3; ++k) f.c[k] = h->dV[k];
-  return f;
-}
 
 template <int D>
 int flush_correction(pfem2gpu_handle* h) {
   if (!h->pending_corr) return PFEM2GPU_OK;
-  LAUNCH(h, "k_correct", k_correct<D>, grid_for(h->n_ub), 0, dmesh(h), h->cell_start + h->nc, dparts(h),
-         dvfield(h));
+  LAUNCH(h, "k_correct", k_correct<D>, grid_for(h->n_ub), 0, dmesh(h), h->cell_start + h->nc, dparts(h), h->dVc);
   h->pending_corr = false;
   h->contrib_valid = false;
   return PFEM2GPU_OK;
@@ -526,7 +522,6 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
   const long spill_base = (long)nch * h->capc;
   const long spill2_base = spill_base + h->cap;
   const int use_dv = (advect && h->pending_corr) ? 1 : 0;
-  const DField dv = dvfield(h);
   if (!advect) CHK(flush_correction<D>(h));
   LAUNCH(h, "k_fill", k_zero3, grid_for(5L * nch + 2 + (long)h->qn_len * kQStripes), 0, h->count, 3L * nch + 2,
          reinterpret_cast<int*>(h->lb), 2L * nch, h->qn, (long)h->qn_len * kQStripes);
@@ -541,7 +536,7 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
     if (h->capturing) return fail(PFEM2GPU_ERR_STATE, "walk queues too small during graph capture");
     CHK(grow_queues(h, qreg * kQStripes));
   }
-  LAUNCH(h, advect ? "k_move" : "k_rebin", k_move_cell<D>, gc, 0, advect, m, h->Vc, nptr, n_add, P, dv, use_dv,
+  LAUNCH(h, advect ? "k_move" : "k_rebin", k_move_cell<D>, gc, 0, advect, m, h->Vc, nptr, n_add, P, h->dVc, use_dv,
          h->cshift, h->capc, bcount, spill_base, spill_n, T, dwalk(h, 0), h->qn, qreg, h->rec_path);
   const int gw = kQStripes * std::max(1, h->walk_grid / kQStripes);
   for (int r = 0; r < rounds; ++r)
@@ -754,7 +749,9 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
       if (!boxh[c]) g[0] = std::nan("");
     }
     h->all_box = std::all_of(boxh.begin(), boxh.end(), [](unsigned char b) { return b != 0; });
-    if ((rc = dalloc(&h->geom, geom.size())) || (rc = dalloc(&h->Vc, (size_t)h->nc * nvc * dim))) return bail(rc);
+    if ((rc = dalloc(&h->geom, geom.size())) || (rc = dalloc(&h->Vc, (size_t)h->nc * nvc * dim)) ||
+        (rc = dalloc(&h->dVc, (size_t)h->nc * nvc * dim)))
+      return bail(rc);
     if (hipMemcpy(h->geom, geom.data(), geom.size() * 8, hipMemcpyHostToDevice))
       return bail(fail(PFEM2GPU_ERR_HIP, "upload geometry"));
   }
@@ -812,7 +809,7 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
   for (const Prm& q : prms) h->qn_len = std::max(h->qn_len, q.n_sub + 2);
   if ((rc = dalloc(&h->qn, (size_t)h->qn_len * kQStripes))) return bail(rc);
   for (int k = 0; k < dim; ++k)
-    if ((rc = dalloc(&h->dV[k], h->ndofs)) || (rc = dalloc(&h->stageF[0][k], h->ndofs)) ||
+    if ((rc = dalloc(&h->stageF[0][k], h->ndofs)) ||
         (rc = dalloc(&h->stageF[1][k], h->ndofs)))
       return bail(rc);
   if (hipHostMalloc(reinterpret_cast<void**>(&h->h_pinned), sizeof(int) * 4, 0) != hipSuccess)
@@ -897,7 +894,6 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
   h->adj_pool.clear();
   for (int k = 0; k < 3; ++k) {
     dfree(h->X[k]);
-    dfree(h->dV[k]);
     dfree(h->stageF[0][k]);
     dfree(h->stageF[1][k]);
   }
@@ -913,6 +909,7 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
   dfree(h->cell_inst);
   dfree(h->geom);
   dfree(h->cvo);
+  dfree(h->cvc);
   dfree(h->ptab);
   dfree(h->Vc);
   dfree(h->cell_start);
@@ -920,6 +917,7 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
   dfree(h->sstart);
   dfree(h->lb);
   dfree(h->d_err);
+  dfree(h->dVc);
   dfree(h->qn);
   dfree(h->scan_blk);
   dfree(h->contrib);
@@ -988,6 +986,21 @@ int pfem2gpu_initialize_maps(pfem2gpu_handle* h) {
   dfree(h->cvo);
   CHK(dalloc(&h->cvo, cvo.size()));
   HIPCHK(hipMemcpy(h->cvo, cvo.data(), cvo.size() * sizeof(int2), hipMemcpyHostToDevice));
+  dfree(h->cvc);
+  if (dim == 2) {  // the cells around each cell vertex, for the 2D walk
+    std::vector<int4> cvc((size_t)h->nc * 4);
+    for (size_t e = 0; e < cvc.size(); ++e) {
+      const int2 r = cvo[e];
+      int ids[4] = {-1, -1, -1, -1};
+      if (r.y > 4)
+        ids[0] = -2;
+      else
+        for (int j = 0; j < r.y; ++j) ids[j] = lst2[r.x + j];
+      cvc[e] = make_int4(ids[0], ids[1], ids[2], ids[3]);
+    }
+    CHK(dalloc(&h->cvc, cvc.size()));
+    HIPCHK(hipMemcpy(h->cvc, cvc.data(), cvc.size() * sizeof(int4), hipMemcpyHostToDevice));
+  }
   dfree(h->v2c_off);
   dfree(h->v2c);
   dfree(h->v2c_dir);
@@ -1105,9 +1118,9 @@ int pfem2gpu_correct_particle_velocity(pfem2gpu_handle* h, const double* const* 
   CHK(field_in(h, V, where, 0, &A));
   CHK(field_in(h, V_old, where, 1, &B));
   if (h->dim == 2)
-    LAUNCH(h, "k_sub", k_sub<2>, grid_for(h->ndofs), 0, h->ndofs, A, B, h->dV[0], h->dV[1], h->dV[2]);
+    LAUNCH(h, "k_sub", k_subc<2>, grid_for((long)h->nc * 4), 0, h->nc, h->cd, A, B, h->dVc);
   else
-    LAUNCH(h, "k_sub", k_sub<3>, grid_for(h->ndofs), 0, h->ndofs, A, B, h->dV[0], h->dV[1], h->dV[2]);
+    LAUNCH(h, "k_sub", k_subc<3>, grid_for((long)h->nc * 8), 0, h->nc, h->cd, A, B, h->dVc);
   h->pending_corr = true;
   return PFEM2GPU_OK;
 }

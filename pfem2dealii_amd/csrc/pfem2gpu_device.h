@@ -44,6 +44,8 @@ struct DMesh {
   const Prm* prm;                // per-instance parameters
   const double* geom;            // [nc][3D]: X_0, X_{2^D-1}, 1/(X_{2^D-1} - X_0) of box cells; X_0[0] = NaN otherwise
   const int2* cvo;               // [nc][2^D]: (v2c offset, count) of each cell vertex
+  const int4* cvc;               // 2D: [nc][4] the (up to 4) cells around each cell vertex, in v2c order,
+                                 // -1 padded; x = -2 when the vertex has more (null in 3D)
   const int* ptab;               // [ninst][pmax]: refillable part -> INT_MIN | pi_0 | pi_1<<8 | pi_2<<16, else 0
 };
 
@@ -806,6 +808,111 @@ __device__ int walk_box(const DMesh& m, const CellCache<D>& cc, const double* xi
     }
   }
   return -1;
+}
+
+// 2D neighbour walk from a box cell with the candidate lists of the cell's
+// four vertices already loaded (r0..r3, from the per-cell table): the same
+// result as walk_box. The candidates' geometry is read in one round of
+// loads; the winner's corners are returned in gw (X0, X1, RH) so that the
+// caller does not read them again. Returns -2 when the closest vertex has
+// more than four cells (the caller then walks through walk_cached).
+__device__ __forceinline__ int sel4(int b, int a0, int a1, int a2, int a3) {
+  return b == 3 ? a3 : (b == 2 ? a2 : (b == 1 ? a1 : a0));
+}
+
+__device__ __forceinline__ int walk_box2(const DMesh& m, const CellCache<2>& cc, const double* xi, const double* x,
+                                         const double* u, int4 r0, int4 r1, int4 r2, int4 r3, double* xi_new,
+                                         double* gw) {
+  double xold[2];
+  fwd_box<2>(cc, xi, xold);
+  const int bestv = closest_vertex_box<2>(cc, xold, u);
+  const int ids[4] = {sel4(bestv, r0.x, r1.x, r2.x, r3.x), sel4(bestv, r0.y, r1.y, r2.y, r3.y),
+                      sel4(bestv, r0.z, r1.z, r2.z, r3.z), sel4(bestv, r0.w, r1.w, r2.w, r3.w)};
+  if (ids[0] == -2) return -2;
+  unsigned hits = 0;
+  int first = -1;
+  double xf[2] = {0.0, 0.0};
+  double gf[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // corners of the first containing candidate
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (ids[j] < 0 || ids[j] == cc.c) continue;
+    double g[6];
+    const double* gp = m.geom + (size_t)ids[j] * 6;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) g[q] = gp[q];
+    double xn[2];
+    bool ok;
+    if (g[0] == g[0]) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) xn[k] = div_by(x[k] - g[k], g[2 + k] - g[k], g[4 + k]);
+      ok = true;
+    } else {
+      int vid2[4];
+      load_ids<2>(m.cv, ids[j], vid2);
+      ok = inv_map_newton<2>(m, vid2, x, xn);
+    }
+    if (ok && inside<2>(xn)) {
+      hits |= 1u << j;
+      if (first < 0) {
+        first = j;
+        xf[0] = xn[0];
+        xf[1] = xn[1];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) gf[q] = g[q];
+      }
+    }
+  }
+  if (hits == 0) return -1;
+  int win = first;  // a single containing candidate wins in any order
+  if ((hits & (hits - 1)) != 0) {
+    // two or more containing candidates: the first in the reference's order
+    const int o0 = m.cvo[(size_t)cc.c * 4 + bestv].x;
+    const int nn = m.cvo[(size_t)cc.c * 4 + bestv].y;
+    double vtp[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) vtp[k] = x[k] - (((bestv >> k) & 1) ? cc.X1[k] : cc.X0[k]);
+    const double t2 = vtp[0] * vtp[0] + vtp[1] * vtp[1];
+    const double inv_t = 1.0 / sqrt(t2);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) vtp[k] = vtp[k] * inv_t;
+    double dots[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) dots[j] = j < nn ? dot_dir<2>(m.v2c_dir + (size_t)(o0 + j) * 2, vtp) : 0.0;
+    unsigned used = 0;
+    win = -1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (r >= nn || win >= 0) break;
+      int bj = -1;
+      double bd = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j < nn && !((used >> j) & 1u)) {
+          if (bj < 0 || dots[j] > bd) {
+            bj = j;
+            bd = dots[j];
+          }
+        }
+      }
+      used |= 1u << bj;
+      if ((hits >> bj) & 1u) win = bj;
+    }
+  }
+  int wid = ids[0];
+#pragma unroll
+  for (int j = 1; j < 4; ++j)
+    if (win == j) wid = ids[j];
+  if (win != first) {  // the reference's order picked another containing candidate
+    inv_candidate<2>(m, wid, x, xf);
+    const double* gp = m.geom + (size_t)wid * 6;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) gf[q] = gp[q];
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) xi_new[k] = xf[k];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) gw[q] = gf[q];
+  return wid;
 }
 
 // Neighbour walk for the particle in cache cc (box fast path or generic).

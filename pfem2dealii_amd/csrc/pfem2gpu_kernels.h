@@ -28,15 +28,35 @@ __global__ void k_zero3(int* __restrict__ a, long na, int* __restrict__ b, long 
   }
 }
 
-// dV = V - V_old per DoF (src/pfem2particlehandler.cpp:120-124)
+// dV = V - V_old (src/pfem2particlehandler.cpp:120-124), stored cell-major
+// like the advecting field: dVc[c][i][k] = V_k[dof(c, i)] - V_old_k[dof(c, i)]
+// (the same subtraction per DoF, repeated for each incident cell), so the
+// correction reads a cell's values with one contiguous load.
 template <int D>
-__global__ void k_sub(int ndofs, DField V, DField Vold, double* __restrict__ d0, double* __restrict__ d1,
-                      double* __restrict__ d2) {
-  double* out[3] = {d0, d1, d2};
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ndofs; j += gridDim.x * blockDim.x) {
+__global__ void k_subc(int nc, const int* __restrict__ cd, DField V, DField Vold, double* __restrict__ dVc) {
+  constexpr int NV = 1 << D;
+  const long total = (long)nc * NV;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int dof = cd[t];
 #pragma unroll
-    for (int k = 0; k < D; ++k) out[k][j] = V.c[k][j] - Vold.c[k][j];
+    for (int k = 0; k < D; ++k) dVc[t * D + k] = V.c[k][dof] - Vold.c[k][dof];
   }
+}
+
+// u_k = sum_i N_i(xi) F[c][i][k] from a cell-major table, vertex-outer,
+// component-inner, from 0.0.
+template <int D>
+__device__ __forceinline__ void gather_table(const double* __restrict__ F, int c, const double* xi, double* u) {
+  constexpr int NV = 1 << D;
+  double N[NV];
+  shape<D>(xi, N);
+  const double* f = F + (size_t)c * NV * D;
+#pragma unroll
+  for (int k = 0; k < D; ++k) u[k] = 0.0;
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+#pragma unroll
+    for (int k = 0; k < D; ++k) u[k] = u[k] + N[i] * f[i * D + k];
 }
 
 // Advecting field in cell-major order: Vc[c][i][k] = V_k[dof(c, i)], so the
@@ -282,8 +302,9 @@ __device__ __forceinline__ bool substeps_in_cell(const DMesh& m, const double* _
 // correct_particle_velocity would have done before the move.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const double* __restrict__ Vc,
-                                                     const int* __restrict__ n_ptr, int n_add, DParts P, DField dV,
-                                                     int use_dv, int cshift, int capc, int* __restrict__ bcount,
+                                                     const int* __restrict__ n_ptr, int n_add, DParts P,
+                                                     const double* __restrict__ dVc, int use_dv, int cshift, int capc,
+                                                     int* __restrict__ bcount,
                                                      long spill_base, int* __restrict__ spill_n, DTmp T, DWalk Q,
                                                      int* __restrict__ qn, long qreg, DPath path) {
   const int n = *n_ptr + n_add;
@@ -315,9 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const
         snap_cached<D>(m, pr, cc, x, xi);
         if (use_dv) {
           double corr[D];
-          int dof[1 << D];
-          load_ids<D>(m.cd, c, dof);
-          gather<D>(dV, dof, xi, corr);
+          gather_table<D>(dVc, c, xi, corr);
 #pragma unroll
           for (int k = 0; k < D; ++k) v[k] = v[k] + corr[k];
         }
@@ -368,12 +387,50 @@ __global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __r
         v[k] = Qin.v[k][i];
         u[k] = Qin.u[k][i];
       }
+      int4 r0 = make_int4(-2, -1, -1, -1), r1 = r0, r2 = r0, r3 = r0;
+      if constexpr (D == 2) {
+        if (m.cvc) {
+          r0 = m.cvc[(size_t)c * 4 + 0];
+          r1 = m.cvc[(size_t)c * 4 + 1];
+          r2 = m.cvc[(size_t)c * 4 + 2];
+          r3 = m.cvc[(size_t)c * 4 + 3];
+        }
+      }
       load_cell<D>(m, Vc, c, cc, false);
       const Prm& pr = m.prm[inst_of(m, c)];
       double xn[D];
-      const int c2 = walk_cached<D>(m, cc, xo, x, u, xn);
+      int c2 = -2;
+      bool have_g = false;
+      double gw[6];
+      if constexpr (D == 2) {
+        if (m.cvc && cc.box) {
+          c2 = walk_box2(m, cc, xo, x, u, r0, r1, r2, r3, xn, gw);
+          have_g = c2 >= 0 && gw[0] == gw[0];
+        }
+      }
+      if (c2 == -2) c2 = walk_cached<D>(m, cc, xo, x, u, xn);
       if (c2 >= 0) {
-        load_cell<D>(m, Vc, c2, cc, true);
+        if (have_g) {  // winner's corners already read: only its field
+          cc.c = c2;
+          cc.box = true;
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            cc.X0[k] = gw[k];
+            cc.X1[k] = gw[D + k];
+            cc.RH[k] = gw[2 * D + k];
+          }
+          if constexpr (D == 2) {
+            const double2* f = reinterpret_cast<const double2*>(Vc + (size_t)c2 * 8);
+#pragma unroll
+            for (int i2 = 0; i2 < 4; ++i2) {
+              const double2 t = f[i2];
+              cc.Vn[i2][0] = t.x;
+              cc.Vn[i2][1] = t.y;
+            }
+          }
+        } else {
+          load_cell<D>(m, Vc, c2, cc, true);
+        }
 #pragma unroll
         for (int k = 0; k < D; ++k) xi[k] = xn[k];
         ++s;
@@ -540,7 +597,7 @@ __device__ __forceinline__ unsigned long long block_scan_u64(unsigned long long 
     if (w < wave) wbase += s;
     tot += s;
   }
-  __syncthreads();
+  // (`part` is written again only after later barriers of the caller)
   *total = tot;
   return wbase + incl - v;
 }
@@ -749,7 +806,10 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
       const int nb = min(s_misc[1], capc);
       load_parts(nb + s_misc[3] - s_misc[2], nb, (long)ch * capc, spill2_base + s_misc[2]);
     }
+    for (int i = threadIdx.x; i < cpw * pmax; i += blockDim.x) s_cnt[i] = 0;
+    for (int i = threadIdx.x; i < cpw; i += blockDim.x) s_ccnt[i] = 0;
   }
+  __syncthreads();
   for (;;) {
     const int ch = s_misc[0];
     if (ch >= nchunks) break;
@@ -766,9 +826,8 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
     const double* s_geo = cbuf_geo(buf);
     const double* s_vc = cbuf_vc(buf);
     const Prm* s_prm = cbuf_prm(buf);
-    for (int i = threadIdx.x; i < ncell * pmax; i += blockDim.x) s_cnt[i] = 0;
-    for (int i = threadIdx.x; i < ncell; i += blockDim.x) s_ccnt[i] = 0;
-    __syncthreads();
+    // (the per-(cell, part) and per-cell counts were cleared by the previous
+    // chunk's last phase)
     // A. reference position, part and key of every particle; counts per
     //    (cell, part)
     int ppl[PRE];
@@ -1077,6 +1136,8 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
     }
     __syncthreads();
     const int o0 = s_misc[8];
+    for (int i = threadIdx.x; i < cpw * pmax; i += blockDim.x) s_cnt[i] = 0;  // for the next chunk
+    for (int i = threadIdx.x; i < cpw; i += blockDim.x) s_ccnt[i] = 0;
     for (int lc = threadIdx.x; lc < ncell; lc += blockDim.x) cs_out[c0 + lc] = o0 + s_cfin[lc];
     if (c1 == nc && threadIdx.x == 0) cs_out[nc] = o0 + nout;
     if (staged) {
@@ -1217,8 +1278,8 @@ __global__ __launch_bounds__(kBlock) void k_node(int ndofs, const int* __restric
 
 // v += interp(dV) (flush of a pending correction outside move)
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_correct(DMesh m, const int* __restrict__ n_ptr, DParts P, DField dV) {
-  constexpr int NV = 1 << D;
+__global__ __launch_bounds__(kBlock) void k_correct(DMesh m, const int* __restrict__ n_ptr, DParts P,
+                                                   const double* __restrict__ dVc) {
   const int n = *n_ptr;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
     const int c = P.cell[p];
@@ -1226,11 +1287,8 @@ __global__ __launch_bounds__(kBlock) void k_correct(DMesh m, const int* __restri
     double x[D], xi[D], corr[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) x[k] = P.x[k][p];
-    int vid[NV], dof[NV];
-    load_ids<D>(m.cv, c, vid);
-    load_ids<D>(m.cd, c, dof);
     recover_xi_c<D>(m, pr, c, x, xi);
-    gather<D>(dV, dof, xi, corr);
+    gather_table<D>(dVc, c, xi, corr);
 #pragma unroll
     for (int k = 0; k < D; ++k) P.v[k][p] = P.v[k][p] + corr[k];
   }

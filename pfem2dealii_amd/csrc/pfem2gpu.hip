@@ -1347,6 +1347,42 @@ int snap_restore(pfem2gpu_handle* h, const Snapshot& s) {
   return PFEM2GPU_OK;
 }
 
+// Replay of a recorded move (the moves of a checkpoint segment recomputed
+// in the reverse sweep): the move kernels run as in the forward (same
+// inputs, same path records) but store each remaining particle at its source
+// index in the staging arrays; k_adj_replay then builds the post-step state
+// from the source map and end cells the forward recorded, so that the
+// redistribution (spill, scan, settle) is not run again. `n_in` is the count
+// at the start of step s (dcnt[s - 1] on the device); with `n_out` = 0 the
+// post-step state is not built (the segment's last move).
+template <int D>
+int replay_move(pfem2gpu_handle* h, const int* dcnt, int s, long n_in, long n_out, const int* src, const int* ecell) {
+  const DMesh m = dmesh(h);
+  const DParts P = dparts(h);
+  DTmp M = dtmp(h);
+  M.src = nullptr;
+  int rounds = 0;
+  for (const Prm& q : h->prm_inst) rounds = std::max(rounds, q.n_sub);
+  const long gc = move_cell_grid(n_in);
+  const long qreg = walk_qreg(n_in);
+  if (qreg * kQStripes > h->qcap) CHK(grow_queues(h, qreg * kQStripes));
+  if (n_in > h->tsize) return fail(PFEM2GPU_ERR_STATE, "staging arrays smaller than the particle count");
+  LAUNCH(h, "k_fill", k_fill_int, grid_for((long)h->qn_len * kQStripes), 0, h->qn, (long)h->qn_len * kQStripes, 0);
+  LAUNCH(h, "k_move", k_move_cell<D>, gc, 0, 1, m, h->Vc, dcnt + s - 1, 0, P, h->dVc, 0, h->cshift, -1, nullptr, 0L,
+         nullptr, M, dwalk(h, 0), h->qn, qreg, h->rec_path);
+  const int gw = kQStripes * std::max(1, h->walk_grid / kQStripes);
+  for (int r = 0; r < rounds; ++r)
+    LAUNCH(h, "k_walk", k_move_walk<D>, gw, 0, m, h->Vc, dwalk(h, r & 1), h->qn + (long)r * kQStripes, qreg,
+           h->cshift, -1, nullptr, 0L, nullptr, M, dwalk(h, (r + 1) & 1), h->qn + (long)(r + 1) * kQStripes,
+           h->rec_path);
+  if (n_out > 0) {
+    LAUNCH(h, "k_adj_replay", k_adj_replay<D>, grid_for(n_out), 0, m, h->Vc, h->pshift, (int)n_out, src, ecell, M, P,
+           h->cell_start + h->nc);
+    h->n_ub = n_out;
+  }
+  return PFEM2GPU_OK;
+}
+
 template <int D>
 int adjoint_impl(pfem2gpu_handle* h, int n_steps, int K, const double* const* Vu, const double* const* Tu,
                  double* const* gv0, double* const* gV, double* loss) {
@@ -1449,9 +1485,8 @@ int adjoint_impl(pfem2gpu_handle* h, int n_steps, int K, const double* const* Vu
     const int s0 = seg * K + 1, s1 = std::min(n_steps, seg * K + K);
     if (seg < nseg - 1) CHK(snap_restore(h, ck[seg]));
     for (int s = s0; s <= s1 && seg < nseg - 1; ++s) {
-      CHK(reserve(h, h->refill_max, 0));
       h->rec_path = paths[s - s0];
-      const int rc = rebin_or_move<D>(h, 1, 0, V);
+      const int rc = replay_move<D>(h, dcnt, s, cnt[s - 1], s < s1 ? cnt[s] : 0, srcs[s], ecell[s]);
       h->rec_path = DPath{nullptr, {nullptr, nullptr, nullptr}, 0};
       CHK(rc);
     }

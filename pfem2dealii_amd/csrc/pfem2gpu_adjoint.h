@@ -65,6 +65,66 @@ __global__ __launch_bounds__(kBlock) void k_loss(int ndofs, DField P, DField T, 
   if (threadIdx.x == 0) atomicAdd(out, 0.5 * part[0]);
 }
 
+// Post-step state of a replayed move from the maps the forward recorded:
+// slot q takes the moved position and the velocity of its source particle
+// (k_move_* in replay mode stored them by source index in M), or, for a
+// refill (src = -1 - part), the particle the redistribution built: the part's
+// sub-cell centre and the advecting field interpolated there (the same
+// expressions as the refill branch of k_settle). Writes the count.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_adj_replay(DMesh m, const double* __restrict__ Vc, int pshift, int n_end,
+                                                      const int* __restrict__ src, const int* __restrict__ end_cell,
+                                                      DTmp M, DParts P, int* __restrict__ count) {
+  constexpr int NV = 1 << D;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *count = n_end;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n_end; q += gridDim.x * blockDim.x) {
+    const int s = src[q], c = end_cell[q];
+    double x[D], v[D];
+    if (s >= 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        x[k] = M.x[k][s];
+        v[k] = M.v[k][s];
+      }
+    } else {
+      const int inst = inst_of(m, c);
+      const Prm& pr = m.prm[inst];
+      const int code = m.ptab[(inst << pshift) + (-1 - s)];
+      double xi[D], N[NV];
+#pragma unroll
+      for (int k = 0; k < D; ++k) xi[k] = ((double)((code >> (8 * k)) & 0xff) + 1.0 / 2) * pr.h[k];
+      const double* g = m.geom + (size_t)c * 3 * D;
+      if (g[0] == g[0]) {
+        CellCache<D> cc;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          cc.X0[k] = g[k];
+          cc.X1[k] = g[D + k];
+        }
+        fwd_box<D>(cc, xi, x);
+      } else {
+        int vid[NV];
+        load_ids<D>(m.cv, c, vid);
+        fwd<D>(m, vid, xi, x);
+      }
+      shape<D>(xi, N);
+      const double* fv = Vc + (size_t)c * NV * D;
+#pragma unroll
+      for (int k = 0; k < D; ++k) v[k] = 0.0;
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int k = 0; k < D; ++k) v[k] = v[k] + N[i] * fv[i * D + k];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      P.x[k][q] = x[k];
+      P.v[k][q] = v[k];
+    }
+    P.cell[q] = c;
+  }
+}
+
 // Box-cell geometry of cell c from the table (the adjoint supports box cells).
 template <int D>
 __device__ __forceinline__ void box_geom(const DMesh& m, int c, double* X0, double* RH) {
@@ -272,8 +332,7 @@ __device__ __forceinline__ void adj_scatter(int key, const double (*add)[3], dou
 // `xt` holds dL/dx after the move on entry and before it on exit. A particle
 // the move deleted has no adjoint (it is no survivor), so lanes with a zero
 // adjoint skip their substeps (a zero adjoint contributes nothing). The
-// loops are uniform across each wavefront (max_sub substeps) for the
-// aggregated scatter.
+// aggregated scatter at the end is reached by every lane of the wavefront.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __restrict__ Vc, int n_start, int max_sub,
                                                     DPath path, double* xt0, double* xt1, double* xt2,
@@ -315,16 +374,34 @@ __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __re
     }
     CellCache<D> cc;
     cc.c = -1;
+    // contributions summed per lane over consecutive substeps in one cell
+    // (acell), flushed directly when the cell changes, and once per wavefront
+    // through the aggregated scatter at the end
+    double acc[NV][3];
+    int acell = -1;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc[i][k] = 0.0;
 #pragma unroll
     for (int s = kAdjMaxSub - 1; s >= 0; --s) {
       if (s >= max_sub) continue;
       const bool act = live && s < n_sub;
-      double add[NV][3];
-      int key = -1;
       if (act) {
         const int c = pc[s];
         if (c != cc.c) load_cell<D>(m, Vc, c, cc, true);  // particles mostly keep their cell
-        key = c;
+        if (c != acell) {
+          if (acell >= 0) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i)
+#pragma unroll
+              for (int k = 0; k < D; ++k) {
+                atomicAdd(&gcv[((size_t)acell * NV + i) * D + k], acc[i][k]);
+                acc[i][k] = 0.0;
+              }
+          }
+          acell = c;
+        }
         double N[NV], dN[NV][3];
         shape_and_grad<D>(pxi[s], N, dN);
         double gx[D];
@@ -335,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __re
           double av = 0.0;
 #pragma unroll
           for (int k = 0; k < D; ++k) {
-            add[i][k] = dt * N[i] * a[k];
+            acc[i][k] += dt * N[i] * a[k];
             av += a[k] * cell_field<D>(Vc, cc, i, k);
           }
 #pragma unroll
@@ -344,8 +421,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __re
 #pragma unroll
         for (int d = 0; d < D; ++d) a[d] += dt * gx[d] * cc.RH[d];
       }
-      adj_scatter<D>(key, add, gcv, s_gcell[wave], s_acc[wave]);
     }
+    adj_scatter<D>(acell, acc, gcv, s_gcell[wave], s_acc[wave]);
     if (p < n_start) {
 #pragma unroll
       for (int k = 0; k < D; ++k) xt[k][p] = a[k];

@@ -222,7 +222,9 @@ __global__ void k_spill_scatter(const int* __restrict__ spill_n, long spill_base
 constexpr int kQStripes = 1024;
 
 // Appends one particle per lane with key >= 0 to its chunk bucket (or the
-// spill region) and queues the lanes with walk set. Wave-uniform.
+// spill region) and queues the lanes with walk set. Wave-uniform. With
+// capc < 0 (replay of a recorded move) the particle is stored at its source
+// index in T.x / T.v instead of a bucket.
 template <int D>
 __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, int src, const double* x,
                                           const double* xo, const double* v, const double* u, int capc,
@@ -250,6 +252,16 @@ __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, i
       Q.s[q] = s;
       Q.src[q] = src;
     }
+  }
+  if (capc < 0) {  // replay: position and velocity stored at the source index
+    if (key >= 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        T.x[k][src] = x[k];
+        T.v[k][src] = v[k];
+      }
+    }
+    return;
   }
   const int slot = wave_group_add(bcount, key);
   if (key >= 0) {

@@ -155,7 +155,7 @@ struct pfem2gpu_handle {
 
 namespace {
 
-constexpr int kSettleLds = 52 * 1024;  // LDS budget of a settle workgroup
+constexpr int kSettleLds = 52 * 1024;  // LDS budget of a settle workgroup (three per CU)
 
 int grid_for(long n) {
   long g = (n + kBlock - 1) / kBlock;
@@ -205,13 +205,14 @@ int t_collect(pfem2gpu_handle* h) {
   return PFEM2GPU_OK;
 }
 
-#define LAUNCH(h, name, kernel, grid, smem, ...)                                  \
+#define LAUNCH_B(h, name, kernel, grid, block, smem, ...)                         \
   do {                                                                            \
     t_begin(h, name);                                                             \
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), smem, h->stream, __VA_ARGS__); \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), smem, h->stream, __VA_ARGS__); \
     t_end(h);                                                                     \
     HIPCHK(hipGetLastError());                                                    \
   } while (0)
+#define LAUNCH(h, name, kernel, grid, smem, ...) LAUNCH_B(h, name, kernel, grid, kBlock, smem, __VA_ARGS__)
 
 template <typename T>
 int dalloc(T** p, size_t n) {
@@ -550,7 +551,7 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
          h->sstart, sfill, T);
   const int gs = std::min(nch, h->settle_grid);
   const int cap_on = advect, refill_on = advect;
-  LAUNCH(h, "k_settle", k_settle<D>, gs, h->settle_smem, cap_on, refill_on, m, h->nc, h->cshift, h->pshift, h->nmax,
+  LAUNCH_B(h, "k_settle", k_settle<D>, gs, kSettleBlock, h->settle_smem, cap_on, refill_on, m, h->nc, h->cshift, h->pshift, h->nmax,
          h->nslots, h->capc, bcount, h->sstart, spill2_base, T, h->count + 3 * nch + 1, h->lb, h->d_err, h->cell_start,
          P, h->Vc, h->gord, h->contrib, h->dirty);
   h->n_ub = nmax_p + (advect ? h->refill_max : 0);
@@ -684,7 +685,7 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
   // (survivors and refills of a chunk) as fit in kSettleLds bytes of LDS
   // (three workgroups per CU), and at most as many incoming particles as the
   // workgroup holds in registers
-  h->cshift = 4;
+  h->cshift = kSettleShift;  // at most 8: the settle keeps chunk-local cell indices in bytes
   h->cpw = 1 << h->cshift;
   h->nchunks = (h->nc + h->cpw - 1) >> h->cshift;
   h->nslots = 64;
@@ -692,7 +693,7 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
     if (settle_layout(dim, h->cpw, h->pmax, 0, ns).bytes > kSettleLds) break;
     h->nslots = ns;
   }
-  h->nmax = std::min(h->nslots, kBlock * (dim == 2 ? SettlePre<2>::n : SettlePre<3>::n));
+  h->nmax = std::min(h->nslots, kSettleBlock * (dim == 2 ? SettlePre<2>::n : SettlePre<3>::n));
   h->settle_smem = settle_layout(dim, h->cpw, h->pmax, h->nmax, h->nslots).bytes;
   if (h->settle_smem > 64 * 1024) {
     delete h;

@@ -473,18 +473,19 @@ __global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __r
 //  D. particles are grouped by (cell, part) and ranked inside their group by
 //     key, which gives every survivor its final slot (cell base + part base
 //     + rank, i.e. (part, key) order); refills are built in their slots;
+//  E1. the (cell, vertex) sums are accumulated sequentially in final order;
 //  C. only then is the chunk's output offset resolved from the totals of
-//     the chunks before it (decoupled look-back), so that the wait overlaps
-//     the chunk's own work;
-//  E. the chunk is written out coalesced in final order, with the new
-//     cell_start, and the (cell, vertex) sums are accumulated sequentially
-//     in the same order.
+//     the chunks before it (decoupled look-back, its first loads issued
+//     before E1), so that the wait overlaps the chunk's own work;
+//  E2. the chunk is written out coalesced in final order, with the new
+//     cell_start.
 // A chunk with more than `nmax` particles resolves its offset in B and takes
 // the remaining steps with its particles in global memory (grouping through
 // the `gord` scratch array, ranking over the group only); it leaves its
 // projection sums to k_contrib_dirty.
 struct SettleLayout {
-  int off_fin;                  // [nslots] final order: x[D], v[D], xi[D] (doubles), src (int); before
+  int off_fin;                  // [nslots] final order: x[D], v[D], xi[D] (doubles), src (int), chunk-local
+                                // cell (byte); before
                                 // the final placement the same bytes hold key (u64), ord (int), xi[D]
                                 // and v[D] (doubles) in arrival order
   int off_cnt, off_base, off_roff;  // [cpw * pmax] per (cell, part)
@@ -499,7 +500,7 @@ __host__ __device__ inline SettleLayout settle_layout(int D, int cpw, int pmax, 
   SettleLayout L;
   int o = 0;
   L.off_fin = o;
-  o += (8 * 3 * D + 4) * nslots;
+  o += (8 * 3 * D + 4 + 1) * nslots;
   o = (o + 15) & ~15;
   L.off_cnt = o;
   o += 4 * cpw * pmax;
@@ -621,15 +622,23 @@ __device__ __forceinline__ unsigned long long block_scan_u64(unsigned long long 
 // predecessors; the prefix is the sum over the lanes up to the nearest chunk
 // with an inclusive prefix (status 2), once none of them is still unpublished.
 // Every lane of the calling wavefront must call it; all get the prefix.
-__device__ __forceinline__ unsigned lb_prefix_wave(unsigned long long* lb, int ch, int* err) {
+__device__ __forceinline__ unsigned long long lb_word(unsigned long long* lb, int k) {
+  return k >= 0 ? __hip_atomic_load(&lb[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (2ULL << 32);
+}
+
+// `first` is this lane's status word of chunk ch-1-lane, loaded by the caller
+// ahead of time (lb_word), so that the first round's latency overlaps the
+// caller's work in between.
+__device__ __forceinline__ unsigned lb_prefix_wave(unsigned long long* lb, int ch, unsigned long long first,
+                                                   int* err) {
   const int lane = threadIdx.x & 63;
   unsigned prefix = 0;
   int base = ch - 1;
   long spins = 0;
+  bool fresh = true;
   while (base >= 0) {
-    const int k = base - lane;
-    const unsigned long long w =
-        k >= 0 ? __hip_atomic_load(&lb[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (2ULL << 32);
+    const unsigned long long w = fresh ? first : lb_word(lb, base - lane);
+    fresh = false;
     const unsigned st = (unsigned)(w >> 32);
     const unsigned long long m2 = __ballot(st == 2);
     const unsigned long long m0 = __ballot(st == 0);
@@ -678,11 +687,18 @@ struct SettlePre {
   static constexpr int n = D == 2 ? 3 : 2;
 };
 
-// Cell data of a chunk (at most 16 cells: box geometry, advecting field,
-// instance and its parameters), carried in registers from its loads to its
-// stores into an LDS cell data buffer, one value per thread and array.
+// Settle workgroup: four wavefronts per chunk of 2^kSettleShift cells.
+constexpr int kSettleBlock = 256;
+constexpr int kSettleShift = 4;
+
+// Cell data of a chunk (box geometry, advecting field, instance and its
+// parameters), carried in registers from its loads to its stores into an LDS
+// cell data buffer, one value per thread and array.
 constexpr int kPrmWords = (int)(sizeof(Prm) / 8);
-static_assert(sizeof(Prm) % 8 == 0 && 16 * kPrmWords <= kBlock, "Prm words per chunk exceed the workgroup");
+static_assert(sizeof(Prm) % 8 == 0 && (1 << kSettleShift) * kPrmWords <= kSettleBlock,
+              "Prm words per chunk exceed the workgroup");
+static_assert((1 << kSettleShift) * 3 * 3 <= kSettleBlock && (1 << kSettleShift) * 8 * 3 <= 2 * kSettleBlock,
+              "cell data per chunk exceeds the workgroup's loads");
 
 template <int D>
 struct CellRegs {
@@ -700,7 +716,7 @@ __device__ __forceinline__ void cells_load(const DMesh& m, const double* __restr
   r.geo = t < ncell * 3 * D ? m.geom[(size_t)c0 * 3 * D + t] : 0.0;
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
-    const int i = t + q * kBlock;
+    const int i = t + q * kSettleBlock;
     r.vc[q] = (do_refill && i < ncell * NV * D) ? Vc[(size_t)c0 * NV * D + i] : 0.0;
   }
   r.prm = t < ncell * kPrmWords ? reinterpret_cast<const double*>(&m.prm[inst_of(m, c0 + t / kPrmWords)])[t % kPrmWords]
@@ -716,7 +732,7 @@ __device__ __forceinline__ void cells_store(const CellRegs<D>& r, int ncell, dou
   if (t < ncell * 3 * D) geo[t] = r.geo;
 #pragma unroll
   for (int q = 0; q < 2; ++q)
-    if (t + q * kBlock < ncell * NV * D) vc[t + q * kBlock] = r.vc[q];
+    if (t + q * kSettleBlock < ncell * NV * D) vc[t + q * kSettleBlock] = r.vc[q];
   if (t < ncell * kPrmWords) reinterpret_cast<double*>(prm)[t] = r.prm;
   if (t < ncell) inst[t] = r.inst;
 }
@@ -736,7 +752,7 @@ __device__ __forceinline__ bool part_code(const Prm& pr, int lin, int* pi) {
 }
 
 template <int D>
-__global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap, int do_refill, DMesh m, int nc, int cshift, int pshift,
+__global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap, int do_refill, DMesh m, int nc, int cshift, int pshift,
                                                   int nmax, int nslots, int capc, const int* __restrict__ bcount,
                                                   const int* __restrict__ sstart, long spill2_base, DTmp T,
                                                   int* __restrict__ ticket, unsigned long long* __restrict__ lb,
@@ -753,6 +769,7 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
   double* s_fv = s_fx + D * nslots;                                // [D][nslots]
   double* s_fxi = s_fv + D * nslots;                               // [D][nslots]
   int* s_fsrc = reinterpret_cast<int*>(s_fxi + D * nslots);        // [nslots]
+  unsigned char* s_fcell = reinterpret_cast<unsigned char*>(s_fsrc + nslots);  // [nslots]
   // before the final placement: key, group order, reference position and
   // velocity of the staged particles, in arrival order (same bytes)
   unsigned long long* s_key = reinterpret_cast<unsigned long long*>(smem_raw + L.off_fin);  // [nmax]
@@ -871,11 +888,17 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
         if (staged_in) {
           s_key[j] = particle_key<D>(px[u]);
 #pragma unroll
-          for (int k = 0; k < D; ++k) {
-            s_sxi[k * nslots + j] = xi[k];
-            s_sv[k * nslots + j] = pv[u][k];
-          }
+          for (int k = 0; k < D; ++k) s_sxi[k * nslots + j] = xi[k];
         }
+      }
+    }
+    if (staged_in) {  // the velocity loads overlap the classification above
+#pragma unroll
+      for (int u = 0; u < PRE; ++u) {
+        const int j = threadIdx.x + u * blockDim.x;
+        if (j < n)
+#pragma unroll
+          for (int k = 0; k < D; ++k) s_sv[k * nslots + j] = pv[u][k];
       }
     }
     for (int j = threadIdx.x + PRE * blockDim.x; j < n; j += blockDim.x) {  // beyond the registers: not staged
@@ -1066,6 +1089,7 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
             s_fv[k * nslots + f] = qv[u][k];
             s_fxi[k * nslots + f] = qxi[u][k];
           }
+          s_fcell[f] = (unsigned char)(ppl[u] >> 16);
           if (P.src) s_fsrc[f] = T.src[chunk_index(threadIdx.x + u * blockDim.x, nb, bucket0, spill0)];
         }
       }
@@ -1115,6 +1139,7 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
             s_fxi[k * nslots + f] = xi[k];
           }
           s_fsrc[f] = -1 - pp;
+          s_fcell[f] = (unsigned char)lc;
         } else {
 #pragma unroll
           for (int k = 0; k < D; ++k) {
@@ -1124,6 +1149,59 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
           P.cell[o0n + f] = c;
           if (P.src) P.src[o0n + f] = -1 - pp;
         }
+      }
+    }
+    // C. output offset of a staged chunk by decoupled look-back: its first
+    //    round of status loads is issued here and waited for only after the
+    //    projection sums, which do not depend on it
+    const bool lb_on = staged && threadIdx.x < 64 && ch > 0;
+    unsigned long long lbw = 0;
+    if (lb_on) lbw = lb_word(lb, ch - 1 - (int)threadIdx.x);
+    __syncthreads();  // final-order arrays complete
+    if (staged) {
+      // E1. per (cell, vertex) sums, sequential in final order: one thread
+      //     per (cell, vertex, component), component D being W
+      for (int t = threadIdx.x; t < ncell * NV * (D + 1); t += blockDim.x) {
+        const int q = t / (D + 1), comp = t - q * (D + 1);
+        const int lc = q / NV, iv = q - lc * NV;
+        const int c = c0 + lc;
+        const int a = s_cfin[lc], b = (lc + 1 < ncell) ? s_cfin[lc + 1] : nout;
+        const double* vv = s_fv + (comp < D ? comp : 0) * nslots;
+        double S = 0.0;
+        int f = a;
+        for (; f + 4 <= b; f += 4) {  // loads of four particles issued together
+          double N[4], w[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+              const double xk = s_fxi[k * nslots + f + e];
+              const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
+              N[e] = (k == 0) ? l : N[e] * l;
+            }
+            w[e] = vv[f + e];
+          }
+#pragma unroll
+          for (int e = 0; e < 4; ++e) S = comp < D ? S + N[e] * w[e] : S + N[e];
+        }
+        for (; f < b; ++f) {
+          double N = 1.0;
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            const double xk = s_fxi[k * nslots + f];
+            const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
+            N = (k == 0) ? l : N * l;
+          }
+          S = comp < D ? S + N * vv[f] : S + N;
+        }
+        contrib[((size_t)c * NV + iv) * (D + 1) + comp] = S;
+      }
+    }
+    if (lb_on) {  // (before the next chunk's loads: a wavefront's loads complete in order)
+      const unsigned prefix = lb_prefix_wave(lb, ch, lbw, err);
+      if (threadIdx.x == 0) {
+        lb_publish(lb, ch, 2u, prefix + (unsigned)nout);
+        s_misc[8] = (int)prefix;
       }
     }
     // the next chunk: particles into the registers, cell data into the other
@@ -1137,15 +1215,6 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
       const int nnb = min(s_misc[5], capc);
       load_parts(nnb + s_misc[7] - s_misc[6], nnb, (long)nch * capc, spill2_base + s_misc[6]);
     }
-    // C. output offset of a staged chunk by decoupled look-back, resolved
-    //    only now so that the wait overlaps the chunk's own work
-    if (staged && threadIdx.x < 64 && ch > 0) {
-      const unsigned prefix = lb_prefix_wave(lb, ch, err);
-      if (threadIdx.x == 0) {
-        lb_publish(lb, ch, 2u, prefix + (unsigned)nout);
-        s_misc[8] = (int)prefix;
-      }
-    }
     __syncthreads();
     const int o0 = s_misc[8];
     for (int i = threadIdx.x; i < cpw * pmax; i += blockDim.x) s_cnt[i] = 0;  // for the next chunk
@@ -1153,44 +1222,15 @@ __global__ __launch_bounds__(kBlock, D == 2 ? 3 : 2) void k_settle(int apply_cap
     for (int lc = threadIdx.x; lc < ncell; lc += blockDim.x) cs_out[c0 + lc] = o0 + s_cfin[lc];
     if (c1 == nc && threadIdx.x == 0) cs_out[nc] = o0 + nout;
     if (staged) {
-      // E. coalesced write-out in final order and the per (cell, vertex)
-      //    sums, sequential in the same order
+      // E2. coalesced write-out in final order
       for (int t = threadIdx.x; t < nout; t += blockDim.x) {
-        int lo = 0, hi = ncell - 1;  // cell of final position t: last lc with s_cfin[lc] <= t
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (s_cfin[mid] <= t)
-            lo = mid;
-          else
-            hi = mid - 1;
-        }
 #pragma unroll
         for (int k = 0; k < D; ++k) {
           P.x[k][o0 + t] = s_fx[k * nslots + t];
           P.v[k][o0 + t] = s_fv[k * nslots + t];
         }
-        P.cell[o0 + t] = c0 + lo;
+        P.cell[o0 + t] = c0 + s_fcell[t];
         if (P.src) P.src[o0 + t] = s_fsrc[t];
-      }
-      // one thread per (cell, vertex, component), component D being W
-      for (int t = threadIdx.x; t < ncell * NV * (D + 1); t += blockDim.x) {
-        const int q = t / (D + 1), comp = t - q * (D + 1);
-        const int lc = q / NV, iv = q - lc * NV;
-        const int c = c0 + lc;
-        const int a = s_cfin[lc], b = (lc + 1 < ncell) ? s_cfin[lc + 1] : nout;
-        const double* vv = s_fv + (comp < D ? comp : 0) * nslots;
-        double S = 0.0;
-        for (int f = a; f < b; ++f) {
-          double N = 1.0;
-#pragma unroll
-          for (int k = 0; k < D; ++k) {
-            const double xk = s_fxi[k * nslots + f];
-            const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
-            N = (k == 0) ? l : N * l;
-          }
-          S = comp < D ? S + N * vv[f] : S + N;
-        }
-        contrib[((size_t)c * NV + iv) * (D + 1) + comp] = S;
       }
     }
     if (nch < nchunks) cells_store<D>(cr, nncell, cbuf_geo(buf ^ 1), cbuf_vc(buf ^ 1), cbuf_prm(buf ^ 1), cbuf_inst(buf ^ 1));

@@ -31,7 +31,8 @@ enum {
   PFEM2GPU_ERR_ARG = 1,     /* invalid argument */
   PFEM2GPU_ERR_HIP = 2,     /* HIP runtime error */
   PFEM2GPU_ERR_STATE = 3,   /* call order violated (e.g. move before initialize_maps) */
-  PFEM2GPU_ERR_MEMORY = 4   /* device allocation failed */
+  PFEM2GPU_ERR_MEMORY = 4,  /* device allocation failed */
+  PFEM2GPU_ERR_IO = 5       /* an output file could not be written */
 };
 
 enum { PFEM2GPU_HOST = 0, PFEM2GPU_DEVICE = 1 };
@@ -127,6 +128,23 @@ int pfem2gpu_download(pfem2gpu_handle* h, double* x, double* xi, double* v, int3
 
 /* Particles per cell ([n_cells], host). Synchronises. */
 int pfem2gpu_cell_counts(pfem2gpu_handle* h, int32_t* counts);
+
+/* Writes the particles (positions and velocities, in the handle's order) as
+ * <directory>/particles-TT.RRR.vtu and, when rank == 0, the index file
+ * <directory>/particles-TT.pvtu listing n_ranks pieces; TT and RRR are
+ * timestep_number and rank zero-padded to two and three digits; `subdomain`
+ * is the value of every particle's subdomain entry. Same text as the
+ * reference. Synchronises.
+ * Replaces output_particle_solution, src/pfem2particlehandler.cpp:217-317 */
+int pfem2gpu_output_particle_solution(pfem2gpu_handle* h, int32_t timestep_number, int32_t rank, int32_t n_ranks,
+                                      int32_t subdomain, const char* directory);
+
+/* The two files of pfem2gpu_output_particle_solution from host arrays
+ * (x, v: [dim][n]); no device use. The piece
+ * (src/pfem2particlehandler.cpp:219-294) and the index file (:296-316). */
+int pfem2gpu_write_particles_vtu(const char* path, int32_t dim, int64_t n, const double* x, const double* v,
+                                 int32_t subdomain);
+int pfem2gpu_write_particles_pvtu(const char* path, int32_t timestep_number, int32_t n_ranks);
 
 /* ---- Batched form (no counterpart in the reference) ----
  * Many independent flow instances in one handle: meshes[i] and params[i]

@@ -58,6 +58,7 @@ EXPORTS = [
     "pfem2gpu_stream", "pfem2gpu_synchronize", "pfem2gpu_set_timing", "pfem2gpu_timings",
     "pfem2gpu_batch_create", "pfem2gpu_batch_layout", "pfem2gpu_step", "pfem2gpu_set_tuning", "pfem2gpu_get_tuning",
     "pfem2gpu_adjoint", "pfem2gpu_move_particles_traced",
+    "pfem2gpu_output_particle_solution", "pfem2gpu_write_particles_vtu", "pfem2gpu_write_particles_pvtu",
 ]
 
 
@@ -101,6 +102,11 @@ def load_library(path=None):
     lib.pfem2gpu_adjoint.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, _vp,
                                      ctypes.POINTER(ctypes.c_double)]
     lib.pfem2gpu_move_particles_traced.argtypes = [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int64, _i64p]
+    lib.pfem2gpu_output_particle_solution.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                      ctypes.c_int32, ctypes.c_char_p]
+    lib.pfem2gpu_write_particles_vtu.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, _vp, _vp,
+                                                 ctypes.c_int32]
+    lib.pfem2gpu_write_particles_pvtu.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32]
     if path is None:
         _lib = lib
     return lib
@@ -110,6 +116,25 @@ def _check(lib, rc, what):
     if rc != 0:
         msg = lib.pfem2gpu_last_error().decode(errors="replace")
         raise Pfem2GpuError("%s failed (code %d): %s" % (what, rc, msg))
+
+
+def write_particles_vtu(path, x, v, subdomain=0):
+    """One rank's particle file from host arrays x, v ([dim][n]); the text of
+    src/pfem2particlehandler.cpp:219-294."""
+    lib = load_library()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    if x.ndim != 2 or v.shape != x.shape:
+        raise ValueError("x and v must be [dim][n] arrays of the same shape")
+    _check(lib, lib.pfem2gpu_write_particles_vtu(os.fsencode(path), x.shape[0], x.shape[1], x.ctypes.data,
+                                                 v.ctypes.data, int(subdomain)), "write_particles_vtu")
+
+
+def write_particles_pvtu(path, timestep_number, n_ranks=1):
+    """The index file of n_ranks pieces (src/pfem2particlehandler.cpp:296-316)."""
+    lib = load_library()
+    _check(lib, lib.pfem2gpu_write_particles_pvtu(os.fsencode(path), int(timestep_number), int(n_ranks)),
+           "write_particles_pvtu")
 
 
 def _is_device(a):
@@ -280,6 +305,13 @@ class ParticleHandler:
         if xi:
             out["xi"] = r
         return out
+
+    def output_particle_solution(self, timestep_number, directory=".", rank=0, n_ranks=1, subdomain=0):
+        """Writes particles-TT.RRR.vtu (and particles-TT.pvtu on rank 0) into
+        `directory`, in the text format of src/pfem2particlehandler.cpp:217-317."""
+        _check(self.lib, self.lib.pfem2gpu_output_particle_solution(
+            self.h, int(timestep_number), int(rank), int(n_ranks), int(subdomain), os.fsencode(directory)),
+            "output_particle_solution")
 
     def cell_counts(self):
         c = np.empty(self.n_cells, dtype=np.int32)

@@ -24,6 +24,7 @@
 
 #include "../../include/pfem2gpu.h"
 #include "pfem2gpu_adjoint.h"
+#include "pfem2gpu_output.h"
 
 using namespace pfem2gpu;
 
@@ -1207,6 +1208,40 @@ int pfem2gpu_download(pfem2gpu_handle* h, double* x, double* xi, double* v, int3
       HIPCHK(hipMemcpyAsync(xi + (size_t)k * n, h->tx[k], n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   }
   HIPCHK(hipStreamSynchronize(h->stream));
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_write_particles_vtu(const char* path, int32_t dim, int64_t n, const double* x, const double* v,
+                                 int32_t subdomain) {
+  if (!path || dim < 1 || dim > 3 || n < 0 || (n > 0 && (!x || !v)))
+    return fail(PFEM2GPU_ERR_ARG, "write_particles_vtu: bad arguments");
+  if (n > 0xffffffffLL) return fail(PFEM2GPU_ERR_ARG, "write_particles_vtu: more than 2^32-1 particles");
+  if (!pfem2gpu_out::write_vtu(path, dim, n, x, v, subdomain))
+    return fail(PFEM2GPU_ERR_IO, std::string("cannot write ") + path);
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_write_particles_pvtu(const char* path, int32_t timestep_number, int32_t n_ranks) {
+  if (!path || n_ranks < 1) return fail(PFEM2GPU_ERR_ARG, "write_particles_pvtu: bad arguments");
+  if (!pfem2gpu_out::write_pvtu(path, timestep_number, n_ranks))
+    return fail(PFEM2GPU_ERR_IO, std::string("cannot write ") + path);
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_output_particle_solution(pfem2gpu_handle* h, int32_t timestep_number, int32_t rank, int32_t n_ranks,
+                                      int32_t subdomain, const char* directory) {
+  if (!h || !directory) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  if (rank < 0 || n_ranks < 1 || rank >= n_ranks) return fail(PFEM2GPU_ERR_ARG, "rank out of range");
+  int64_t n = 0;
+  CHK(pfem2gpu_particle_count(h, &n));
+  std::vector<double> x((size_t)h->dim * n), v((size_t)h->dim * n);
+  CHK(pfem2gpu_download(h, x.data(), nullptr, v.data(), nullptr));
+  std::string dir(directory);
+  if (!dir.empty() && dir.back() != '/') dir += '/';
+  const std::string step = pfem2gpu_out::padded(timestep_number, 2);
+  CHK(pfem2gpu_write_particles_vtu((dir + "particles-" + step + "." + pfem2gpu_out::padded(rank, 3) + ".vtu").c_str(),
+                                   h->dim, n, x.data(), v.data(), subdomain));
+  if (rank == 0) CHK(pfem2gpu_write_particles_pvtu((dir + "particles-" + step + ".pvtu").c_str(), timestep_number, n_ranks));
   return PFEM2GPU_OK;
 }
 

@@ -75,6 +75,14 @@ const char* pfem2gpu_last_error(void);
  * src/pfem2particlehandler.cpp:319-324 */
 int pfem2gpu_initialize_maps(pfem2gpu_handle* h);
 
+/* The vertex -> cells map initialize_maps built: offsets [n_vertices + 1]
+ * (host, required); cells [offsets[n_vertices]] in ascending order per vertex
+ * and dirs [offsets[n_vertices]][dim], the unit directions from the vertex to
+ * each cell's centre (either may be NULL). Synchronises.
+ * Replaces reading vertex_to_cells / vertex_to_cell_centers,
+ * src/pfem2particlehandler.h:54-55 */
+int pfem2gpu_vertex_to_cells(pfem2gpu_handle* h, int32_t* offsets, int32_t* cells, double* dirs);
+
 /* seed_particles(): one particle at the centre of every sub-cell part of
  * every cell, velocity interpolated from V. src/pfem2particlehandler.cpp:42-109 */
 int pfem2gpu_seed_particles(pfem2gpu_handle* h, const double* const* V, int where);

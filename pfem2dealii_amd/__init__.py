@@ -59,6 +59,7 @@ EXPORTS = [
     "pfem2gpu_batch_create", "pfem2gpu_batch_layout", "pfem2gpu_step", "pfem2gpu_set_tuning", "pfem2gpu_get_tuning",
     "pfem2gpu_adjoint", "pfem2gpu_move_particles_traced",
     "pfem2gpu_output_particle_solution", "pfem2gpu_write_particles_vtu", "pfem2gpu_write_particles_pvtu",
+    "pfem2gpu_vertex_to_cells",
 ]
 
 
@@ -107,6 +108,7 @@ def load_library(path=None):
     lib.pfem2gpu_write_particles_vtu.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, _vp, _vp,
                                                  ctypes.c_int32]
     lib.pfem2gpu_write_particles_pvtu.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32]
+    lib.pfem2gpu_vertex_to_cells.argtypes = [_vp, _vp, _vp, _vp]
     if path is None:
         _lib = lib
     return lib
@@ -192,6 +194,7 @@ class ParticleHandler:
         self.dim = int(mesh["dim"])
         self.n_dofs = int(mesh["n_dofs"])
         self.n_cells = int(np.asarray(mesh["cell_vertices"]).shape[0])
+        self.n_vertices = int(np.asarray(mesh["X"]).shape[1])
         keep = []
         m = _pack_mesh(mesh, keep)
         p = _pack_params(params)
@@ -313,6 +316,17 @@ class ParticleHandler:
             self.h, int(timestep_number), int(rank), int(n_ranks), int(subdomain), os.fsencode(directory)),
             "output_particle_solution")
 
+    def vertex_to_cells(self):
+        """(offsets [n_vertices + 1], cells, dirs [n, dim]) of the map built by
+        initialize_maps (src/pfem2particlehandler.cpp:319-324)."""
+        off = np.empty(self.n_vertices + 1, dtype=np.int32)
+        _check(self.lib, self.lib.pfem2gpu_vertex_to_cells(self.h, off.ctypes.data, None, None), "vertex_to_cells")
+        cells = np.empty(int(off[-1]), dtype=np.int32)
+        dirs = np.empty((int(off[-1]), self.dim))
+        _check(self.lib, self.lib.pfem2gpu_vertex_to_cells(self.h, off.ctypes.data, cells.ctypes.data,
+                                                           dirs.ctypes.data), "vertex_to_cells")
+        return off, cells, dirs
+
     def cell_counts(self):
         c = np.empty(self.n_cells, dtype=np.int32)
         _check(self.lib, self.lib.pfem2gpu_cell_counts(self.h, c.ctypes.data), "cell_counts")
@@ -413,3 +427,4 @@ class BatchHandler(ParticleHandler):
         self.dof_offsets = do
         self.n_cells = int(co[-1])
         self.n_dofs = int(do[-1])
+        self.n_vertices = int(sum(np.asarray(m["X"]).shape[1] for m in meshes))

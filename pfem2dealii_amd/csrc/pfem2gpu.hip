@@ -83,8 +83,6 @@ struct pfem2gpu_handle {
   double* Vc = nullptr;    // [nc][nvc][dim] advecting field, cell-major
   int pshift = 0;
   bool maps_ready = false;
-  std::vector<double> hX;  // host copies used by initialize_maps
-  std::vector<int> hcv;
 
   // particles
   long cap = 0;
@@ -703,8 +701,6 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
 
   int rc;
   // mesh upload
-  h->hX.assign(mesh->vertices, mesh->vertices + (size_t)dim * h->nv);
-  h->hcv.assign(mesh->cell_vertices, mesh->cell_vertices + (size_t)h->nc * nvc);
   for (int k = 0; k < dim; ++k) {
     if ((rc = dalloc(&h->X[k], h->nv))) return bail(rc);
     if (hipMemcpy(h->X[k], mesh->vertices + (size_t)k * h->nv, h->nv * sizeof(double), hipMemcpyHostToDevice))
@@ -939,80 +935,88 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
 }
 
 // GridTools::vertex_to_cell_map + vertex_to_cell_centers_directions, built on
-// the host and uploaded. Centre = vertex sum * (1/n_vertices); direction
+// the device (k_v2c_*). Centre = vertex sum * (1/n_vertices); direction
 // normalised by multiplying with 1/norm (deal.II Tensor division).
 int pfem2gpu_initialize_maps(pfem2gpu_handle* h) {
   if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
-  const int dim = h->dim, nvc = h->nvc;
-  std::vector<int> off(h->nv + 1, 0);
-  for (long e = 0; e < (long)h->nc * nvc; ++e) off[h->hcv[e] + 1]++;
-  for (int v = 0; v < h->nv; ++v) off[v + 1] += off[v];
-  std::vector<int> lst(off[h->nv]);
-  {
-    std::vector<int> pos(off.begin(), off.end() - 1);
-    for (int c = 0; c < h->nc; ++c)
-      for (int i = 0; i < nvc; ++i) lst[pos[h->hcv[(size_t)c * nvc + i]]++] = c;
+  const int dim = h->dim, nvc = h->nvc, nv = h->nv;
+  const long ne = (long)h->nc * nvc;
+  const DMesh m = dmesh(h);
+  // scratch: counts, offsets, fill cursors, the unsorted lists, scan blocks
+  int *cnt = nullptr, *off = nullptr, *fill = nullptr, *lst = nullptr, *cnt2 = nullptr, *blk = nullptr;
+  int* off2 = nullptr;
+  const int nb = (nv + kScanChunk - 1) / kScanChunk;
+  auto release = [&]() {
+    dfree(cnt);
+    dfree(off);
+    dfree(fill);
+    dfree(lst);
+    dfree(cnt2);
+    dfree(blk);
+    dfree(off2);
+  };
+  int rc = PFEM2GPU_OK;
+  if ((rc = dalloc(&cnt, nv + 1)) || (rc = dalloc(&off, nv + 1)) || (rc = dalloc(&fill, nv + 1)) ||
+      (rc = dalloc(&lst, ne)) || (rc = dalloc(&cnt2, nv + 1)) || (rc = dalloc(&blk, nb + 1)) ||
+      (rc = dalloc(&off2, nv + 1))) {
+    release();
+    return rc;
   }
-  // cells are visited in ascending order; drop repeats (degenerate cells)
-  std::vector<int> off2(h->nv + 1, 0), lst2;
-  lst2.reserve(lst.size());
-  for (int v = 0; v < h->nv; ++v) {
-    int last = -1;
-    for (int e = off[v]; e < off[v + 1]; ++e)
-      if (lst[e] != last) {
-        lst2.push_back(lst[e]);
-        last = lst[e];
-      }
-    off2[v + 1] = (int)lst2.size();
-  }
-  std::vector<double> dirs(lst2.size() * dim);
-  const double inv_n = 1.0 / (double)nvc;
-  for (int v = 0; v < h->nv; ++v)
-    for (int e = off2[v]; e < off2[v + 1]; ++e) {
-      const int c = lst2[e];
-      double ctr[3] = {0, 0, 0}, d[3] = {0, 0, 0};
-      for (int i = 0; i < nvc; ++i)
-        for (int k = 0; k < dim; ++k) ctr[k] = ctr[k] + h->hX[(size_t)k * h->nv + h->hcv[(size_t)c * nvc + i]];
-      for (int k = 0; k < dim; ++k) d[k] = ctr[k] * inv_n - h->hX[(size_t)k * h->nv + v];
-      double m2 = 0.0;
-      for (int k = 0; k < dim; ++k) m2 = m2 + d[k] * d[k];
-      const double inv_m = 1.0 / std::sqrt(m2);
-      for (int k = 0; k < dim; ++k) dirs[(size_t)e * dim + k] = d[k] * inv_m;
-    }
-  std::vector<int2> cvo((size_t)h->nc * nvc);
-  for (int c = 0; c < h->nc; ++c)
-    for (int i = 0; i < nvc; ++i) {
-      const int v = h->hcv[(size_t)c * nvc + i];
-      cvo[(size_t)c * nvc + i] = make_int2(off2[v], off2[v + 1] - off2[v]);
-    }
-  dfree(h->cvo);
-  CHK(dalloc(&h->cvo, cvo.size()));
-  HIPCHK(hipMemcpy(h->cvo, cvo.data(), cvo.size() * sizeof(int2), hipMemcpyHostToDevice));
-  dfree(h->cvc);
-  if (dim == 2) {  // the cells around each cell vertex, for the 2D walk
-    std::vector<int4> cvc((size_t)h->nc * 4);
-    for (size_t e = 0; e < cvc.size(); ++e) {
-      const int2 r = cvo[e];
-      int ids[4] = {-1, -1, -1, -1};
-      if (r.y > 4)
-        ids[0] = -2;
-      else
-        for (int j = 0; j < r.y; ++j) ids[j] = lst2[r.x + j];
-      cvc[e] = make_int4(ids[0], ids[1], ids[2], ids[3]);
-    }
-    CHK(dalloc(&h->cvc, cvc.size()));
-    HIPCHK(hipMemcpy(h->cvc, cvc.data(), cvc.size() * sizeof(int4), hipMemcpyHostToDevice));
-  }
-  dfree(h->v2c_off);
-  dfree(h->v2c);
-  dfree(h->v2c_dir);
-  CHK(dalloc(&h->v2c_off, off2.size()));
-  CHK(dalloc(&h->v2c, lst2.size()));
-  CHK(dalloc(&h->v2c_dir, dirs.size()));
-  HIPCHK(hipMemcpy(h->v2c_off, off2.data(), off2.size() * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->v2c, lst2.data(), std::max<size_t>(1, lst2.size()) * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->v2c_dir, dirs.data(), std::max<size_t>(1, dirs.size()) * 8, hipMemcpyHostToDevice));
+  auto scan_into = [&](const int* in, int* out) -> int {  // exclusive; out[nv] = total
+    LAUNCH(h, "scan", k_scan_reduce, nb, 0, in, nv, blk);
+    LAUNCH(h, "scan", k_scan_blocks, 1, 0, blk, nb, out + nv);
+    LAUNCH(h, "scan", k_scan_down, nb, 0, in, nv, blk, out);
+    return PFEM2GPU_OK;
+  };
+  auto build = [&]() -> int {
+    LAUNCH(h, "k_fill", k_zero3, grid_for(2L * (nv + 1)), 0, cnt, (long)nv + 1, fill, (long)nv + 1, nullptr, 0L);
+    LAUNCH(h, "k_maps", k_v2c_count, grid_for(ne), 0, h->cv, ne, cnt);
+    CHK(scan_into(cnt, off));
+    LAUNCH(h, "k_maps", k_v2c_fill, grid_for(ne), 0, h->cv, ne, nvc, off, fill, lst);
+    LAUNCH(h, "k_maps", k_v2c_sort, grid_for(nv), 0, nv, off, lst, cnt2);
+    CHK(scan_into(cnt2, off2));
+    int total = 0;
+    HIPCHK(hipMemcpyAsync(&total, off2 + nv, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h->v2c_off);
+    dfree(h->v2c);
+    dfree(h->v2c_dir);
+    dfree(h->cvo);
+    dfree(h->cvc);
+    CHK(dalloc(&h->v2c, total));
+    CHK(dalloc(&h->v2c_dir, (size_t)total * dim));
+    CHK(dalloc(&h->cvo, ne));
+    if (dim == 2) CHK(dalloc(&h->cvc, ne));
+    const double inv_n = 1.0 / (double)nvc;
+    if (dim == 2)
+      LAUNCH(h, "k_maps", k_v2c_emit<2>, grid_for(nv), 0, m, nv, off, lst, off2, h->v2c, h->v2c_dir, inv_n);
+    else
+      LAUNCH(h, "k_maps", k_v2c_emit<3>, grid_for(nv), 0, m, nv, off, lst, off2, h->v2c, h->v2c_dir, inv_n);
+    LAUNCH(h, "k_maps", k_cell_vertex_maps, grid_for(ne), 0, ne, h->cv, off2, h->v2c, h->cvo, h->cvc);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->v2c_off = off2;  // kept: the per-vertex offsets
+    off2 = nullptr;
+    return PFEM2GPU_OK;
+  };
+  rc = build();
+  release();
+  if (rc != PFEM2GPU_OK) return rc;
   h->maps_ready = true;
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_vertex_to_cells(pfem2gpu_handle* h, int32_t* offsets, int32_t* cells, double* dirs) {
+  if (!h || !offsets) return fail(PFEM2GPU_ERR_ARG, "null argument");
+  if (!h->maps_ready) return fail(PFEM2GPU_ERR_STATE, "vertex_to_cells before initialize_maps");
+  HIPCHK(hipMemcpyAsync(offsets, h->v2c_off, (h->nv + 1) * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const int total = offsets[h->nv];
+  if (cells && total > 0)
+    HIPCHK(hipMemcpyAsync(cells, h->v2c, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  if (dirs && total > 0)
+    HIPCHK(hipMemcpyAsync(dirs, h->v2c_dir, (size_t)total * h->dim * sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
   return PFEM2GPU_OK;
 }
 

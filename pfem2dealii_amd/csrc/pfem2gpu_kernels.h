@@ -1246,6 +1246,100 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
   }
 }
 
+// ------------------------------------------------------------------ maps
+// initialize_maps (src/pfem2particlehandler.cpp:319-324): for every vertex
+// the cells that contain it, ascending and each once (a degenerate cell can
+// list a vertex twice), and the unit direction from the vertex to each such
+// cell's centre (vertex average), built on the device: count, scatter,
+// per-vertex sort and de-duplication, then the compacted lists with their
+// directions; plus the per (cell, vertex) views of the lists used by the walk.
+
+__global__ void k_v2c_count(const int* __restrict__ cv, long ne, int* __restrict__ cnt) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[cv[e]], 1);
+}
+
+__global__ void k_v2c_fill(const int* __restrict__ cv, long ne, int nvc, const int* __restrict__ off,
+                           int* __restrict__ fill, int* __restrict__ lst) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+    const int v = cv[e];
+    lst[off[v] + atomicAdd(&fill[v], 1)] = (int)(e / nvc);
+  }
+}
+
+// each vertex's list sorted ascending (insertion sort: lists are short),
+// repeats dropped in place; cnt2[v] = cells kept
+__global__ void k_v2c_sort(int nv, const int* __restrict__ off, int* __restrict__ lst, int* __restrict__ cnt2) {
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
+    const int a = off[v], b = off[v + 1];
+    for (int i = a + 1; i < b; ++i) {
+      const int key = lst[i];
+      int j = i - 1;
+      while (j >= a && lst[j] > key) {
+        lst[j + 1] = lst[j];
+        --j;
+      }
+      lst[j + 1] = key;
+    }
+    int w = a;
+    for (int i = a; i < b; ++i)
+      if (i == a || lst[i] != lst[w - 1]) lst[w++] = lst[i];
+    cnt2[v] = w - a;
+  }
+}
+
+// compacted lists and directions d = c - X_v, c = (sum_i X_i) * (1 / 2^D)
+// (sequential sum over the cell's vertices), normalised as d * (1 / |d|)
+template <int D>
+__global__ void k_v2c_emit(DMesh m, int nv, const int* __restrict__ off, const int* __restrict__ lst,
+                           const int* __restrict__ off2, int* __restrict__ lst2, double* __restrict__ dirs,
+                           double inv_n) {
+  constexpr int NV = 1 << D;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
+    const int a = off[v], o = off2[v], cnt = off2[v + 1] - o;
+    for (int j = 0; j < cnt; ++j) {
+      const int c = lst[a + j];
+      lst2[o + j] = c;
+      double ctr[D], d[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) ctr[k] = 0.0;
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        const int vi = m.cv[(size_t)c * NV + i];
+#pragma unroll
+        for (int k = 0; k < D; ++k) ctr[k] = ctr[k] + m.X[k][vi];
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k) d[k] = ctr[k] * inv_n - m.X[k][v];
+      double m2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) m2 = m2 + d[k] * d[k];
+      const double inv_m = 1.0 / sqrt(m2);
+#pragma unroll
+      for (int k = 0; k < D; ++k) dirs[(size_t)(o + j) * D + k] = d[k] * inv_m;
+    }
+  }
+}
+
+// per (cell, vertex): (offset, count) of the vertex's list; in 2D also the
+// list itself when it has at most four cells (x = -2 otherwise)
+__global__ void k_cell_vertex_maps(long ne, const int* __restrict__ cv, const int* __restrict__ off2,
+                                   const int* __restrict__ lst2, int2* __restrict__ cvo, int4* __restrict__ cvc) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
+    const int v = cv[e];
+    const int o = off2[v], n = off2[v + 1] - o;
+    cvo[e] = make_int2(o, n);
+    if (cvc) {
+      int ids[4] = {-1, -1, -1, -1};
+      if (n > 4)
+        ids[0] = -2;
+      else
+        for (int j = 0; j < n; ++j) ids[j] = lst2[o + j];
+      cvc[e] = make_int4(ids[0], ids[1], ids[2], ids[3]);
+    }
+  }
+}
+
 // ------------------------------------------------------------- projection
 // Per (cell, vertex) sums S_k = sum_p N_i(xi_p) v_p,k and W = sum_p N_i(xi_p),
 // sequential over the cell's particles in stored order.

@@ -152,24 +152,30 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const int* __restrict__ in
 // -------------------------------------------------------------- move passes
 // Adds, for every distinct key >= 0 held by the lanes of the wavefront, the
 // number of lanes holding it to arr[key] (one atomic per key) and returns each
-// lane's old value plus its rank among the lanes with the same key. Must be
-// called by all 64 lanes.
+// lane's old value plus its rank among the lanes with the same key. The keys
+// are grouped first (ballots only); the group leaders then issue their
+// atomics together, so the wavefront waits for one round trip however many
+// keys it holds. Must be called by all 64 lanes.
 __device__ __forceinline__ int wave_group_add(int* __restrict__ arr, int key) {
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (1ULL << lane) - 1ULL;
-  int result = 0;
+  int leader_of = lane, rank = 0, cnt = 0;
   unsigned long long pending = __ballot(key >= 0);
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
     const int lk = __shfl(key, leader, 64);
     const unsigned long long grp = __ballot(key == lk) & pending;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&arr[lk], __popcll(grp));
-    base = __shfl(base, leader, 64);
-    if ((grp >> lane) & 1ULL) result = base + __popcll(grp & below);
+    if ((grp >> lane) & 1ULL) {
+      leader_of = leader;
+      rank = __popcll(grp & below);
+    }
+    if (lane == leader) cnt = __popcll(grp);
     pending &= ~grp;
   }
-  return result;
+  int base = 0;
+  if (key >= 0 && leader_of == lane) base = atomicAdd(&arr[key], cnt);
+  base = __shfl(base, leader_of, 64);
+  return base + rank;
 }
 
 // Spill binning: particles that did not fit their chunk's bucket are counted

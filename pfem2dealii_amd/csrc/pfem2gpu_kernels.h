@@ -239,12 +239,16 @@ __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, i
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (1ULL << lane) - 1ULL;
   const unsigned long long wm = __ballot(walk);
+  const int stripe = blockIdx.x % kQStripes;
+  const int qleader = wm ? __ffsll((unsigned long long)wm) - 1 : 0;
+  // the queue reservation and the bucket reservations are issued before
+  // either is waited for (one round trip for both)
+  int qb = 0;
+  if (wm && lane == qleader) qb = atomicAdd(&qn[stripe], __popcll(wm));
+  int slot = 0;
+  if (capc >= 0) slot = wave_group_add(bcount, key);
   if (wm) {
-    const int stripe = blockIdx.x % kQStripes;
-    const int leader = __ffsll((unsigned long long)wm) - 1;
-    int qb = 0;
-    if (lane == leader) qb = atomicAdd(&qn[stripe], __popcll(wm));
-    qb = __shfl(qb, leader, 64);
+    qb = __shfl(qb, qleader, 64);
     if (walk) {
       const long q = (long)stripe * qreg + qb + __popcll(wm & below);
 #pragma unroll
@@ -269,7 +273,6 @@ __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, i
     }
     return;
   }
-  const int slot = wave_group_add(bcount, key);
   if (key >= 0) {
     long dst = (long)key * capc + slot;
     if (slot >= capc) dst = spill_base + atomicAdd(spill_n, 1);

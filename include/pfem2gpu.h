@@ -175,11 +175,13 @@ int pfem2gpu_batch_layout(pfem2gpu_handle* h, int64_t* cell_offsets, int64_t* do
  * pointers (a first step with no pending correction runs eagerly). */
 int pfem2gpu_step(pfem2gpu_handle* h, const double* const* V, double* const* V_proj, int use_graph);
 
-/* Launch shape of the walk rounds and the settle kernel (chosen by the
- * autotuner in pfem2dealii_amd/autotune.py). Neither changes results. */
+/* Launch shape of the move passes and the settle kernel (chosen by the
+ * autotuner in pfem2dealii_amd/autotune.py). None changes results. */
 typedef struct {
   int32_t walk_grid;    /* workgroups of a walk round: multiple of 1024 in [1024, 65536] */
   int32_t settle_grid;  /* workgroups of the settle kernel: [1, 65536] */
+  int32_t move_grid;    /* workgroups of the in-cell move pass: 0 = one particle per thread,
+                           else [256, 1048576] workgroups that loop over the particles */
 } pfem2gpu_tuning;
 int pfem2gpu_set_tuning(pfem2gpu_handle* h, const pfem2gpu_tuning* t);
 int pfem2gpu_get_tuning(pfem2gpu_handle* h, pfem2gpu_tuning* t);

@@ -5,6 +5,7 @@ handle (the measured steps advance the simulation; the launch shape does not
 change results, only timing):
   walk_grid   workgroups of a walk round of the move
   settle_grid workgroups of the redistribution/projection kernel
+  move_grid   workgroups of the in-cell move pass (0: one particle per thread)
 Each candidate is timed over `steps` steps with HIP events on the handle's
 stream after one untimed step. Results are cached per workload signature;
 `tuning_mi355x.json` beside this file holds the shapes found on MI355X for the
@@ -17,6 +18,7 @@ TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_mi355x.
 
 WALK_GRIDS = (1024, 2048, 4096, 8192, 16384)
 SETTLE_GRIDS = (512, 1024, 2048, 4096, 8192)
+MOVE_GRIDS = (0, 4096, 8192, 16384, 32768)
 
 _cache = {}
 
@@ -69,9 +71,11 @@ def tune(handle, step, signature=None, steps=2, cache_path=None, log=None, force
     handle.set_tuning(**best)
     best_ms = _time_steps(handle, step, steps)
     trials = [dict(best, walk_grid=g) for g in WALK_GRIDS]
-    for stage in ("walk", "settle"):
+    for stage in ("walk", "settle", "move"):
         if stage == "settle":
             trials = [dict(best, settle_grid=g) for g in SETTLE_GRIDS]
+        elif stage == "move":
+            trials = [dict(best, move_grid=g) for g in MOVE_GRIDS]
         for t in trials:
             if t == best:
                 continue

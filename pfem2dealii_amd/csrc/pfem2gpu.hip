@@ -137,6 +137,7 @@ struct pfem2gpu_handle {
   // tuning (set by pfem2gpu_set_tuning; chosen by the autotuner)
   int walk_grid = 4096;     // workgroups of a walk round (multiple of kQStripes)
   int settle_grid = 2048;   // workgroups of the settle kernel
+  int move_grid = 0;        // workgroups of the in-cell move pass (0: one particle per thread)
   // graph-captured harness step
   bool capturing = false;
   hipGraph_t graph = nullptr;
@@ -281,13 +282,17 @@ void drop_graph(pfem2gpu_handle* h) {
   h->graph = nullptr;
 }
 
-// Workgroups of the in-cell move pass over n particles, and the walk-queue
-// stripe size for it: the particles of the ceil(grid / kQStripes)
-// workgroups that write to one stripe.
-long move_cell_grid(long n) { return std::max(1L, std::min((n + kBlock - 1) / kBlock, 1L << 20)); }
+// Workgroups of the in-cell move pass over n particles (the tuned grid, or
+// one particle per thread), and the walk-queue stripe size for it: the
+// particles of the ceil(grid / kQStripes) workgroups that write to one stripe.
+long move_cell_grid(const pfem2gpu_handle* h, long n) {
+  long g = std::max(1L, std::min((n + kBlock - 1) / kBlock, 1L << 20));
+  if (h->move_grid > 0) g = std::min(g, (long)h->move_grid);
+  return g;
+}
 
-long walk_qreg(long n) {
-  const long gc = move_cell_grid(n);
+long walk_qreg(const pfem2gpu_handle* h, long n) {
+  const long gc = move_cell_grid(h, n);
   const long per_block = (long)kBlock * ((n + gc * kBlock - 1) / (gc * kBlock));
   return ((gc + kQStripes - 1) / kQStripes) * per_block;
 }
@@ -399,7 +404,7 @@ int grow(pfem2gpu_handle* h, long need, long keep, int capc) {
     h->pcell = ncell;
     h->cap = ncap;
   }
-  CHK(grow_queues(h, walk_qreg(h->cap) * kQStripes));
+  CHK(grow_queues(h, walk_qreg(h, h->cap) * kQStripes));
   if (tsize != h->tsize) {
     HIPCHK(hipStreamSynchronize(h->stream));
     for (int k = 0; k < 3; ++k) {
@@ -530,8 +535,8 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
   int rounds = 0;
   if (advect)
     for (const Prm& q : h->prm_inst) rounds = std::max(rounds, q.n_sub);
-  const long gc = move_cell_grid(nmax_p);
-  const long qreg = walk_qreg(nmax_p);
+  const long gc = move_cell_grid(h, nmax_p);
+  const long qreg = walk_qreg(h, nmax_p);
   if (qreg * kQStripes > h->qcap) {
     if (h->capturing) return fail(PFEM2GPU_ERR_STATE, "walk queues too small during graph capture");
     CHK(grow_queues(h, qreg * kQStripes));
@@ -1403,8 +1408,8 @@ int replay_move(pfem2gpu_handle* h, const int* dcnt, int s, long n_in, long n_ou
   M.src = nullptr;
   int rounds = 0;
   for (const Prm& q : h->prm_inst) rounds = std::max(rounds, q.n_sub);
-  const long gc = move_cell_grid(n_in);
-  const long qreg = walk_qreg(n_in);
+  const long gc = move_cell_grid(h, n_in);
+  const long qreg = walk_qreg(h, n_in);
   if (qreg * kQStripes > h->qcap) CHK(grow_queues(h, qreg * kQStripes));
   if (n_in > h->tsize) return fail(PFEM2GPU_ERR_STATE, "staging arrays smaller than the particle count");
   LAUNCH(h, "k_fill", k_fill_int, grid_for((long)h->qn_len * kQStripes), 0, h->qn, (long)h->qn_len * kQStripes, 0);
@@ -1598,8 +1603,11 @@ int pfem2gpu_set_tuning(pfem2gpu_handle* h, const pfem2gpu_tuning* t) {
   if (t->walk_grid < kQStripes || t->walk_grid > 64 * kQStripes || (t->walk_grid % kQStripes) != 0)
     return fail(PFEM2GPU_ERR_ARG, "walk_grid must be a multiple of 1024 in [1024, 65536]");
   if (t->settle_grid < 1 || t->settle_grid > 65536) return fail(PFEM2GPU_ERR_ARG, "settle_grid must be in [1, 65536]");
+  if (t->move_grid != 0 && (t->move_grid < 256 || t->move_grid > (1 << 20)))
+    return fail(PFEM2GPU_ERR_ARG, "move_grid must be 0 or in [256, 1048576]");
   h->walk_grid = t->walk_grid;
   h->settle_grid = t->settle_grid;
+  h->move_grid = t->move_grid;
   drop_graph(h);
   return PFEM2GPU_OK;
 }
@@ -1608,6 +1616,7 @@ int pfem2gpu_get_tuning(pfem2gpu_handle* h, pfem2gpu_tuning* t) {
   if (!h || !t) return fail(PFEM2GPU_ERR_ARG, "null argument");
   t->walk_grid = h->walk_grid;
   t->settle_grid = h->settle_grid;
+  t->move_grid = h->move_grid;
   return PFEM2GPU_OK;
 }
 

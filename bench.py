@@ -295,6 +295,9 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="tune launch shapes instead of the shipped table")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration) and exit")
+    ap.add_argument("--section", choices=("batched", "adjoint"), default=None,
+                    help="only run the batched comparison or the adjoint cost measurement (for profiling) and "
+                         "print its result")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU design is documented, not built); "
@@ -302,6 +305,14 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.section == "batched":
+        print(json.dumps({"section": "batched",
+                          "batched": batched_comparison(max(1, args.batched_steps), 1, force_tune=args.autotune)}),
+              flush=True)
+        return
+    if args.section == "adjoint":
+        print(json.dumps({"section": "adjoint", "adjoint": adjoint_cost(max(1, args.adjoint_reps))}), flush=True)
         return
 
     import torch

@@ -694,11 +694,11 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
   h->nchunks = (h->nc + h->cpw - 1) >> h->cshift;
   h->nslots = 64;
   for (int ns = 64; ns <= 8192; ns += 64) {
-    if (settle_layout(dim, h->cpw, h->pmax, 0, ns).bytes > kSettleLds) break;
+    if (settle_layout(dim, h->cpw, h->pmax, ns).bytes > kSettleLds) break;
     h->nslots = ns;
   }
   h->nmax = std::min(h->nslots, kSettleBlock * (dim == 2 ? SettlePre<2>::n : SettlePre<3>::n));
-  h->settle_smem = settle_layout(dim, h->cpw, h->pmax, h->nmax, h->nslots).bytes;
+  h->settle_smem = settle_layout(dim, h->cpw, h->pmax, h->nslots).bytes;
   if (h->settle_smem > 64 * 1024) {
     delete h;
     return fail(PFEM2GPU_ERR_ARG, "particles_per_cell too large for the redistribution stage");

@@ -504,8 +504,7 @@ struct SettleLayout {
   int bytes;
 };
 
-__host__ __device__ inline SettleLayout settle_layout(int D, int cpw, int pmax, int nmax, int nslots) {
-  (void)nmax;
+__host__ __device__ inline SettleLayout settle_layout(int D, int cpw, int pmax, int nslots) {
   SettleLayout L;
   int o = 0;
   L.off_fin = o;
@@ -624,17 +623,17 @@ __device__ __forceinline__ unsigned long long block_scan_u64(unsigned long long 
   return wbase + incl - v;
 }
 
-// Reference position of x in chunk cell lc from the LDS geometry (box) or
-// the generic path, snapped as in recover_xi_c.
+// Status word of chunk k (chunks before the first count as an inclusive
+// prefix of 0).
+__device__ __forceinline__ unsigned long long lb_word(unsigned long long* lb, int k) {
+  return k >= 0 ? __hip_atomic_load(&lb[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (2ULL << 32);
+}
+
 // The same exclusive prefix looked up by one whole wavefront: lane i reads
 // the status word of chunk ch-1-i, so one round of loads covers 64
 // predecessors; the prefix is the sum over the lanes up to the nearest chunk
 // with an inclusive prefix (status 2), once none of them is still unpublished.
 // Every lane of the calling wavefront must call it; all get the prefix.
-__device__ __forceinline__ unsigned long long lb_word(unsigned long long* lb, int k) {
-  return k >= 0 ? __hip_atomic_load(&lb[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (2ULL << 32);
-}
-
 // `first` is this lane's status word of chunk ch-1-lane, loaded by the caller
 // ahead of time (lb_word), so that the first round's latency overlaps the
 // caller's work in between.
@@ -671,6 +670,8 @@ __device__ __forceinline__ unsigned lb_prefix_wave(unsigned long long* lb, int c
   return prefix;
 }
 
+// Reference position of x in chunk cell c from the LDS geometry (box) or
+// the generic path, snapped as in recover_xi_c.
 template <int D>
 __device__ __forceinline__ void settle_xi(const DMesh& m, const Prm& pr, int c, const double* __restrict__ geo,
                                           const double* x, double* xi) {
@@ -686,8 +687,6 @@ __device__ __forceinline__ void settle_xi(const DMesh& m, const Prm& pr, int c, 
   inv_cached<D>(m, cc, x, xi);
   snap_cached<D>(m, pr, cc, x, xi);
 }
-
-constexpr int kSettlePre = 3;  // particles per thread loaded with the chunk's cell data
 
 // Particles per thread a staged chunk holds in registers between the
 // classification and the final placement.
@@ -773,7 +772,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int cpw = 1 << cshift;
   const int pmax = 1 << pshift;  // part buckets per cell, padded to a power of two
-  const SettleLayout L = settle_layout(D, cpw, pmax, nmax, nslots);
+  const SettleLayout L = settle_layout(D, cpw, pmax, nslots);
   double* s_fx = reinterpret_cast<double*>(smem_raw + L.off_fin);  // [D][nslots]
   double* s_fv = s_fx + D * nslots;                                // [D][nslots]
   double* s_fxi = s_fv + D * nslots;                               // [D][nslots]

@@ -1156,7 +1156,7 @@ int pfem2gpu_project_particle_fields(pfem2gpu_handle* h, double* const* V_out, i
   const int nchunks = h->nchunks;
   if (h->dim == 2) {
     if (h->contrib_valid)
-      LAUNCH(h, "k_contrib_dirty", k_contrib_dirty<2>, std::min(nchunks, 16384), 0, dmesh(h), h->nc, h->cpw,
+      LAUNCH(h, "k_contrib_dirty", k_contrib_dirty<2>, std::min((nchunks + kBlock - 1) / kBlock, 4096), 0, dmesh(h), h->nc, h->cpw,
              h->dirty, h->cell_start, dparts(h), h->contrib);
     else
       LAUNCH(h, "k_contrib", k_contrib<2>, grid_for(ne), 0, dmesh(h), h->nc, h->cell_start, dparts(h), h->contrib);
@@ -1164,7 +1164,7 @@ int pfem2gpu_project_particle_fields(pfem2gpu_handle* h, double* const* V_out, i
            out[2]);
   } else {
     if (h->contrib_valid)
-      LAUNCH(h, "k_contrib_dirty", k_contrib_dirty<3>, std::min(nchunks, 16384), 0, dmesh(h), h->nc, h->cpw,
+      LAUNCH(h, "k_contrib_dirty", k_contrib_dirty<3>, std::min((nchunks + kBlock - 1) / kBlock, 4096), 0, dmesh(h), h->nc, h->cpw,
              h->dirty, h->cell_start, dparts(h), h->contrib);
     else
       LAUNCH(h, "k_contrib", k_contrib<3>, grid_for(ne), 0, dmesh(h), h->nc, h->cell_start, dparts(h), h->contrib);
@@ -1499,7 +1499,7 @@ int adjoint_impl(pfem2gpu_handle* h, int n_steps, int K, const double* const* Vu
   CHK(mem.get(&W, nd));
   CHK(mem.get(&dloss, 1));
   const DMesh m = dmesh(h);
-  LAUNCH(h, "k_contrib_dirty", k_contrib_dirty<D>, std::min(h->nchunks, 16384), 0, m, nc, h->cpw, h->dirty,
+  LAUNCH(h, "k_contrib_dirty", k_contrib_dirty<D>, std::min((h->nchunks + kBlock - 1) / kBlock, 4096), 0, m, nc, h->cpw, h->dirty,
          h->cell_start, dparts(h), h->contrib);
   LAUNCH(h, "k_node_w", k_node_w<D>, grid_for(nd), 0, nd, h->d2c_off, h->d2c, h->contrib, Pn[0], Pn[1], Pn[2], W);
   LAUNCH(h, "k_fill", k_fill_double, 1, 0, dloss, 1L);

@@ -1392,17 +1392,29 @@ __global__ __launch_bounds__(kBlock) void k_contrib(DMesh m, int nc, const int* 
     contrib_one<D>(m, (int)(t / NV), (int)(t % NV), cs, P, contrib);
 }
 
-// The same sums for the cells of chunks the settle kernel could not stage.
+// The same sums for the cells of chunks the settle kernel could not stage
+// (rare): a workgroup reads the flags of blockDim.x chunks in one coalesced
+// pass, lists its flagged chunks in LDS and sums each with all its threads.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_contrib_dirty(DMesh m, int nc, int cpw, const unsigned char* __restrict__ dirty,
                                                          const int* __restrict__ cs, DParts P,
                                                          double* __restrict__ contrib) {
   constexpr int NV = 1 << D;
+  __shared__ int s_list[kBlock];
+  __shared__ int s_n;
   const int nchunks = (nc + cpw - 1) / cpw;
-  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-    if (!dirty[ch]) continue;
-    const int c0 = ch * cpw, ncell = min(nc, c0 + cpw) - c0;
-    for (int t = threadIdx.x; t < ncell * NV; t += blockDim.x) contrib_one<D>(m, c0 + t / NV, t % NV, cs, P, contrib);
+  for (int base = blockIdx.x * blockDim.x; base < nchunks; base += gridDim.x * blockDim.x) {
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const int ch = base + threadIdx.x;
+    if (ch < nchunks && dirty[ch]) s_list[atomicAdd(&s_n, 1)] = ch;
+    __syncthreads();
+    const int nd = s_n;
+    for (int l = 0; l < nd; ++l) {
+      const int c0 = s_list[l] * cpw, ncell = min(nc, c0 + cpw) - c0;
+      for (int t = threadIdx.x; t < ncell * NV; t += blockDim.x) contrib_one<D>(m, c0 + t / NV, t % NV, cs, P, contrib);
+    }
+    __syncthreads();
   }
 }
 

@@ -77,6 +77,57 @@ def test_perturbed_3d_newton(gpu_available):
     _run_pair(cfg, 2, mesh=mesh, x=x, cells=cells)
 
 
+def _star_mesh(n):
+    """Five non-affine quads around a centre vertex (centre, spoke k, spoke k+1,
+    outer corner), each split into n x n sub-quads: the centre vertex touches
+    five cells, so the 2D walk takes its general path there."""
+    K = 5
+    th = [2.0 * np.pi * k / K for k in range(K)]
+    C = np.zeros(2)
+    S = [np.array([np.cos(t), np.sin(t)]) for t in th]
+    M = [1.25 * np.array([np.cos(t + np.pi / K), np.sin(t + np.pi / K)]) for t in th]
+    verts, index, cells = [], {}, []
+
+    def vid(p):
+        key = (round(p[0], 9), round(p[1], 9))
+        if key not in index:
+            index[key] = len(verts)
+            verts.append(p)
+        return index[key]
+
+    for k in range(K):
+        corners = [C, S[k], S[(k + 1) % K], M[k]]
+        lat = {}
+        for a in range(n + 1):
+            for b in range(n + 1):
+                u, w = a / n, b / n
+                N = [(1 - u) * (1 - w), u * (1 - w), (1 - u) * w, u * w]
+                p = sum(Ni * ci for Ni, ci in zip(N, corners))
+                lat[a, b] = vid(p)
+        for b in range(n):
+            for a in range(n):
+                cells.append([lat[a, b], lat[a + 1, b], lat[a, b + 1], lat[a + 1, b + 1]])
+    X = np.array(verts).T.copy()
+    cv = np.array(cells, dtype=np.int32)
+    return {"dim": 2, "n": [n, n], "X": X, "cell_vertices": cv, "cell_dofs": cv, "n_dofs": X.shape[1]}
+
+
+def test_vertex_of_five_cells(gpu_available):
+    _say("star mesh: five non-affine quads around a vertex, 8x8 each, rotation, 3 steps")
+    mesh = _star_mesh(8)
+    X = mesh["X"]
+    V = np.stack([-X[1], X[0]])
+    cfg = {"mesh": mesh, "V": V,
+           "params": dict(n_sub=5, dt=0.05, max_per_part=5, particles_per_cell=[3, 3])}
+    centre = int(np.argmin(np.hypot(X[0], X[1])))
+    assert (mesh["cell_vertices"] == centre).sum() == 5
+    x, cells = seed_in_cells(mesh, 9, seed=21)
+    o, g = _run_pair(cfg, 3, mesh=mesh, x=x, cells=cells)
+    # particles crossed into and out of the five cells around the centre
+    around = np.nonzero((mesh["cell_vertices"] == centre).any(axis=1))[0]
+    assert np.isin(g.download()["cell"], around).sum() > 0
+
+
 def test_seed_lattice(gpu_available):
     _say("seed_particles lattice 2D and 3D")
     for name, scale in (("channel2d", 0.5), ("abc3d", 1.0 / 16)):

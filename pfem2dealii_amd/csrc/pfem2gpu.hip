@@ -107,7 +107,9 @@ struct pfem2gpu_handle {
   bool pending_corr = false;
   double* stageF[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   unsigned long long* d_rejected = nullptr;
-  int* h_pinned = nullptr;
+  int* h_pinned = nullptr;  // [0..1] synchronous reads, [2..3] adjoint loss, [4..5] count and error after a settle
+  hipEvent_t settle_ev = nullptr;  // recorded after [4..5] are copied, following each eager settle
+  bool settle_ev_valid = false;    // [4] is the current count once settle_ev completes
   long n_ub = 0;  // host-side upper bound of the particle count
   int cpw = 16, pmax = 1, nmax = 512, nslots = 0, settle_smem = 0;  // cpw = 1 << cshift
   unsigned char* dirty = nullptr;  // per settle chunk: sums left to k_contrib_dirty
@@ -457,11 +459,22 @@ int read_count(pfem2gpu_handle* h, long* n) {
 
 // Makes room for `extra` particles beyond the current count (particle
 // arrays) and for re-binning the current plus `rebin_extra` particles
-// (staging buckets). Reads the count back only when the host-side bound says
-// the capacity might not suffice.
+// (staging buckets). When the host-side bound says the capacity might not
+// suffice, the count after the last eager settle (copied to pinned memory
+// behind an event) replaces the bound: waiting for that event leaves the work
+// queued after the settle running. Only if that count does not fit either is
+// the stream drained and the capacity grown.
 int reserve(pfem2gpu_handle* h, long extra, long rebin_extra) {
   if (h->capturing) return PFEM2GPU_OK;  // capacity was secured before the capture began
-  if (h->n_ub + extra <= h->cap && capc_for(h, h->n_ub + rebin_extra) <= h->capc) return PFEM2GPU_OK;
+  auto fits = [&](long nb) { return nb + extra <= h->cap && capc_for(h, nb + rebin_extra) <= h->capc; };
+  if (fits(h->n_ub)) return PFEM2GPU_OK;
+  if (h->settle_ev_valid) {
+    HIPCHK(hipEventSynchronize(h->settle_ev));
+    if (h->h_pinned[5] != 0)
+      return fail(PFEM2GPU_ERR_HIP, "redistribution look-back wait timed out; particle layout is invalid");
+    h->n_ub = h->h_pinned[4];
+    if (fits(h->n_ub)) return PFEM2GPU_OK;
+  }
   long n = 0;
   CHK(read_count(h, &n));
   const int capc = capc_for(h, n + rebin_extra);
@@ -560,6 +573,13 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
          P, h->Vc, h->gord, h->contrib, h->dirty);
   h->n_ub = nmax_p + (advect ? h->refill_max : 0);
   h->contrib_valid = true;
+  h->settle_ev_valid = false;
+  if (!h->capturing) {  // the count after this settle, for reserve()
+    HIPCHK(hipMemcpyAsync(h->h_pinned + 4, h->cell_start + h->nc, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(h->h_pinned + 5, h->d_err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipEventRecord(h->settle_ev, h->stream));
+    h->settle_ev_valid = true;
+  }
   return PFEM2GPU_OK;
 }
 
@@ -815,8 +835,10 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
     if ((rc = dalloc(&h->stageF[0][k], h->ndofs)) ||
         (rc = dalloc(&h->stageF[1][k], h->ndofs)))
       return bail(rc);
-  if (hipHostMalloc(reinterpret_cast<void**>(&h->h_pinned), sizeof(int) * 4, 0) != hipSuccess)
+  if (hipHostMalloc(reinterpret_cast<void**>(&h->h_pinned), sizeof(int) * 8, 0) != hipSuccess)
     return bail(fail(PFEM2GPU_ERR_MEMORY, "hipHostMalloc"));
+  if (hipEventCreateWithFlags(&h->settle_ev, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(PFEM2GPU_ERR_HIP, "hipEventCreate"));
   if (hipMemsetAsync(h->d_err, 0, sizeof(int), h->stream) != hipSuccess ||
       hipMemsetAsync(h->cell_start, 0, (h->nc + 1) * sizeof(int), h->stream) != hipSuccess ||
       hipStreamSynchronize(h->stream) != hipSuccess)
@@ -927,6 +949,7 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
   dfree(h->d_rejected);
   dfree(h->dirty);
   if (h->h_pinned) (void)hipHostFree(h->h_pinned);
+  if (h->settle_ev) (void)hipEventDestroy(h->settle_ev);
   for (auto& t : h->pending) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -1094,6 +1117,7 @@ int pfem2gpu_clear_particles(pfem2gpu_handle* h) {
   h->pending_corr = false;
   h->contrib_valid = false;
   h->n_ub = 0;
+  h->settle_ev_valid = false;
   return PFEM2GPU_OK;
 }
 
@@ -1310,6 +1334,7 @@ int pfem2gpu_step(pfem2gpu_handle* h, const double* const* V, double* const* V_p
     }
   }
   HIPCHK(hipGraphLaunch(h->graph_exec, h->stream));
+  h->settle_ev_valid = false;  // the graph's settle records no count
   h->n_ub += h->refill_max;
   h->pending_corr = true;
   h->contrib_valid = false;
@@ -1387,6 +1412,7 @@ int snap_restore(pfem2gpu_handle* h, const Snapshot& s) {
   HIPCHK(hipMemcpyAsync(h->pcell, s.cell, s.n * 4, hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(hipMemcpyAsync(h->cell_start, s.cs, (h->nc + 1) * 4, hipMemcpyDeviceToDevice, h->stream));
   h->n_ub = s.n;
+  h->settle_ev_valid = false;
   h->pending_corr = false;
   h->contrib_valid = false;
   return PFEM2GPU_OK;
@@ -1424,6 +1450,7 @@ int replay_move(pfem2gpu_handle* h, const int* dcnt, int s, long n_in, long n_ou
     LAUNCH(h, "k_adj_replay", k_adj_replay<D>, grid_for(n_out), 0, m, h->Vc, h->pshift, (int)n_out, src, ecell, M, P,
            h->cell_start + h->nc);
     h->n_ub = n_out;
+    h->settle_ev_valid = false;
   }
   return PFEM2GPU_OK;
 }

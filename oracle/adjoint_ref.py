@@ -11,34 +11,54 @@ x <- x + dt u(x) with u = sum_i N_i(xi) V[dof_i] on box cells
 refills are placed at sub-cell centres with v = sum_i N_i(xi_c) V[dof_i]
 (src/pfem2particlehandler.cpp:381-409); after the last step the projection
 P = S / W (src/pfem2particlehandler.cpp:191-209) and L = 1/2 |P - target|^2.
-2D structured box meshes only (the adjoint workload).
+Structured box meshes in 2D and 3D (the 2D adjoint workload and a 3D check).
 """
 import numpy as np
 
 
 def _shape(xi):
-    a, b = xi[0], xi[1]
-    l0a, l0b = 1.0 - a, 1.0 - b
-    return np.stack([l0a * l0b, a * l0b, l0a * b, a * b])
+    """N_i(xi) = prod_k (bit_k(i) ? xi_k : 1 - xi_k), the product taken in k
+    order (as the GPU's shape values), as [2^D][n]."""
+    D = xi.shape[0]
+    out = []
+    for i in range(1 << D):
+        v = xi[0] if (i & 1) else (1.0 - xi[0])
+        for k in range(1, D):
+            v = v * (xi[k] if (i >> k) & 1 else (1.0 - xi[k]))
+        out.append(v)
+    return np.stack(out)
 
 
 def _dshape(xi):
     """dN_i/dxi_d as [i][d][n]."""
-    a, b = xi[0], xi[1]
-    return np.stack([np.stack([-(1.0 - b), -(1.0 - a)]), np.stack([1.0 - b, -a]),
-                     np.stack([-b, 1.0 - a]), np.stack([b, a])])
+    D = xi.shape[0]
+    out = []
+    for i in range(1 << D):
+        row = []
+        for d in range(D):
+            v = np.ones_like(xi[0])
+            for k in range(D):
+                if k == d:
+                    v = v * (1.0 if (i >> k) & 1 else -1.0)
+                else:
+                    v = v * (xi[k] if (i >> k) & 1 else (1.0 - xi[k]))
+            row.append(v)
+        out.append(np.stack(row))
+    return np.stack(out)
 
 
 class FrozenModel:
     def __init__(self, mesh, params, x0, cells0, srcs, end_cells):
-        assert mesh["dim"] == 2
+        self.D = int(mesh["dim"])
+        assert self.D in (2, 3)
+        self.NV = 1 << self.D
         self.mesh = mesh
-        self.nx, self.ny = mesh["n"]
+        self.n = list(mesh["n"])
         self.X = mesh["X"]
         self.cv = mesh["cell_vertices"]
         self.n_sub = int(params["n_sub"])
         self.dt = float(params["dt"]) / float(self.n_sub)
-        self.q = list(params["particles_per_cell"])
+        self.q = list(params["particles_per_cell"])[: self.D]
         self.hp = [1.0 / q for q in self.q]
         self.x0 = np.array(x0, dtype=np.float64)
         self.c0 = np.array(cells0, dtype=np.int64)
@@ -50,36 +70,66 @@ class FrozenModel:
     def _X0h(self, c):
         v0 = self.cv[c, 0]
         X0 = self.X[:, v0]
-        h = np.stack([self.X[0, self.cv[c, 1]] - X0[0], self.X[1, self.cv[c, 2]] - X0[1]])
+        h = np.stack([self.X[k, self.cv[c, 1 << k]] - X0[k] for k in range(self.D)])
         return X0, h
 
     def _interp(self, V, c, xi):
+        """u_k = sum_i N_i V_k[dof_i], vertex-outer, from 0.0"""
         N = _shape(xi)
-        u = np.zeros((2, c.shape[0]))
-        for i in range(4):
+        u = np.zeros((self.D, c.shape[0]))
+        for i in range(self.NV):
             d = self.cv[c, i]
-            u[0] = u[0] + N[i] * V[0][d]
-            u[1] = u[1] + N[i] * V[1][d]
+            for k in range(self.D):
+                u[k] = u[k] + N[i] * V[k][d]
         return u
+
+    def _fwd(self, c, xi):
+        """x = sum_i N_i X_i (the forward map of a refill position)"""
+        N = _shape(xi)
+        x = np.zeros((self.D, c.shape[0]))
+        for i in range(self.NV):
+            for k in range(self.D):
+                x[k] = x[k] + N[i] * self.X[k, self.cv[c, i]]
+        return x
+
+    def _parts(self, pp):
+        """part indices (x slowest) of linear part numbers pp"""
+        pk = [None] * self.D
+        rem = pp.copy()
+        for k in range(self.D - 1, -1, -1):
+            Q = self.q[k] + 1
+            pk[k] = rem % Q
+            rem = rem // Q
+        return pk
 
     def _locate(self, x, c):
         """current cell if it contains x (faces inclusive), else the cell that
         contains x on the structured grid; -1 outside the mesh"""
         X0, h = self._X0h(c)
         xi = (x - X0) / h
-        keep = (xi[0] >= 0) & (xi[0] <= 1) & (xi[1] >= 0) & (xi[1] <= 1)
+        keep = np.ones(c.shape[0], dtype=bool)
+        for k in range(self.D):
+            keep &= (xi[k] >= 0) & (xi[k] <= 1)
         lo = self.X[:, 0]
         hx = self.mesh["h"]
-        i = np.floor((x[0] - lo[0]) / hx[0]).astype(np.int64)
-        j = np.floor((x[1] - lo[1]) / hx[1]).astype(np.int64)
-        out = (i < 0) | (i >= self.nx) | (j < 0) | (j >= self.ny) | ~np.isfinite(x[0]) | ~np.isfinite(x[1])
-        cn = np.where(out, -1, i + self.nx * np.clip(j, 0, self.ny - 1))
+        out = np.zeros(c.shape[0], dtype=bool)
+        cn = np.zeros(c.shape[0], dtype=np.int64)
+        stride = 1
+        for k in range(self.D):
+            ik = np.floor((x[k] - lo[k]) / hx[k])
+            ok = np.isfinite(x[k])
+            ik = np.where(ok, ik, -1).astype(np.int64)
+            out |= (ik < 0) | (ik >= self.n[k]) | ~ok
+            cn += np.clip(ik, 0, self.n[k] - 1) * stride
+            stride *= self.n[k]
+        cn = np.where(out, -1, cn)
         return np.where(keep, c, cn)
 
     def forward(self, V, v0, record=False):
         """Final (x, v, cell) after all steps. record=True runs the locate and
         stores the cell, reference position and liveness of every particle at
         every substep (the frozen path); otherwise the stored path is reused."""
+        D = self.D
         if record:
             self.paths = []
         x, v, c = self.x0.copy(), np.array(v0, dtype=np.float64).copy(), self.c0.copy()
@@ -111,8 +161,8 @@ class FrozenModel:
             # redistribution with the recorded identity map
             nq = src.shape[0]
             surv = src >= 0
-            xn = np.zeros((2, nq))
-            vn = np.zeros((2, nq))
+            xn = np.zeros((D, nq))
+            vn = np.zeros((D, nq))
             xn[:, surv] = x[:, src[surv]]
             vn[:, surv] = v[:, src[surv]]
             if record:
@@ -120,46 +170,38 @@ class FrozenModel:
                 assert np.array_equal(cc[src[surv]], ec[surv]), "reference cells differ from the recorded ones"
             ref = ~surv
             if ref.any():
-                pp = -1 - src[ref]
-                Q1 = self.q[1] + 1
-                pi0, pi1 = pp // Q1, pp % Q1
-                xi = np.stack([(pi0 + 0.5) * self.hp[0], (pi1 + 0.5) * self.hp[1]])
+                pk = self._parts(-1 - src[ref])
+                xi = np.stack([(pk[k] + 0.5) * self.hp[k] for k in range(D)])
                 cr = ec[ref]
-                N = _shape(xi)
-                xr = np.zeros((2, cr.shape[0]))
-                for i in range(4):
-                    xr[0] = xr[0] + N[i] * self.X[0, self.cv[cr, i]]
-                    xr[1] = xr[1] + N[i] * self.X[1, self.cv[cr, i]]
-                xn[:, ref] = xr
+                xn[:, ref] = self._fwd(cr, xi)
                 vn[:, ref] = self._interp(V, cr, xi)
             x, v, c = xn, vn, ec.copy()
         return x, v, c
 
     def _snap(self, x, c, xi):
         """exact sub-cell centre for particles placed there (refills)"""
-        pk0 = (xi[0] * self.q[0]).astype(np.int64)
-        pk1 = (xi[1] * self.q[1]).astype(np.int64)
-        xc = np.stack([(pk0 + 0.5) * self.hp[0], (pk1 + 0.5) * self.hp[1]])
-        N = _shape(xc)
-        F = np.zeros_like(x)
-        for i in range(4):
-            F[0] = F[0] + N[i] * self.X[0, self.cv[c, i]]
-            F[1] = F[1] + N[i] * self.X[1, self.cv[c, i]]
-        hit = (F[0] == x[0]) & (F[1] == x[1]) & (pk0 >= 0) & (pk0 < self.q[0]) & (pk1 >= 0) & (pk1 < self.q[1])
+        D = self.D
+        pk = [(xi[k] * self.q[k]).astype(np.int64) for k in range(D)]
+        xc = np.stack([(pk[k] + 0.5) * self.hp[k] for k in range(D)])
+        F = self._fwd(c, xc)
+        hit = np.ones(x.shape[1], dtype=bool)
+        for k in range(D):
+            hit &= (F[k] == x[k]) & (pk[k] >= 0) & (pk[k] < self.q[k])
         return np.where(hit, xc, xi)
 
     def project(self, x, v, c):
+        D = self.D
         X0, h = self._X0h(c)
         xi = (x - X0) / h
         xi = self._snap(x, c, xi)
         N = _shape(xi)
         nd = self.X.shape[1]
-        S = np.zeros((2, nd))
+        S = np.zeros((D, nd))
         W = np.zeros(nd)
-        for i in range(4):
+        for i in range(self.NV):
             d = self.cv[c, i]
-            np.add.at(S[0], d, N[i] * v[0])
-            np.add.at(S[1], d, N[i] * v[1])
+            for k in range(D):
+                np.add.at(S[k], d, N[i] * v[k])
             np.add.at(W, d, N[i])
         return S / W, W, xi
 
@@ -171,6 +213,7 @@ class FrozenModel:
 
     def grad(self, V, v0, target):
         """(L, dL/dv0, dL/dV) by the reverse sweep over the recorded path."""
+        D = self.D
         x, v, c = self.forward(V, v0, record=True)
         P, W, xi = self.project(x, v, c)
         r = P - target
@@ -181,34 +224,33 @@ class FrozenModel:
         nd = self.X.shape[1]
         av = np.zeros_like(v)
         ax = np.zeros_like(x)
-        for i in range(4):
+        for i in range(self.NV):
             d = self.cv[c, i]
             a = r[:, d] / W[d]
             av += N[i] * a
             coef = (a * (v - P[:, d])).sum(axis=0)
             ax += dN[i] * coef
         ax = ax / h
-        gV = np.zeros((2, nd))
+        gV = np.zeros((D, nd))
         for s in range(len(self.srcs) - 1, -1, -1):
             src, ec = self.srcs[s], self.end_cells[s]
             path = self.paths[s]
             n_start = path[0][0].shape[0]
-            xs = np.zeros((2, n_start))
-            vs = np.zeros((2, n_start))
+            xs = np.zeros((D, n_start))
+            vs = np.zeros((D, n_start))
             surv = src >= 0
             xs[:, src[surv]] = ax[:, surv]
             vs[:, src[surv]] = av[:, surv]
             ref = ~surv
             if ref.any():
-                pp = -1 - src[ref]
-                Q1 = self.q[1] + 1
-                xi_r = np.stack([(pp // Q1 + 0.5) * self.hp[0], (pp % Q1 + 0.5) * self.hp[1]])
+                pk = self._parts(-1 - src[ref])
+                xi_r = np.stack([(pk[k] + 0.5) * self.hp[k] for k in range(D)])
                 Nr = _shape(xi_r)
                 cr = ec[ref]
-                for i in range(4):
+                for i in range(self.NV):
                     dd = self.cv[cr, i]
-                    np.add.at(gV[0], dd, Nr[i] * av[0, ref])
-                    np.add.at(gV[1], dd, Nr[i] * av[1, ref])
+                    for k in range(D):
+                        np.add.at(gV[k], dd, Nr[i] * av[k, ref])
             alive_end = path[self.n_sub][2]
             a = np.where(alive_end, xs, 0.0)
             for k in range(self.n_sub - 1, -1, -1):
@@ -218,12 +260,14 @@ class FrozenModel:
                 dNk = _dshape(xik)
                 _, hk = self._X0h(cc)
                 gx = np.zeros_like(a)
-                for i in range(4):
+                for i in range(self.NV):
                     dd = self.cv[cc, i]
                     m = alive_end
-                    np.add.at(gV[0], dd[m], self.dt * Nk[i][m] * a[0][m])
-                    np.add.at(gV[1], dd[m], self.dt * Nk[i][m] * a[1][m])
-                    av_i = a[0] * V[0][dd] + a[1] * V[1][dd]
+                    for kk in range(D):
+                        np.add.at(gV[kk], dd[m], self.dt * Nk[i][m] * a[kk][m])
+                    av_i = a[0] * V[0][dd]
+                    for kk in range(1, D):
+                        av_i = av_i + a[kk] * V[kk][dd]
                     gx += dNk[i] * av_i
                 a = np.where(alive_end, a + self.dt * gx / hk, 0.0)
             ax, av = a, vs

@@ -187,6 +187,13 @@ def config(name, scale=1.0):
         x, cells = seed_uniform(mesh, np.full(nc, 16), seed=5)
         params = dict(n_sub=5, dt=cfl_dt(mesh, V), max_per_part=5, particles_per_cell=[4, 4])
         steps = 20
+    elif name == "adjoint3d":    # 16^3 Q1 unit cube, 8 random/cell, ABC, 10 steps (3D adjoint check)
+        mesh = structured_mesh(sc([16, 16, 16]), [0.0, 0.0, 0.0], [1.0, 1.0, 1.0])
+        V = abc_flow(mesh)
+        nc = mesh["cell_vertices"].shape[0]
+        x, cells = seed_uniform(mesh, np.full(nc, 8), seed=6)
+        params = dict(n_sub=5, dt=cfl_dt(mesh, V), max_per_part=5, particles_per_cell=[2, 2, 2])
+        steps = 10
     else:
         raise KeyError(name)
     return {"name": name, "mesh": mesh, "V": V, "x": x, "cells": cells, "params": params, "steps": steps}

@@ -21,6 +21,7 @@ SENTINEL = -7.0
 def trace(cfg, V, steps):
     """(x0, cells0, srcs, end_cells) of `steps` oracle moves under V."""
     mesh, params = cfg["mesh"], cfg["params"]
+    D = mesh["dim"]
     q = params["particles_per_cell"]
     x, cells = cfg["x"], cfg["cells"]
     x0, c0 = x.copy(), cells.copy()
@@ -29,15 +30,19 @@ def trace(cfg, V, steps):
         o = Oracle(mesh, params)
         o.initialize_maps()
         n = cells.shape[0]
-        v = np.stack([np.arange(n, dtype=np.float64), np.full(n, SENTINEL)])
+        v = np.zeros((D, n))
+        v[0] = np.arange(n, dtype=np.float64)
+        v[1] = SENTINEL
         assert o.insert(x, cells, v) == 0
         o.move(V)
         d = o.download()
         surv = d["v"][1] == SENTINEL
         src = np.empty(d["cell"].shape[0], dtype=np.int64)
         src[surv] = d["v"][0][surv].astype(np.int64)
-        pk = [np.floor(d["xi"][k][~surv] * q[k]).astype(np.int64) for k in range(2)]
-        src[~surv] = -1 - (pk[0] * (q[1] + 1) + pk[1])
+        lin = np.zeros(int((~surv).sum()), dtype=np.int64)
+        for k in range(D):  # part number, x slowest
+            lin = lin * (q[k] + 1) + np.floor(d["xi"][k][~surv] * q[k]).astype(np.int64)
+        src[~surv] = -1 - lin
         srcs.append(src)
         ecs.append(d["cell"].astype(np.int64))
         x, cells = d["x"], d["cell"]
@@ -45,15 +50,23 @@ def trace(cfg, V, steps):
     return x0, c0, srcs, ecs
 
 
-@pytest.fixture(scope="module")
-def case():
-    cfg = W.config("adjoint2d", scale=1.0 / 16)
-    V = W.poiseuille(cfg["mesh"], 1.0)
-    Vt = W.poiseuille(cfg["mesh"], 1.05)
-    x0, c0, srcs, ecs = trace(cfg, V, 4)
+@pytest.fixture(scope="module", params=["adjoint2d", "adjoint3d"])
+def case(request):
+    if request.param == "adjoint2d":  # 16x8 channel, Poiseuille, 4 steps
+        cfg = W.config("adjoint2d", scale=1.0 / 16)
+        V = W.poiseuille(cfg["mesh"], 1.0)
+        Vt = W.poiseuille(cfg["mesh"], 1.05)
+        steps = 4
+    else:  # 8^3 unit cube, ABC flow, 3 steps; particles leave through the faces
+        cfg = W.config("adjoint3d", scale=1.0 / 2)
+        V = W.abc_flow(cfg["mesh"])
+        Vt = W.abc_flow(cfg["mesh"], A=1.05)
+        steps = 3
+    x0, c0, srcs, ecs = trace(cfg, V, steps)
     assert any((s < 0).any() for s in srcs), "workload must exercise refills"
     m = FrozenModel(cfg["mesh"], cfg["params"], x0, c0, srcs, ecs)
-    v0 = np.stack([np.interp(x0[1], [0, 1], [0.3, 0.9]), 0.1 * np.sin(3 * x0[0])])
+    v0 = np.stack([np.interp(x0[1], [0, 1], [0.3, 0.9]), 0.1 * np.sin(3 * x0[0])] +
+                  [0.2 * np.cos(2 * x0[k]) for k in range(2, cfg["mesh"]["dim"])])
     # target: the projection of a perturbed run along the same frozen path
     m.forward(V, v0, record=True)
     xt, vt, ct = m.forward(Vt, 1.02 * v0)

@@ -2,10 +2,17 @@
 gradient of the same frozen-decision steps and against central finite
 differences. Marked gpu.
 
-Workload: the adjoint channel (Q1 mesh of [0,2]x[0,1] at 1/8 of 256x128 per
-direction, 16 seeded uniform particles per cell), Poiseuille amplitude 1.0,
-20 steps of move, final projection, loss 1/2 |P - P(amplitude 1.05)|^2,
-checkpoints every 5 steps; 64 seeded random directions, epsilon 1e-6."""
+Workloads:
+  - 2D: the adjoint channel (Q1 mesh of [0,2]x[0,1] at 1/8 of 256x128 per
+    direction, 16 seeded uniform particles per cell), Poiseuille amplitude
+    1.0, 20 steps of move, final projection, loss 1/2 |P - P(amplitude
+    1.05)|^2, checkpoints every 5 steps;
+  - 3D: the 16^3 Q1 unit cube with 8 seeded uniform particles per cell, ABC
+    flow (A, B, C) = (1, 0.7, 0.4), 12 steps (particles leave through the
+    faces, refills fill the emptied parts), target from A = 1.05,
+    checkpoints every 5 steps (a last segment of 2 steps).
+Central differences along 64 (2D) or 16 (3D) seeded random directions,
+epsilon 1e-6."""
 import sys
 
 import numpy as np
@@ -15,8 +22,12 @@ from pfem2dealii_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-STEPS = 20
-CKPT = 5
+CASES = {
+    "2d": dict(workload="adjoint2d", scale=1.0 / 8, steps=20, ckpt=5, fd_dirs=64,
+               field=lambda mesh, a: W.poiseuille(mesh, a)),
+    "3d": dict(workload="adjoint3d", scale=1.0, steps=12, ckpt=5, fd_dirs=16,
+               field=lambda mesh, a: W.abc_flow(mesh, A=a)),
+}
 
 
 def _say(msg):
@@ -36,42 +47,52 @@ def _handle(cfg, V, x=None, cells=None, v=None):
     return h
 
 
-@pytest.fixture(scope="module")
-def adjoint_case():
+@pytest.fixture(scope="module", params=sorted(CASES))
+def adjoint_case(request):
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     from oracle.adjoint_ref import FrozenModel
-    cfg = W.config("adjoint2d", scale=1.0 / 8)
+    cs = CASES[request.param]
+    steps, ckpt = cs["steps"], cs["ckpt"]
+    _say("%s: %s at scale %g, %d steps, checkpoints every %d" % (request.param, cs["workload"], cs["scale"], steps,
+                                                                  ckpt))
+    cfg = W.config(cs["workload"], scale=cs["scale"])
     mesh = cfg["mesh"]
-    V1 = W.poiseuille(mesh, 1.0)
-    V105 = W.poiseuille(mesh, 1.05)
+    V1 = cs["field"](mesh, 1.0)
+    V105 = cs["field"](mesh, 1.05)
     ht = _handle(cfg, V105)
-    for _ in range(STEPS):
+    for _ in range(steps):
         ht.move_particles(V105)
     target = ht.project_particle_fields()
+    ht.close()
+    assert np.isfinite(target).all()
     h = _handle(cfg, V1)
     d0 = h.download(xi=False)
     Vd = torch.tensor(V1, dtype=torch.float64, device="cuda")
     Td = torch.tensor(target, dtype=torch.float64, device="cuda")
-    loss, gv0, gV = h.adjoint(STEPS, CKPT, Vd, Td)
+    loss, gv0, gV = h.adjoint(steps, ckpt, Vd, Td)
     hr = _handle(cfg, V1)
     d0r = hr.download(xi=False)
     assert all(np.array_equal(d0[k], d0r[k]) for k in ("x", "v", "cell"))
     srcs, ecs = [], []
-    for _ in range(STEPS):
+    for _ in range(steps):
         srcs.append(hr.move_particles_traced(V1))
         ecs.append(hr.download(xi=False)["cell"])
     fin = h.download(xi=False)
     assert np.array_equal(fin["cell"], ecs[-1]) and np.array_equal(fin["x"], hr.download(xi=False)["x"])
+    assert any((s < 0).any() for s in srcs), "workload must exercise refills"
+    h.close()
+    hr.close()
     ref = FrozenModel(mesh, cfg["params"], d0["x"], d0["cell"], srcs, ecs)
-    return {"cfg": cfg, "V": V1, "target": target, "v0": d0["v"], "x0": d0["x"], "c0": d0["cell"], "loss": loss,
-            "gv0": gv0.cpu().numpy(), "gV": gV.cpu().numpy(), "ref": ref}
+    return {"name": request.param, "steps": steps, "ckpt": ckpt, "fd_dirs": cs["fd_dirs"], "cfg": cfg, "V": V1,
+            "target": target, "v0": d0["v"], "x0": d0["x"], "c0": d0["cell"], "loss": loss, "gv0": gv0.cpu().numpy(),
+            "gV": gV.cpu().numpy(), "ref": ref}
 
 
 def test_adjoint_matches_cpu_reference_gradient(adjoint_case):
-    _say("adjoint vs FP64 CPU reference gradient (32x16 channel, 20 steps)")
     c = adjoint_case
+    _say("%s: adjoint vs FP64 CPU reference gradient" % c["name"])
     L, gv0, gV = c["ref"].grad(c["V"], c["v0"], c["target"])
     assert abs(c["loss"] - L) <= 1e-11 * abs(L)
     assert np.abs(c["gv0"] - gv0).max() <= 1e-10 * np.abs(gv0).max()
@@ -80,14 +101,14 @@ def test_adjoint_matches_cpu_reference_gradient(adjoint_case):
 
 
 def test_adjoint_matches_finite_differences(adjoint_case):
-    _say("adjoint vs central finite differences, 64 directions, eps 1e-6")
     c = adjoint_case
+    _say("%s: adjoint vs central finite differences, %d directions, eps 1e-6" % (c["name"], c["fd_dirs"]))
     ref = c["ref"]
     ref.grad(c["V"], c["v0"], c["target"])  # records the frozen path
     rng = np.random.default_rng(7)
     eps = 1e-6
     worst = 0.0
-    for _ in range(64):
+    for _ in range(c["fd_dirs"]):
         dV = rng.standard_normal(c["V"].shape)
         dv = rng.standard_normal(c["v0"].shape)
         gdot = float((c["gV"] * dV).sum() + (c["gv0"] * dv).sum())
@@ -99,16 +120,17 @@ def test_adjoint_matches_finite_differences(adjoint_case):
 
 
 def test_adjoint_through_autograd(adjoint_case):
-    _say("adjoint wrapped in torch.autograd.Function")
     import torch
     c = adjoint_case
+    _say("%s: adjoint wrapped in torch.autograd.Function" % c["name"])
     cfg = c["cfg"]
 
     class AdvectProject(torch.autograd.Function):
         @staticmethod
         def forward(ctx, V, v0):
             h = _handle(cfg, None, x=c["x0"], cells=c["c0"], v=v0.detach().cpu().numpy())
-            loss, gv0, gV = h.adjoint(STEPS, CKPT, V.detach().contiguous(), torch.tensor(c["target"], device="cuda"))
+            T = torch.tensor(c["target"], device="cuda")
+            loss, gv0, gV = h.adjoint(c["steps"], c["ckpt"], V.detach().contiguous(), T)
             ctx.save_for_backward(gv0, gV)
             h.close()
             return torch.tensor(loss, dtype=torch.float64, device=V.device)

@@ -351,24 +351,28 @@ __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __re
       a[k] = p < n_start ? xt[k][p] : 0.0;
       live = live || a[k] != 0.0;
     }
-    int n_sub = 0;
-    double dt = 0.0;
+    // the whole path in one round of loads, issued together with the
+    // adjoint's (a lane that turns out dead ignores what it loaded)
     int pc[kAdjMaxSub];
     double pxi[kAdjMaxSub][D];
 #pragma unroll
-    for (int s = 0; s < kAdjMaxSub; ++s) {  // the whole path, one round of loads
+    for (int s = 0; s < kAdjMaxSub; ++s) {
       pc[s] = -1;
 #pragma unroll
       for (int k = 0; k < D; ++k) pxi[s][k] = 0.0;
-      if (live && s < max_sub) {
+      if (p < n_start && s < max_sub) {
         const long o = s * path.stride + p;
         pc[s] = path.c[o];
 #pragma unroll
         for (int k = 0; k < D; ++k) pxi[s][k] = path.xi[k][o];
       }
     }
-    if (live) {
-      const Prm& pr = m.prm[inst_of(m, pc[0])];
+    // substep count and length: one instance's (a uniform load) or the
+    // instance of the particle's first cell
+    int n_sub = m.prm[0].n_sub;
+    double dt = m.prm[0].dt_sub;
+    if (m.cell_inst && live) {
+      const Prm& pr = m.prm[m.cell_inst[pc[0]]];
       n_sub = pr.n_sub;
       dt = pr.dt_sub;
     }

@@ -154,6 +154,42 @@ int pfem2gpu_write_particles_vtu(const char* path, int32_t dim, int64_t n, const
                                  int32_t subdomain);
 int pfem2gpu_write_particles_pvtu(const char* path, int32_t timestep_number, int32_t n_ranks);
 
+/* ---- Particle records of the reference's MPI exchange ----
+ * A record is the byte image pfem2Particle::write_data produces
+ * (src/pfem2particle.cpp:221-256): u32 id, i32 tria_position (the cell),
+ * i32 cell_dofs[2^dim], then f64 location, reference_location, velocity and
+ * velocity_ext, dim values each, with no padding. */
+
+/* serialized_size_in_bytes() (src/pfem2particle.cpp:213-219): 88 for dim 2,
+ * 136 for dim 3, 0 otherwise. */
+int64_t pfem2gpu_particle_record_size(int32_t dim);
+
+/* The send side of send_recv_particles (src/pfem2particlehandler.cpp:528-548):
+ * serialises the particles of the listed cells into `out`, destination by
+ * destination, records back to back. Destination i owns the cells
+ * cells[dest_offsets[i] .. dest_offsets[i+1]) (host arrays; each cell listed
+ * at most once); within a destination the records follow the handle's
+ * particle order. bytes[i] (host, [n_dest]) receives destination i's byte
+ * count (n_send_data); they are filled in even when `capacity` is too small,
+ * which is then an error. The id written is the particle's slot in the
+ * handle and velocity_ext is written as zero (the reference sets it at the
+ * start of every substep before reading it). With remove != 0 the packed
+ * particles leave the handle (:484-486). `out` is host or device memory
+ * (`where`; device memory 8-byte aligned). Synchronises. */
+int pfem2gpu_pack_particles(pfem2gpu_handle* h, int32_t n_dest, const int32_t* dest_offsets, const int32_t* cells,
+                            void* out, int64_t capacity, int where, int64_t* bytes, int remove);
+
+/* The receive side (src/pfem2particlehandler.cpp:607-615 with the reading
+ * constructor, src/pfem2particle.cpp:17-44): appends the particles of
+ * n_bytes of records (host or device memory per `where`), taking each one's
+ * location, velocity and cell (tria_position); the reference location is
+ * recomputed by the handle's inverse map and the DoFs come from its mesh.
+ * n_bytes must be a whole number of records (the reference's AssertThrow at
+ * :612-615). Records whose location is not inside their cell are rejected
+ * and counted in *n_rejected (may be NULL). Synchronises. */
+int pfem2gpu_unpack_particles(pfem2gpu_handle* h, const void* data, int64_t n_bytes, int where,
+                              int64_t* n_rejected);
+
 /* ---- Batched form (no counterpart in the reference) ----
  * Many independent flow instances in one handle: meshes[i] and params[i]
  * describe instance i; their cells, vertices and DoFs are concatenated in

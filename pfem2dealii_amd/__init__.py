@@ -59,7 +59,8 @@ EXPORTS = [
     "pfem2gpu_batch_create", "pfem2gpu_batch_layout", "pfem2gpu_step", "pfem2gpu_set_tuning", "pfem2gpu_get_tuning",
     "pfem2gpu_adjoint", "pfem2gpu_move_particles_traced",
     "pfem2gpu_output_particle_solution", "pfem2gpu_write_particles_vtu", "pfem2gpu_write_particles_pvtu",
-    "pfem2gpu_vertex_to_cells",
+    "pfem2gpu_vertex_to_cells", "pfem2gpu_particle_record_size", "pfem2gpu_pack_particles",
+    "pfem2gpu_unpack_particles",
 ]
 
 
@@ -109,6 +110,11 @@ def load_library(path=None):
                                                  ctypes.c_int32]
     lib.pfem2gpu_write_particles_pvtu.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32]
     lib.pfem2gpu_vertex_to_cells.argtypes = [_vp, _vp, _vp, _vp]
+    lib.pfem2gpu_particle_record_size.argtypes = [ctypes.c_int32]
+    lib.pfem2gpu_particle_record_size.restype = ctypes.c_int64
+    lib.pfem2gpu_pack_particles.argtypes = [_vp, ctypes.c_int32, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int, _vp,
+                                            ctypes.c_int]
+    lib.pfem2gpu_unpack_particles.argtypes = [_vp, _vp, ctypes.c_int64, ctypes.c_int, _i64p]
     if path is None:
         _lib = lib
     return lib
@@ -137,6 +143,12 @@ def write_particles_pvtu(path, timestep_number, n_ranks=1):
     lib = load_library()
     _check(lib, lib.pfem2gpu_write_particles_pvtu(os.fsencode(path), int(timestep_number), int(n_ranks)),
            "write_particles_pvtu")
+
+
+def particle_record_size(dim):
+    """Bytes of one particle record of the reference's MPI exchange
+    (pfem2Particle::serialized_size_in_bytes, src/pfem2particle.cpp:213-219)."""
+    return int(load_library().pfem2gpu_particle_record_size(int(dim)))
 
 
 def _is_device(a):
@@ -326,6 +338,54 @@ class ParticleHandler:
         _check(self.lib, self.lib.pfem2gpu_vertex_to_cells(self.h, off.ctypes.data, cells.ctypes.data,
                                                            dirs.ctypes.data), "vertex_to_cells")
         return off, cells, dirs
+
+    def pack_particles(self, cells_per_dest, remove=False, out=None):
+        """Serialises the particles of the listed cells into records of the
+        reference's wire format (the send side of send_recv_particles,
+        src/pfem2particlehandler.cpp:528-548): cells_per_dest[i] lists the
+        cells of destination i. Returns (data, bytes_per_dest); data is a new
+        host uint8 array, or `out` (a torch CUDA uint8 tensor) when given.
+        remove=True takes the packed particles out of the handle."""
+        lists = [np.ascontiguousarray(c, dtype=np.int32).ravel() for c in cells_per_dest]
+        off = np.zeros(len(lists) + 1, dtype=np.int32)
+        off[1:] = np.cumsum([len(c) for c in lists])
+        cells = np.concatenate(lists) if lists else np.zeros(0, dtype=np.int32)
+        rec = particle_record_size(self.dim)
+        if cells.size and (cells.min() < 0 or cells.max() >= self.n_cells):
+            raise ValueError("cell id out of range")
+        need = int(self.cell_counts()[cells].astype(np.int64).sum()) * rec if cells.size else 0
+        nbytes = np.zeros(max(len(lists), 1), dtype=np.int64)
+        if out is None:
+            data = np.empty(need, dtype=np.uint8)
+            ptr, cap, where = (data.ctypes.data if need else None), need, HOST
+        else:
+            if not _is_device(out):
+                raise ValueError("out must be a torch CUDA tensor")
+            data = out
+            ptr, cap, where = out.data_ptr(), out.numel() * out.element_size(), DEVICE
+        _check(self.lib, self.lib.pfem2gpu_pack_particles(self.h, len(lists), off.ctypes.data,
+                                                          cells.ctypes.data if cells.size else None, ptr, cap,
+                                                          where, nbytes.ctypes.data, 1 if remove else 0),
+               "pack_particles")
+        return data, nbytes[:len(lists)].copy()
+
+    def unpack_particles(self, data):
+        """Appends the particles of a buffer of records (the receive side,
+        src/pfem2particlehandler.cpp:607-615): a host bytes/uint8 array or a
+        torch CUDA uint8 tensor. Returns the number of rejected records
+        (location not inside the record's cell)."""
+        rej = ctypes.c_int64(0)
+        if _is_device(data):
+            n = data.numel() * data.element_size()
+            _check(self.lib, self.lib.pfem2gpu_unpack_particles(self.h, data.data_ptr(), n, DEVICE,
+                                                                ctypes.byref(rej)), "unpack_particles")
+        else:
+            buf = np.ascontiguousarray(np.frombuffer(data, dtype=np.uint8) if isinstance(data, (bytes, bytearray))
+                                       else data, dtype=np.uint8)
+            _check(self.lib, self.lib.pfem2gpu_unpack_particles(self.h, buf.ctypes.data if buf.size else None,
+                                                                buf.size, HOST, ctypes.byref(rej)),
+                   "unpack_particles")
+        return rej.value
 
     def cell_counts(self):
         c = np.empty(self.n_cells, dtype=np.int32)

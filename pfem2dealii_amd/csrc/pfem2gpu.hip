@@ -25,6 +25,7 @@
 #include "../../include/pfem2gpu.h"
 #include "pfem2gpu_adjoint.h"
 #include "pfem2gpu_output.h"
+#include "pfem2gpu_wire.h"
 
 using namespace pfem2gpu;
 
@@ -1678,6 +1679,145 @@ int pfem2gpu_timings(pfem2gpu_handle* h, int cap, const char** names, int64_t* l
   }
   *n_out = (int)h->totals.size();
   return PFEM2GPU_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <int D>
+int pack_impl(pfem2gpu_handle* h, int32_t n_dest, const int32_t* dest_offsets, const int32_t* cells, void* out,
+              int64_t capacity, int where, int64_t* bytes, int remove) {
+  const long L = dest_offsets[n_dest];
+  const long rec = WireLayout<D>::kSize;
+  for (int i = 0; i < n_dest; ++i) bytes[i] = 0;
+  if (L == 0) return PFEM2GPU_OK;
+  DevBuf mem(h);
+  int *dcells = nullptr, *cnt = nullptr, *offs = nullptr;
+  CHK(mem.get(&dcells, L));
+  CHK(mem.get(&cnt, L));
+  CHK(mem.get(&offs, L + 1));
+  HIPCHK(hipMemcpyAsync(dcells, cells, L * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  LAUNCH(h, "k_wire", k_wire_count, grid_for(L), 0, (int)L, dcells, h->cell_start, cnt);
+  CHK(scan(h, cnt, (int)L, offs));
+  std::vector<int> ho(L + 1);
+  HIPCHK(hipMemcpyAsync(ho.data(), offs, (L + 1) * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int i = 0; i < n_dest; ++i) bytes[i] = (int64_t)(ho[dest_offsets[i + 1]] - ho[dest_offsets[i]]) * rec;
+  const long total = ho[L];
+  if (total * rec > capacity)
+    return fail(PFEM2GPU_ERR_ARG, "pack_particles: output holds " + std::to_string(capacity) + " bytes, " +
+                                      std::to_string(total * rec) + " needed");
+  if (total > 0) {
+    unsigned long long* dst = nullptr;
+    if (where == PFEM2GPU_DEVICE) dst = static_cast<unsigned long long*>(out);
+    else CHK(mem.get(&dst, (size_t)total * WireLayout<D>::kWords));
+    const long tiles = (total + kWireTile - 1) / kWireTile;
+    LAUNCH_B(h, "k_wire", k_wire_pack<D>, (int)std::min<long>(tiles, 65536), kWireTile, 0, dmesh(h), (int)L, dcells,
+             offs, total, h->cell_start, dparts(h), dst);
+    if (where == PFEM2GPU_HOST)
+      HIPCHK(hipMemcpyAsync(out, dst, (size_t)total * rec, hipMemcpyDeviceToHost, h->stream));
+  }
+  if (remove && total > 0) {
+    unsigned char* flag = nullptr;
+    int *kcnt = nullptr, *ncs = nullptr;
+    CHK(mem.get(&flag, h->nc));
+    CHK(mem.get(&kcnt, h->nc));
+    CHK(mem.get(&ncs, h->nc + 1));
+    LAUNCH(h, "k_fill", k_fill_bytes, grid_for(h->nc), 0, flag, (long)h->nc);
+    LAUNCH(h, "k_wire", k_wire_flag, grid_for(L), 0, (int)L, dcells, flag);
+    LAUNCH(h, "k_wire", k_wire_keep, grid_for(h->nc), 0, h->nc, h->cell_start, flag, kcnt);
+    CHK(scan(h, kcnt, h->nc, ncs));
+    const DTmp T = dtmp(h);
+    LAUNCH(h, "k_wire", k_wire_compact<D>, grid_for(h->n_ub), 0, h->nc, h->cell_start, ncs, flag, dparts(h), T);
+    LAUNCH(h, "k_copy", k_copy_state, grid_for(h->n_ub), 0, ncs + h->nc, T.x[0], T.x[1], D > 2 ? T.x[2] : nullptr,
+           T.v[0], T.v[1], D > 2 ? T.v[2] : nullptr, T.cell, h->px[0], h->px[1], D > 2 ? h->px[2] : nullptr,
+           h->pv[0], h->pv[1], D > 2 ? h->pv[2] : nullptr, h->pcell);
+    HIPCHK(hipMemcpyAsync(h->cell_start, ncs, (h->nc + 1) * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
+    h->contrib_valid = false;
+    h->settle_ev_valid = false;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));  // the pool buffers are reused after return
+  return PFEM2GPU_OK;
+}
+
+template <int D>
+int unpack_impl(pfem2gpu_handle* h, const void* data, int64_t n_bytes, int where, int64_t* n_rejected) {
+  const long n = n_bytes / WireLayout<D>::kSize;
+  if (n == 0) return PFEM2GPU_OK;
+  CHK(reserve(h, n, n));
+  DevBuf mem(h);
+  const unsigned long long* src = nullptr;
+  if (where == PFEM2GPU_DEVICE) {
+    src = static_cast<const unsigned long long*>(data);
+  } else {
+    unsigned long long* st = nullptr;
+    CHK(mem.get(&st, (size_t)n * WireLayout<D>::kWords));
+    HIPCHK(hipMemcpyAsync(st, data, (size_t)n_bytes, hipMemcpyHostToDevice, h->stream));
+    src = st;
+  }
+  const long tiles = (n + kWireTile - 1) / kWireTile;
+  LAUNCH_B(h, "k_wire", k_wire_unpack<D>, (int)std::min<long>(tiles, 65536), kWireTile, 0, src, n, h->tx[0], h->tx[1],
+           h->tx[2], h->tv[0], h->tv[1], h->tv[2], h->tcell);
+  const double* sx[3] = {h->tx[0], h->tx[1], h->tx[2]};
+  const double* sv[3] = {h->tv[0], h->tv[1], h->tv[2]};
+  CHK(append_staged(h, n, sx, sv, h->tcell, n_rejected));
+  HIPCHK(hipStreamSynchronize(h->stream));  // the staging buffer returns to the pool
+  return PFEM2GPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t pfem2gpu_particle_record_size(int32_t dim) {
+  return dim == 2 ? WireLayout<2>::kSize : dim == 3 ? WireLayout<3>::kSize : 0;
+}
+
+
+int pfem2gpu_pack_particles(pfem2gpu_handle* h, int32_t n_dest, const int32_t* dest_offsets, const int32_t* cells,
+                            void* out, int64_t capacity, int where, int64_t* bytes, int remove) {
+  if (!h || !dest_offsets || !bytes) return fail(PFEM2GPU_ERR_ARG, "pack_particles: null argument");
+  if (n_dest < 0) return fail(PFEM2GPU_ERR_ARG, "pack_particles: n_dest < 0");
+  if (where != PFEM2GPU_HOST && where != PFEM2GPU_DEVICE) return fail(PFEM2GPU_ERR_ARG, "bad where");
+  if (dest_offsets[0] != 0) return fail(PFEM2GPU_ERR_ARG, "pack_particles: dest_offsets[0] must be 0");
+  for (int i = 0; i < n_dest; ++i)
+    if (dest_offsets[i + 1] < dest_offsets[i])
+      return fail(PFEM2GPU_ERR_ARG, "pack_particles: dest_offsets must be non-decreasing");
+  const long L = dest_offsets[n_dest];
+  if (L > 0 && !cells) return fail(PFEM2GPU_ERR_ARG, "pack_particles: null cell list");
+  if (L > h->nc) return fail(PFEM2GPU_ERR_ARG, "pack_particles: more listed cells than the mesh has");
+  std::vector<char> seen(h->nc, 0);
+  for (long i = 0; i < L; ++i) {
+    const int c = cells[i];
+    if (c < 0 || c >= h->nc) return fail(PFEM2GPU_ERR_ARG, "pack_particles: cell id out of range");
+    if (seen[c]) return fail(PFEM2GPU_ERR_ARG, "pack_particles: cell " + std::to_string(c) + " listed twice");
+    seen[c] = 1;
+  }
+  if (capacity < 0 || (capacity > 0 && !out)) return fail(PFEM2GPU_ERR_ARG, "pack_particles: bad output buffer");
+  if (where == PFEM2GPU_DEVICE && (reinterpret_cast<uintptr_t>(out) & 7))
+    return fail(PFEM2GPU_ERR_ARG, "pack_particles: device output must be 8-byte aligned");
+  CHK(flush(h));
+  return h->dim == 2 ? pack_impl<2>(h, n_dest, dest_offsets, cells, out, capacity, where, bytes, remove)
+                     : pack_impl<3>(h, n_dest, dest_offsets, cells, out, capacity, where, bytes, remove);
+}
+
+int pfem2gpu_unpack_particles(pfem2gpu_handle* h, const void* data, int64_t n_bytes, int where,
+                              int64_t* n_rejected) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  if (n_rejected) *n_rejected = 0;
+  if (where != PFEM2GPU_HOST && where != PFEM2GPU_DEVICE) return fail(PFEM2GPU_ERR_ARG, "bad where");
+  const int64_t rec = pfem2gpu_particle_record_size(h->dim);
+  if (n_bytes < 0 || n_bytes % rec != 0)
+    return fail(PFEM2GPU_ERR_ARG, "unpack_particles: " + std::to_string(n_bytes) +
+                                      " bytes is not a whole number of " + std::to_string(rec) + "-byte records");
+  if (n_bytes > 0 && !data) return fail(PFEM2GPU_ERR_ARG, "unpack_particles: null data");
+  if (n_bytes / rec > 0x7fffffffL) return fail(PFEM2GPU_ERR_ARG, "unpack_particles: too many records");
+  if (where == PFEM2GPU_DEVICE && (reinterpret_cast<uintptr_t>(data) & 7))
+    return fail(PFEM2GPU_ERR_ARG, "unpack_particles: device data must be 8-byte aligned");
+  CHK(flush(h));
+  return h->dim == 2 ? unpack_impl<2>(h, data, n_bytes, where, n_rejected)
+                     : unpack_impl<3>(h, data, n_bytes, where, n_rejected);
 }
 
 }  // extern "C"

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     txt = open(os.path.join(ROOT, "include", "pfem2gpu.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*|void\*)\s+(pfem2gpu_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*|void\*)\s+(pfem2gpu_\w+)\s*\(", txt, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

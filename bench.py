@@ -282,6 +282,72 @@ def calibrate_copy(n=1 << 27, reps=5):
     print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "reps": reps}), flush=True)
 
 
+def wire_cost(workload="taylor_green2d", scale=1.0, reps=3):
+    """Particle records of the reference's MPI exchange (88 bytes in 2D, 136
+    in 3D) on the full workload: after one move, the particles of the upper
+    half of the cells (one subdomain's worth) are packed into a device
+    buffer, then unpacked into an empty handle on the same mesh. Kernel
+    times from HIP events around the launches (median over `reps` calls after
+    one warm-up call); algorithmic bytes per record: pack reads x and v and
+    writes the record (16*dim + record bytes), unpack reads the record and
+    writes x, v and cell to the staging arrays (record + 16*dim + 4 bytes).
+    The unpack call also re-bins the received particles (the insert path),
+    which its wall time includes."""
+    import torch
+    from pfem2dealii_amd import ParticleHandler, particle_record_size, workloads as W
+    cfg = W.config(workload, scale=scale)
+    mesh = cfg["mesh"]
+    dim = mesh["dim"]
+    rec = particle_record_size(dim)
+    V = torch.tensor(cfg["V"], dtype=torch.float64, device="cuda")
+    h = ParticleHandler(mesh, cfg["params"])
+    h.initialize_maps()
+    h.insert_particles(cfg["x"], cfg["cells"])
+    h.set_velocity_from_field(V)
+    h.move_particles(V)
+    nc = h.n_cells
+    lists = [np.arange(nc // 2, nc, dtype=np.int32)]
+    n_rec = int(h.cell_counts()[nc // 2:].astype(np.int64).sum())
+    buf = torch.empty(n_rec * rec, dtype=torch.uint8, device="cuda")
+    h.pack_particles(lists, out=buf)
+    walls, kms = [], []
+    for _ in range(reps):
+        h.set_timing(True)
+        t0 = time.perf_counter()
+        h.pack_particles(lists, out=buf)
+        walls.append(time.perf_counter() - t0)
+        kms.append(h.timings()["k_wire_pack"][1])
+        h.set_timing(False)
+    h.close()
+    pack_ms, pack_wall = float(np.median(kms)), float(np.median(walls)) * 1e3
+    h2 = ParticleHandler(mesh, cfg["params"])
+    h2.initialize_maps()
+    h2.reserve_particles(n_rec)
+    h2.unpack_particles(buf)
+    uwalls, ukms = [], []
+    for _ in range(reps):
+        h2.clear_particles()
+        h2.synchronize()
+        h2.set_timing(True)
+        t0 = time.perf_counter()
+        rej = h2.unpack_particles(buf)
+        uwalls.append(time.perf_counter() - t0)
+        ukms.append(h2.timings()["k_wire_unpack"][1])
+        h2.set_timing(False)
+        assert rej == 0 and h2.count() == n_rec
+    h2.close()
+    unpack_ms, unpack_wall = float(np.median(ukms)), float(np.median(uwalls)) * 1e3
+    pack_b = n_rec * (16 * dim + rec)
+    unpack_b = n_rec * (rec + 16 * dim + 4)
+    return {"workload": workload, "records": n_rec, "record_bytes": rec,
+            "pack_kernel_ms": round(pack_ms, 4), "pack_call_ms": round(pack_wall, 3),
+            "pack_GBps": round(pack_b / (pack_ms * 1e-3) / 1e9, 1),
+            "pack_frac": round(pack_b / (pack_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "unpack_kernel_ms": round(unpack_ms, 4), "unpack_call_ms": round(unpack_wall, 3),
+            "unpack_GBps": round(unpack_b / (unpack_ms * 1e-3) / 1e9, 1),
+            "unpack_frac": round(unpack_b / (unpack_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,7 +361,9 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="tune launch shapes instead of the shipped table")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration) and exit")
-    ap.add_argument("--section", choices=("batched", "adjoint"), default=None,
+    ap.add_argument("--wire-reps", type=int, default=3,
+                    help="repetitions of the particle-record pack/unpack measurement (0: skip)")
+    ap.add_argument("--section", choices=("batched", "adjoint", "wire"), default=None,
                     help="only run the batched comparison or the adjoint cost measurement (for profiling) and "
                          "print its result")
     args = ap.parse_args()
@@ -313,6 +381,10 @@ def main():
         return
     if args.section == "adjoint":
         print(json.dumps({"section": "adjoint", "adjoint": adjoint_cost(max(1, args.adjoint_reps))}), flush=True)
+        return
+    if args.section == "wire":
+        print(json.dumps({"section": "wire", "wire": wire_cost(args.workload, args.scale, max(1, args.wire_reps))}),
+              flush=True)
         return
 
     import torch
@@ -432,6 +504,8 @@ def main():
         out["batched"] = batched_comparison(args.batched_steps, 1, force_tune=args.autotune)
     if args.adjoint_reps > 0:
         out["adjoint"] = adjoint_cost(args.adjoint_reps)
+    if args.wire_reps > 0:
+        out["wire"] = wire_cost(args.workload, args.scale, args.wire_reps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload)
         out["cpu_baseline"]["host"] = {"nproc": os.cpu_count()}

@@ -1712,9 +1712,9 @@ int pack_impl(pfem2gpu_handle* h, int32_t n_dest, const int32_t* dest_offsets, c
     unsigned long long* dst = nullptr;
     if (where == PFEM2GPU_DEVICE) dst = static_cast<unsigned long long*>(out);
     else CHK(mem.get(&dst, (size_t)total * WireLayout<D>::kWords));
-    const long tiles = (total + kWireTile - 1) / kWireTile;
-    LAUNCH_B(h, "k_wire", k_wire_pack<D>, (int)std::min<long>(tiles, 65536), kWireTile, 0, dmesh(h), (int)L, dcells,
-             offs, total, h->cell_start, dparts(h), dst);
+    const long groups = (L + kWireCells - 1) / kWireCells;
+    LAUNCH_B(h, "k_wire_pack", k_wire_pack<D>, (int)std::min<long>(groups, 65536), kWireTile, 0, dmesh(h), (int)L,
+             dcells, offs, h->cell_start, dparts(h), dst);
     if (where == PFEM2GPU_HOST)
       HIPCHK(hipMemcpyAsync(out, dst, (size_t)total * rec, hipMemcpyDeviceToHost, h->stream));
   }
@@ -1757,7 +1757,7 @@ int unpack_impl(pfem2gpu_handle* h, const void* data, int64_t n_bytes, int where
     src = st;
   }
   const long tiles = (n + kWireTile - 1) / kWireTile;
-  LAUNCH_B(h, "k_wire", k_wire_unpack<D>, (int)std::min<long>(tiles, 65536), kWireTile, 0, src, n, h->tx[0], h->tx[1],
+  LAUNCH_B(h, "k_wire_unpack", k_wire_unpack<D>, (int)std::min<long>(tiles, 65536), kWireTile, 0, src, n, h->tx[0], h->tx[1],
            h->tx[2], h->tv[0], h->tv[1], h->tv[2], h->tcell);
   const double* sx[3] = {h->tx[0], h->tx[1], h->tx[2]};
   const double* sv[3] = {h->tv[0], h->tv[1], h->tv[2]};

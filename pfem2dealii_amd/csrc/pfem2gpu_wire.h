@@ -45,59 +45,73 @@ __global__ void k_wire_count(int L, const int* __restrict__ cells, const int* __
   }
 }
 
-// Record r of the output belongs to listed cell i with offs[i] <= r <
-// offs[i+1] (offs: exclusive scan of the listed counts) and is that cell's
-// particle r - offs[i]. Each thread builds one record in LDS; the tile is
-// then written out with contiguous 64-bit stores. The id written is the
-// particle's slot in the handle (the reference never reads ids in arithmetic,
-// and its own are uninitialised counters); velocity_ext is written as zero
-// (the reference overwrites it at the start of every substep,
+// Records of the listed cells [i0, i0 + kWireCells) are contiguous in the
+// output, offs[i0] .. offs[i0 + kWireCells) (offs: exclusive scan of the listed
+// counts), so a workgroup takes one such group: the group's offsets and cell
+// ids go to LDS, and each thread finds its record's cell there (a search over
+// at most kWireCells entries), builds the record in LDS, and the tile is
+// written out with contiguous 64-bit stores. The id written is the particle's
+// slot in the handle (the reference never reads ids in arithmetic, and its
+// own are uninitialised counters); velocity_ext is written as zero (the
+// reference overwrites it at the start of every substep,
 // src/pfem2particlehandler.cpp:152-161, before anything reads it).
+constexpr int kWireCells = 32;
+
 template <int D>
 __global__ __launch_bounds__(kWireTile) void k_wire_pack(DMesh m, int L, const int* __restrict__ cells,
-                                                        const int* __restrict__ offs, long total,
-                                                        const int* __restrict__ cs, DParts P,
-                                                        unsigned long long* __restrict__ out) {
+                                                        const int* __restrict__ offs, const int* __restrict__ cs,
+                                                        DParts P, unsigned long long* __restrict__ out) {
   using WL = WireLayout<D>;
   constexpr int W = WL::kWords;
   constexpr int NV = WL::NV;
   __shared__ unsigned long long s_rec[kWireTile * W];
-  const long ntiles = (total + kWireTile - 1) / kWireTile;
-  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const long r0 = tile * kWireTile;
-    const int cnt = (int)min((long)kWireTile, total - r0);
-    const long r = r0 + threadIdx.x;
-    if ((int)threadIdx.x < cnt) {
-      int lo = 0, hi = L;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if ((long)offs[mid] <= r) lo = mid;
-        else hi = mid;
+  __shared__ int s_off[kWireCells + 1];
+  __shared__ int s_cell[kWireCells];
+  const int ngroups = (L + kWireCells - 1) / kWireCells;
+  for (int g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const int i0 = g * kWireCells;
+    const int nci = min(kWireCells, L - i0);
+    if ((int)threadIdx.x <= nci) s_off[threadIdx.x] = offs[i0 + threadIdx.x];
+    if ((int)threadIdx.x < nci) s_cell[threadIdx.x] = cells[i0 + threadIdx.x];
+    __syncthreads();
+    const long base = s_off[0];
+    const int total = s_off[nci] - s_off[0];
+    for (int j0 = 0; j0 < total; j0 += kWireTile) {
+      const int cnt = min(kWireTile, total - j0);
+      if ((int)threadIdx.x < cnt) {
+        const int j = s_off[0] + j0 + threadIdx.x;
+        int lo = 0, hi = nci;  // largest lo with s_off[lo] <= j
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_off[mid] <= j) lo = mid;
+          else hi = mid;
+        }
+        const int c = s_cell[lo];
+        const int p = cs[c] + (j - s_off[lo]);
+        double x[D], xi[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) x[k] = P.x[k][p];
+        recover_xi_c<D>(m, m.prm[inst_of(m, c)], c, x, xi);
+        unsigned long long* w = s_rec + threadIdx.x * W;
+        w[0] = wire_pair((unsigned int)p, (unsigned int)c);
+#pragma unroll
+        for (int q = 0; q < NV / 2; ++q)
+          w[WL::kDofs / 8 + q] = wire_pair((unsigned int)m.cd[(size_t)c * NV + 2 * q],
+                                           (unsigned int)m.cd[(size_t)c * NV + 2 * q + 1]);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          w[WL::kLoc / 8 + k] = (unsigned long long)__double_as_longlong(x[k]);
+          w[WL::kRef / 8 + k] = (unsigned long long)__double_as_longlong(xi[k]);
+          w[WL::kVel / 8 + k] = (unsigned long long)__double_as_longlong(P.v[k][p]);
+          w[WL::kExt / 8 + k] = 0ULL;
+        }
       }
-      const int c = cells[lo];
-      const int p = cs[c] + (int)(r - offs[lo]);
-      double x[D], xi[D];
-#pragma unroll
-      for (int k = 0; k < D; ++k) x[k] = P.x[k][p];
-      recover_xi_c<D>(m, m.prm[inst_of(m, c)], c, x, xi);
-      unsigned long long* w = s_rec + threadIdx.x * W;
-      w[0] = wire_pair((unsigned int)p, (unsigned int)c);
-#pragma unroll
-      for (int j = 0; j < NV / 2; ++j)
-        w[WL::kDofs / 8 + j] = wire_pair((unsigned int)m.cd[(size_t)c * NV + 2 * j],
-                                         (unsigned int)m.cd[(size_t)c * NV + 2 * j + 1]);
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        w[WL::kLoc / 8 + k] = (unsigned long long)__double_as_longlong(x[k]);
-        w[WL::kRef / 8 + k] = (unsigned long long)__double_as_longlong(xi[k]);
-        w[WL::kVel / 8 + k] = (unsigned long long)__double_as_longlong(P.v[k][p]);
-        w[WL::kExt / 8 + k] = 0ULL;
-      }
+      __syncthreads();
+      unsigned long long* dst = out + (base + j0) * W;
+      for (int i = threadIdx.x; i < cnt * W; i += kWireTile) dst[i] = s_rec[i];
+      __syncthreads();
     }
-    __syncthreads();
-    unsigned long long* dst = out + r0 * W;
-    for (int i = threadIdx.x; i < cnt * W; i += kWireTile) dst[i] = s_rec[i];
-    __syncthreads();
+    __syncthreads();  // s_off and s_cell are rewritten for the next group
   }
 }
 

@@ -163,6 +163,53 @@ def test_insert_rejects_outside(gpu_available):
     assert_particles_match(g.download(), o.download())
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_box_reference_position_is_the_host_quotient(gpu_available, dim):
+    """The box inverse map xi_k = (x_k - X0_k) / (X1_k - X0_k) divides with
+    the edge's reciprocal and one FMA residual step; the result must be the
+    IEEE quotient bit for bit (the reference's box case). Edge lengths that
+    are not powers of two (a mesh spacing of (hi - lo) / n with awkward ends),
+    positions at random reference coordinates and exactly on the faces."""
+    _say("box inverse map against the host quotient, dim %d" % dim)
+    from pfem2dealii_amd import ParticleHandler
+    n = [7, 5, 3][:dim]
+    mesh = W.structured_mesh(n, [0.1, -0.3, 1.0 / 7.0][:dim], [2.0 + 1.0 / 3.0, 0.7, 1e3 / 3.0][:dim])
+    nc = int(np.prod(n))
+    per = 400
+    cells = np.repeat(np.arange(nc, dtype=np.int32), per)
+    rng = np.random.default_rng(5 + dim)
+    xi = rng.random((dim, cells.shape[0]))
+    xi[:, ::9] = 0.0
+    xi[:, 4::9] = 1.0
+    xi[:, 2::11] = np.nextafter(1.0, 0.0)
+    cv = mesh["cell_vertices"]
+    X = mesh["X"]
+    X0 = np.stack([X[k][cv[cells, 0]] for k in range(dim)])
+    X1 = np.stack([X[k][cv[cells, (1 << dim) - 1]] for k in range(dim)])
+    x = X0 + xi * (X1 - X0)
+    x = np.minimum(np.maximum(x, X0), X1)  # inside the cell after rounding
+    params = dict(n_sub=5, dt=1e-3, max_per_part=5, particles_per_cell=[3, 3, 3][:dim])
+    h = ParticleHandler(mesh, params)
+    h.initialize_maps()
+    assert h.insert_particles(x, cells) == 0
+    d = h.download(xi=True)
+    order = np.lexsort([d["x"][k] for k in range(dim)][::-1] + [d["cell"]])
+    ref = np.lexsort([x[k] for k in range(dim)][::-1] + [cells])
+    xs, cs = x[:, ref], cells[ref]
+    assert np.array_equal(d["cell"][order], cs) and np.array_equal(d["x"][:, order], xs)
+    c0 = np.stack([X[k][cv[cs, 0]] for k in range(dim)])
+    c1 = np.stack([X[k][cv[cs, (1 << dim) - 1]] for k in range(dim)])
+    want = (xs - c0) / (c1 - c0)
+    # no position is near a sub-cell centre, where the library snaps a
+    # particle placed exactly at the centre's image back to the centre
+    q = np.array(params["particles_per_cell"], dtype=float)[:, None]
+    centre = (np.floor(want * q) + 0.5) / q
+    assert not np.all(np.abs(want - centre) <= 1e-9, axis=0).any()
+    got = d["xi"][:, order]
+    assert np.array_equal(got.view(np.int64), want.view(np.int64))
+    h.close()
+
+
 def test_empty_handler(gpu_available):
     _say("empty handler: move and project with no particles")
     cfg = W.config("channel2d", scale=0.25)

@@ -15,7 +15,8 @@ events around every launch, for the per-kernel durations and the roofline of
 the dominant kernel (algorithmic bytes per launch / average launch duration;
 the traffic figure comes from the committed counter summary). Then the
 256-instance batched form against plain launches, the adjoint's cost
-against its forward, and a CPU baseline: the oracle in reference-order mode on
+against its forward, the pack and unpack of the reference's particle records
+for half the workload's cells, and a CPU baseline: the oracle in reference-order mode on
 a bounded sample of the same workload, timed on this host. Prints one JSON
 line.
 """

@@ -750,26 +750,38 @@ __device__ int walk_box(const DMesh& m, const CellCache<D>& cc, const double* xi
   const int o0 = oc.x, nn = oc.y;
   if (nn > 8) return -2;  // caller falls back to the generic walk
   unsigned hits = 0;
-  int first = -1;
+  int first = -1, cfirst = -1;
   double xf[D];
-  for (int j = 0; j < nn; ++j) {
-    const int c2 = m.v2c[o0 + j];
-    if (c2 == cc.c) continue;  // the current cell already failed the same test
+  auto test = [&](int j, int c2) {
+    if (c2 == cc.c) return;  // the current cell already failed the same test
     double xn[D];
     if (inv_candidate<D>(m, c2, x, xn) && inside<D>(xn)) {
       hits |= 1u << j;
       if (first < 0) {
         first = j;
+        cfirst = c2;
 #pragma unroll
         for (int k = 0; k < D; ++k) xf[k] = xn[k];
       }
     }
+  };
+  if constexpr (D == 3) {
+    // the candidate ids in one round of loads (nn >= 1: the list holds the
+    // current cell), then the candidates in list order
+    int ids[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) ids[j] = m.v2c[o0 + (j < nn ? j : 0)];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (j < nn) test(j, ids[j]);
+  } else {
+    for (int j = 0; j < nn; ++j) test(j, m.v2c[o0 + j]);
   }
   if (hits == 0) return -1;
   if ((hits & (hits - 1)) == 0) {
 #pragma unroll
     for (int k = 0; k < D; ++k) xi_new[k] = xf[k];
-    return m.v2c[o0 + first];
+    return cfirst;
   }
   // two or more containing candidates: the first in the reference's order
   double vtp[D];

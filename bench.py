@@ -196,11 +196,14 @@ def batched_comparison(steps, warmup, n_inst=256, scale=1.0, log=None, force_tun
             "speedup": b_rate / p_rate, "tuning": tuning}
 
 
-def adjoint_cost(reps=3, scale=1.0):
-    """Cost of the adjoint against its forward on the adjoint channel (256x128
-    Q1 on [0,2]x[0,1], 16 seeded uniform particles per cell, Poiseuille
-    amplitude 1, 20 steps of 5 substeps, final projection, checkpoints every 5
-    steps). Forward: the 20 moves and the projection. Adjoint: one
+def adjoint_cost(reps=3, scale=1.0, workload="adjoint2d"):
+    """Cost of the adjoint against its forward. adjoint2d: the adjoint channel
+    (256x128 Q1 on [0,2]x[0,1], 16 seeded uniform particles per cell,
+    Poiseuille amplitude 1, 20 steps of 5 substeps, target from amplitude
+    1.05); adjoint3d: the unit cube (16^3 Q1 times `scale` per direction, 8
+    seeded uniform particles per cell, ABC flow with A = 1, 10 steps, target
+    from A = 1.05). Final projection, checkpoints every 5 steps. Forward: the
+    moves and the projection. Adjoint: one
     pfem2gpu_adjoint call (its own checkpointed forward, the loss, the reverse
     sweep with segment recompute). Wall clock with the stream drained on both
     sides; each repetition re-inserts the same initial particles into the
@@ -208,14 +211,16 @@ def adjoint_cost(reps=3, scale=1.0):
     buffers of the first); median of `reps` after one warm-up repetition."""
     import torch
     from pfem2dealii_amd import ParticleHandler, workloads as W
-    cfg = W.config("adjoint2d", scale=scale)
+    cfg = W.config(workload, scale=scale)
     steps, K = cfg["steps"], 5
+    field = (lambda a: W.poiseuille(cfg["mesh"], a)) if workload == "adjoint2d" else \
+        (lambda a: W.abc_flow(cfg["mesh"], A=a))
 
     def fresh(amp):
         h = ParticleHandler(cfg["mesh"], cfg["params"])
         h.initialize_maps()
         h.insert_particles(cfg["x"], cfg["cells"])
-        Vd = torch.tensor(W.poiseuille(cfg["mesh"], amp), dtype=torch.float64, device="cuda")
+        Vd = torch.tensor(field(amp), dtype=torch.float64, device="cuda")
         h.set_velocity_from_field(Vd)
         h.synchronize()
         return h, Vd
@@ -264,7 +269,7 @@ def adjoint_cost(reps=3, scale=1.0):
     hf.close()
     ha.close()
     f, a = float(np.median(fwd)), float(np.median(adj))
-    return {"workload": "adjoint2d", "mesh": "x".join(str(a_) for a_ in cfg["mesh"]["n"]), "particles_at_start": n0,
+    return {"workload": workload, "mesh": "x".join(str(a_) for a_ in cfg["mesh"]["n"]), "particles_at_start": n0,
             "steps": steps, "checkpoint_every": K, "forward_ms": round(f * 1e3, 3), "adjoint_ms": round(a * 1e3, 3),
             "ratio": round(a / f, 3), "loss": loss}
 
@@ -381,7 +386,8 @@ def main():
               flush=True)
         return
     if args.section == "adjoint":
-        print(json.dumps({"section": "adjoint", "adjoint": adjoint_cost(max(1, args.adjoint_reps))}), flush=True)
+        print(json.dumps({"section": "adjoint", "adjoint": adjoint_cost(max(1, args.adjoint_reps)),
+                          "adjoint3d": adjoint_cost(max(1, args.adjoint_reps), 4.0, "adjoint3d")}), flush=True)
         return
     if args.section == "wire":
         print(json.dumps({"section": "wire", "wire": wire_cost(args.workload, args.scale, max(1, args.wire_reps))}),
@@ -505,6 +511,7 @@ def main():
         out["batched"] = batched_comparison(args.batched_steps, 1, force_tune=args.autotune)
     if args.adjoint_reps > 0:
         out["adjoint"] = adjoint_cost(args.adjoint_reps)
+        out["adjoint3d"] = adjoint_cost(args.adjoint_reps, 4.0, "adjoint3d")
     if args.wire_reps > 0:
         out["wire"] = wire_cost(args.workload, args.scale, args.wire_reps)
     if not args.no_cpu:

@@ -332,7 +332,8 @@ __device__ __forceinline__ void adj_scatter(int key, const double (*add)[3], dou
 // `xt` holds dL/dx after the move on entry and before it on exit. A particle
 // the move deleted has no adjoint (it is no survivor), so lanes with a zero
 // adjoint skip their substeps (a zero adjoint contributes nothing). The
-// aggregated scatter at the end is reached by every lane of the wavefront.
+// aggregated scatters (at a cell change and at the end) are reached by every
+// lane of the wavefront.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __restrict__ Vc, int n_start, int max_sub,
                                                     DPath path, double* xt0, double* xt1, double* xt2,
@@ -379,8 +380,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __re
     CellCache<D> cc;
     cc.c = -1;
     // contributions summed per lane over consecutive substeps in one cell
-    // (acell), flushed directly when the cell changes, and once per wavefront
-    // through the aggregated scatter at the end
+    // (acell), handed to the aggregated scatter when the cell changes and
+    // once more at the end
     double acc[NV][3];
     int acell = -1;
 #pragma unroll
@@ -391,21 +392,24 @@ __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __re
     for (int s = kAdjMaxSub - 1; s >= 0; --s) {
       if (s >= max_sub) continue;
       const bool act = live && s < n_sub;
+      // a lane whose cell changes here hands its sums for the previous cell
+      // to the aggregated scatter (wave-uniform: every lane takes part), so
+      // lanes of one cell flushing together add once per (cell, vertex,
+      // component) instead of contending on the same addresses
+      const bool fl = act && pc[s] != acell && acell >= 0;
+      if (__ballot(fl)) {
+        adj_scatter<D>(fl ? acell : -1, acc, gcv, s_gcell[wave], s_acc[wave]);
+        if (fl) {
+#pragma unroll
+          for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int k = 0; k < D; ++k) acc[i][k] = 0.0;
+        }
+      }
       if (act) {
         const int c = pc[s];
         if (c != cc.c) load_cell<D>(m, Vc, c, cc, true);  // particles mostly keep their cell
-        if (c != acell) {
-          if (acell >= 0) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i)
-#pragma unroll
-              for (int k = 0; k < D; ++k) {
-                atomicAdd(&gcv[((size_t)acell * NV + i) * D + k], acc[i][k]);
-                acc[i][k] = 0.0;
-              }
-          }
-          acell = c;
-        }
+        acell = c;
         double N[NV], dN[NV][3];
         shape_and_grad<D>(pxi[s], N, dN);
         double gx[D];

@@ -9,8 +9,9 @@
 // last step the nodal projection P_j = S_j / W_j, S_j = sum_p N_j v_p,
 // W_j = sum_p N_j, and the loss L = 1/2 |P - target|^2.
 // Reverse: the projection's scatter becomes a gather (k_adj_init), the
-// interpolation's gather becomes a scatter into dL/dV (k_adj_move,
-// k_adj_unmap). dL/dV is accumulated with FP64 atomics.
+// interpolation's gather becomes a scatter into a per-(cell, vertex)
+// accumulator (k_adj_move, k_adj_unmap; FP64 atomics after aggregation over
+// each wavefront), gathered into dL/dV per DoF at the end (k_adj_gather).
 #pragma once
 
 #include "pfem2gpu_kernels.h"
@@ -208,49 +209,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_init(DMesh m, const int* __restr
   }
 }
 
-// Reverse of the redistribution: survivors hand their adjoints back to the
-// particle they came from; refills (constant position, velocity interpolated
-// from V) scatter N_i(xi_c) * dL/dv into dL/dV.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_adj_unmap(DMesh m, int pshift, int n_end, const int* __restrict__ src,
-                                                     const int* __restrict__ end_cell, const double* xe0,
-                                                     const double* xe1, const double* xe2, const double* ve0,
-                                                     const double* ve1, const double* ve2, double* xs0, double* xs1,
-                                                     double* xs2, double* vs0, double* vs1, double* vs2, double* g0,
-                                                     double* g1, double* g2) {
-  constexpr int NV = 1 << D;
-  const double* xe[3] = {xe0, xe1, xe2};
-  const double* ve[3] = {ve0, ve1, ve2};
-  double* xs[3] = {xs0, xs1, xs2};
-  double* vs[3] = {vs0, vs1, vs2};
-  double* g[3] = {g0, g1, g2};
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n_end; q += gridDim.x * blockDim.x) {
-    const int s = src[q];
-    if (s >= 0) {
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        xs[k][s] = xe[k][q];
-        vs[k][s] = ve[k][q];
-      }
-      continue;
-    }
-    const int c = end_cell[q];
-    const int inst = inst_of(m, c);
-    const Prm& pr = m.prm[inst];
-    const int code = m.ptab[(inst << pshift) + (-1 - s)];
-    double xi[D], N[NV];
-#pragma unroll
-    for (int d = 0; d < D; ++d) xi[d] = ((double)((code >> (8 * d)) & 0xff) + 1.0 / 2) * pr.h[d];
-    shape<D>(xi, N);
-    int dof[NV];
-    load_ids<D>(m.cd, c, dof);
-#pragma unroll
-    for (int i = 0; i < NV; ++i)
-#pragma unroll
-      for (int k = 0; k < D; ++k) atomicAdd(&g[k][dof[i]], N[i] * ve[k][q]);
-  }
-}
-
 // dL/dV[k][j] += sum over the (cell, vertex) entries of DoF j of the
 // per-(cell, vertex) accumulator (ascending cell, as the projection sums).
 template <int D>
@@ -320,6 +278,61 @@ __device__ __forceinline__ void adj_scatter(int key, const double (*add)[3], dou
     atomicAdd(&gcv[(size_t)s_gcell[gi] * W + (l - gi * W)], s_acc[l]);
   }
   wave_lds_sync();
+}
+
+// Reverse of the redistribution: survivors hand their adjoints back to the
+// particle they came from; refills (constant position, velocity interpolated
+// from V) add N_i(xi_c) * dL/dv to the per-(cell, vertex) accumulator gcv
+// through the aggregated scatter (the refills of a cell are neighbours in the
+// end state, so the lanes of one cell add once per entry). Every lane of a
+// wavefront reaches the scatter.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_adj_unmap(DMesh m, int pshift, int n_end, const int* __restrict__ src,
+                                                     const int* __restrict__ end_cell, const double* xe0,
+                                                     const double* xe1, const double* xe2, const double* ve0,
+                                                     const double* ve1, const double* ve2, double* xs0, double* xs1,
+                                                     double* xs2, double* vs0, double* vs1, double* vs2,
+                                                     double* __restrict__ gcv) {
+  constexpr int NV = 1 << D;
+  __shared__ int s_gcell[kBlock / 64][kAdjGroups];
+  __shared__ double s_acc[kBlock / 64][kAdjGroups * NV * D];
+  const int wave = threadIdx.x >> 6;
+  const double* xe[3] = {xe0, xe1, xe2};
+  const double* ve[3] = {ve0, ve1, ve2};
+  double* xs[3] = {xs0, xs1, xs2};
+  double* vs[3] = {vs0, vs1, vs2};
+  for (long base = (long)blockIdx.x * blockDim.x; base < n_end; base += (long)gridDim.x * blockDim.x) {
+    const int q = (int)(base + threadIdx.x);
+    const int s = q < n_end ? src[q] : 0;
+    int key = -1;
+    double add[NV][3];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) add[i][k] = 0.0;
+    if (q < n_end && s >= 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        xs[k][s] = xe[k][q];
+        vs[k][s] = ve[k][q];
+      }
+    } else if (q < n_end) {
+      const int c = end_cell[q];
+      const int inst = inst_of(m, c);
+      const Prm& pr = m.prm[inst];
+      const int code = m.ptab[(inst << pshift) + (-1 - s)];
+      double xi[D], N[NV];
+#pragma unroll
+      for (int d = 0; d < D; ++d) xi[d] = ((double)((code >> (8 * d)) & 0xff) + 1.0 / 2) * pr.h[d];
+      shape<D>(xi, N);
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int k = 0; k < D; ++k) add[i][k] = N[i] * ve[k][q];
+      key = c;
+    }
+    adj_scatter<D>(key, add, gcv, s_gcell[wave], s_acc[wave]);
+  }
 }
 
 // Reverse of the substeps of one move for every particle of the step's start

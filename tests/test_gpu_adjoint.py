@@ -147,3 +147,32 @@ def test_adjoint_through_autograd(adjoint_case):
     assert abs(loss.item() - c["loss"]) <= 1e-12 * abs(c["loss"])
     assert np.abs(V.grad.cpu().numpy() - 3.0 * c["gV"]).max() <= 1e-10 * np.abs(c["gV"]).max()
     assert np.abs(v0.grad.cpu().numpy() - 3.0 * c["gv0"]).max() <= 1e-10 * np.abs(c["gv0"]).max()
+
+
+def test_adjoint_rejects_non_box_cells_and_too_many_substeps():
+    """The adjoint differentiates the box-cell inverse map and holds at most 8
+    substeps of path per particle; other cases are refused before any
+    device work."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pfem2dealii_amd import ParticleHandler, Pfem2GpuError
+    from tests.harness import perturbed_mesh, seed_in_cells
+    _say("adjoint argument checks: non-box mesh, 9 substeps")
+    cfg = W.config("channel2d", scale=0.25)
+    mesh = perturbed_mesh(cfg["mesh"], 0.2, seed=3)
+    x, cells = seed_in_cells(mesh, 4, seed=4)
+    h = ParticleHandler(mesh, cfg["params"])
+    h.initialize_maps()
+    h.insert_particles(x, cells)
+    V = torch.tensor(cfg["V"], dtype=torch.float64, device="cuda")
+    with pytest.raises(Pfem2GpuError, match="box"):
+        h.adjoint(2, 1, V, V)
+    h.close()
+    params = dict(cfg["params"], n_sub=9)
+    h = ParticleHandler(cfg["mesh"], params)
+    h.initialize_maps()
+    h.insert_particles(cfg["x"], cfg["cells"])
+    with pytest.raises(Pfem2GpuError, match="substeps"):
+        h.adjoint(2, 1, V, V)
+    h.close()

@@ -120,7 +120,6 @@ struct pfem2gpu_handle {
   double* qx[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double* qxo[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double* qv[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-  double* qu[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   int* qcell[2] = {nullptr, nullptr};
   int* qs[2] = {nullptr, nullptr};
   int* qsrc[2] = {nullptr, nullptr};
@@ -306,7 +305,6 @@ void free_queues(pfem2gpu_handle* h) {
       dfree(h->qx[b][k]);
       dfree(h->qxo[b][k]);
       dfree(h->qv[b][k]);
-      dfree(h->qu[b][k]);
     }
     dfree(h->qcell[b]);
     dfree(h->qs[b]);
@@ -326,7 +324,6 @@ int grow_queues(pfem2gpu_handle* h, long need) {
       CHK(dalloc(&h->qx[b][k], need));
       CHK(dalloc(&h->qxo[b][k], need));
       CHK(dalloc(&h->qv[b][k], need));
-      CHK(dalloc(&h->qu[b][k], need));
     }
     CHK(dalloc(&h->qcell[b], need));
     CHK(dalloc(&h->qs[b], need));
@@ -342,7 +339,6 @@ DWalk dwalk(pfem2gpu_handle* h, int b) {
     q.x[k] = h->qx[b][k];
     q.xo[k] = h->qxo[b][k];
     q.v[k] = h->qv[b][k];
-    q.u[k] = h->qu[b][k];
   }
   q.cell = h->qcell[b];
   q.s = h->qs[b];

@@ -69,13 +69,13 @@ struct DTmp {
 
 // Walk queue: particles that left their cell in a substep, with the state
 // the neighbour walk needs (position after the substep, reference position
-// before it, the substep's velocity, the substep index) and the carried
-// velocity and source index.
+// before it, the substep index) and the carried velocity and source index.
+// The substep's velocity is not queued: the walk recomputes it bit for bit
+// from the left cell's field at the queued reference position.
 struct DWalk {
   double* x[3];
   double* xo[3];
   double* v[3];
-  double* u[3];
   int* cell;
   int* s;
   int* src;

@@ -233,7 +233,7 @@ constexpr int kQStripes = 1024;
 // index in T.x / T.v instead of a bucket.
 template <int D>
 __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, int src, const double* x,
-                                          const double* xo, const double* v, const double* u, int capc,
+                                          const double* xo, const double* v, int capc,
                                           int* __restrict__ bcount, long spill_base, int* __restrict__ spill_n,
                                           DTmp& T, DWalk& Q, int* __restrict__ qn, long qreg) {
   const int lane = threadIdx.x & 63;
@@ -256,7 +256,6 @@ __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, i
         Q.x[k][q] = x[k];
         Q.xo[k][q] = xo[k];
         Q.v[k][q] = v[k];
-        Q.u[k][q] = u[k];
       }
       Q.cell[q] = cell;
       Q.s[q] = s;
@@ -365,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const
         if (!walk) key = c >> cshift;
       }
     }
-    move_emit<D>(key, walk, cc.c, s, p, x, xi, v, u, capc, bcount, spill_base, spill_n, T, Q, qn, qreg);
+    move_emit<D>(key, walk, cc.c, s, p, x, xi, v, capc, bcount, spill_base, spill_n, T, Q, qn, qreg);
   }
 }
 
@@ -406,7 +405,6 @@ __global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __r
         x[k] = Qin.x[k][i];
         xo[k] = Qin.xo[k][i];
         v[k] = Qin.v[k][i];
-        u[k] = Qin.u[k][i];
       }
       int4 r0 = make_int4(-2, -1, -1, -1), r1 = r0, r2 = r0, r3 = r0;
       if constexpr (D == 2) {
@@ -417,7 +415,8 @@ __global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __r
           r3 = m.cvc[(size_t)c * 4 + 3];
         }
       }
-      load_cell<D>(m, Vc, c, cc, false);
+      load_cell<D>(m, Vc, c, cc, true);
+      gather_cached<D>(Vc, cc, xo, u);  // the failing substep's velocity, bit for bit
       const Prm& pr = m.prm[inst_of(m, c)];
       double xn[D];
       int c2 = -2;
@@ -459,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __r
         if (!walk) key = c2 >> cshift;
       }
     }
-    move_emit<D>(key, walk, cc.c, s, src, x, xi, v, u, capc, bcount, spill_base, spill_n, T, Qout, qn_out, qreg);
+    move_emit<D>(key, walk, cc.c, s, src, x, xi, v, capc, bcount, spill_base, spill_n, T, Qout, qn_out, qreg);
   }
 }
 

@@ -230,10 +230,11 @@ constexpr int kQStripes = 1024;
 // Appends one particle per lane with key >= 0 to its chunk bucket (or the
 // spill region) and queues the lanes with walk set. Wave-uniform. With
 // capc < 0 (replay of a recorded move) the particle is stored at its source
-// index in T.x / T.v instead of a bucket.
+// index in T.x / T.v instead of a bucket. The source index is queued only
+// when the move needs it (keep_src: source maps, paths or a replay).
 template <int D>
-__device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, int src, const double* x,
-                                          const double* xo, const double* v, int capc,
+__device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, int src, bool keep_src,
+                                          const double* x, const double* xo, const double* v, int capc,
                                           int* __restrict__ bcount, long spill_base, int* __restrict__ spill_n,
                                           DTmp& T, DWalk& Q, int* __restrict__ qn, long qreg) {
   const int lane = threadIdx.x & 63;
@@ -259,7 +260,7 @@ __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, i
       }
       Q.cell[q] = cell;
       Q.s[q] = s;
-      Q.src[q] = src;
+      if (keep_src) Q.src[q] = src;
     }
   }
   if (capc < 0) {  // replay: position and velocity stored at the source index
@@ -328,6 +329,7 @@ __global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const
                                                      long spill_base, int* __restrict__ spill_n, DTmp T, DWalk Q,
                                                      int* __restrict__ qn, long qreg, DPath path) {
   const int n = *n_ptr + n_add;
+  const bool keep_src = T.src || capc < 0 || path.c;
   for (long base = (long)blockIdx.x * blockDim.x; base < n; base += (long)gridDim.x * blockDim.x) {
     const int p = (int)(base + threadIdx.x);
     int key = -1, s = 0;
@@ -364,7 +366,7 @@ __global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const
         if (!walk) key = c >> cshift;
       }
     }
-    move_emit<D>(key, walk, cc.c, s, p, x, xi, v, capc, bcount, spill_base, spill_n, T, Q, qn, qreg);
+    move_emit<D>(key, walk, cc.c, s, p, keep_src, x, xi, v, capc, bcount, spill_base, spill_n, T, Q, qn, qreg);
   }
 }
 
@@ -380,6 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __r
   const int stripe = blockIdx.x % kQStripes;
   const int per = gridDim.x / kQStripes;
   const int nq = qn_in[stripe];
+  const bool keep_src = T.src || capc < 0 || path.c;
   for (long base = (long)(blockIdx.x / kQStripes) * blockDim.x; base < nq; base += (long)per * blockDim.x) {
     const int i = (int)(stripe * qreg + base + threadIdx.x);
     const bool in = base + threadIdx.x < nq;
@@ -398,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __r
     if (in) {
       const int c = Qin.cell[i];
       s = Qin.s[i];
-      src = Qin.src[i];
+      if (keep_src) src = Qin.src[i];
       double xo[D];
 #pragma unroll
       for (int k = 0; k < D; ++k) {
@@ -458,7 +461,8 @@ __global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __r
         if (!walk) key = c2 >> cshift;
       }
     }
-    move_emit<D>(key, walk, cc.c, s, src, x, xi, v, capc, bcount, spill_base, spill_n, T, Qout, qn_out, qreg);
+    move_emit<D>(key, walk, cc.c, s, src, keep_src, x, xi, v, capc, bcount, spill_base, spill_n, T, Qout, qn_out,
+                 qreg);
   }
 }
 

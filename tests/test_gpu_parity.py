@@ -61,6 +61,14 @@ def test_abc3d_small(gpu_available):
     _run_pair(W.config("abc3d", scale=1.0 / 8), 3)
 
 
+def test_sixteen_substeps(gpu_available):
+    """n_sub = 16: walks at late substep indices, 16 rounds of walk queues."""
+    _say("channel 16x8, 16 substeps, 2 steps")
+    cfg = W.config("channel2d", scale=0.25)
+    cfg = dict(cfg, params=dict(cfg["params"], n_sub=16))
+    _run_pair(cfg, 2)
+
+
 def test_perturbed_2d_newton(gpu_available):
     _say("perturbed 2D mesh 24x12 (Newton inverse map), 3 steps")
     cfg = W.config("channel2d", scale=0.375)

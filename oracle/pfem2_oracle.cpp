@@ -507,7 +507,7 @@ struct Oracle {
     }
   }
 
-  void project(double* const* Vout) {
+  void project(double* const* Vout, double* Wout = nullptr) {
     std::vector<std::vector<double>> S(dim, std::vector<double>(ndofs, 0.0));
     std::vector<double> W(ndofs, 0.0);
     if (mode == ORC_MODE_REFERENCE) {
@@ -542,6 +542,8 @@ struct Oracle {
     }
     for (int64_t j = 0; j < ndofs; ++j)
       for (int k = 0; k < dim; ++k) Vout[k][j] = S[k][j] / W[j];
+    if (Wout)
+      for (int64_t j = 0; j < ndofs; ++j) Wout[j] = W[j];
   }
 
   void initialize_maps() {
@@ -685,6 +687,13 @@ void orc_project(void* h, double* Vout) {
   Oracle* o = static_cast<Oracle*>(h);
   double* Vp[3] = {Vout, Vout + o->ndofs, o->dim > 2 ? Vout + 2 * o->ndofs : nullptr};
   o->project(Vp);
+}
+
+// project with the nodal weights W_j = sum_p N_j(xi_p) as well ([ndofs])
+void orc_project_weights(void* h, double* Vout, double* W) {
+  Oracle* o = static_cast<Oracle*>(h);
+  double* Vp[3] = {Vout, Vout + o->ndofs, o->dim > 2 ? Vout + 2 * o->ndofs : nullptr};
+  o->project(Vp, W);
 }
 
 int64_t orc_count(void* h) { return (int64_t) static_cast<Oracle*>(h)->P.size(); }

@@ -47,6 +47,7 @@ def _load():
         lib.orc_move.restype = ctypes.c_int
         lib.orc_move.argtypes = [ctypes.c_void_p, _d]
         lib.orc_project.argtypes = [ctypes.c_void_p, _d]
+        lib.orc_project_weights.argtypes = [ctypes.c_void_p, _d, _d]
         lib.orc_count.restype = ctypes.c_int64
         lib.orc_count.argtypes = [ctypes.c_void_p]
         lib.orc_download.argtypes = [ctypes.c_void_p, _d, _d, _d, _i]
@@ -135,6 +136,13 @@ class Oracle:
         out = np.empty((self.dim, self.ndofs), dtype=np.float64)
         self.lib.orc_project(self.h, _dp(out))
         return out
+
+    def project_weights(self):
+        """(projected V, nodal weights W_j = sum_p N_j(xi_p))."""
+        out = np.empty((self.dim, self.ndofs), dtype=np.float64)
+        w = np.empty(self.ndofs, dtype=np.float64)
+        self.lib.orc_project_weights(self.h, _dp(out), _dp(w))
+        return out, w
 
     def count(self):
         return int(self.lib.orc_count(self.h))

@@ -343,6 +343,7 @@ DWalk dwalk(pfem2gpu_handle* h, int b) {
   q.cell = h->qcell[b];
   q.s = h->qs[b];
   q.src = h->qsrc[b];
+  q.err = h->d_err;
   return q;
 }
 
@@ -436,14 +437,21 @@ int capc_for(const pfem2gpu_handle* h, long n) {
   return (int)std::min(c, 1L << 20);
 }
 
+// The failure a set of device error flags stands for (PFEM2GPU_OK if none).
+int device_error(int flags) {
+  if (flags & kErrLookback)
+    return fail(PFEM2GPU_ERR_HIP, "redistribution look-back wait timed out; particle layout is invalid");
+  if (flags & kErrQueue)
+    return fail(PFEM2GPU_ERR_STATE, "a walk-queue stripe overflowed; particles were dropped");
+  return PFEM2GPU_OK;
+}
+
 // Synchronises and reports device-side failures (a look-back wait that
-// timed out would have produced a wrong layout).
+// timed out, or a walk queue that overflowed, leaves a wrong layout).
 int check_device(pfem2gpu_handle* h) {
   HIPCHK(hipMemcpyAsync(h->h_pinned + 1, h->d_err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
-  if (h->h_pinned[1] != 0)
-    return fail(PFEM2GPU_ERR_HIP, "redistribution look-back wait timed out; particle layout is invalid");
-  return PFEM2GPU_OK;
+  return device_error(h->h_pinned[1]);
 }
 
 int read_count(pfem2gpu_handle* h, long* n) {
@@ -467,8 +475,7 @@ int reserve(pfem2gpu_handle* h, long extra, long rebin_extra) {
   if (fits(h->n_ub)) return PFEM2GPU_OK;
   if (h->settle_ev_valid) {
     HIPCHK(hipEventSynchronize(h->settle_ev));
-    if (h->h_pinned[5] != 0)
-      return fail(PFEM2GPU_ERR_HIP, "redistribution look-back wait timed out; particle layout is invalid");
+    CHK(device_error(h->h_pinned[5]));
     h->n_ub = h->h_pinned[4];
     if (fits(h->n_ub)) return PFEM2GPU_OK;
   }
@@ -545,8 +552,12 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
   int rounds = 0;
   if (advect)
     for (const Prm& q : h->prm_inst) rounds = std::max(rounds, q.n_sub);
-  const long gc = move_cell_grid(h, nmax_p);
-  const long qreg = walk_qreg(h, nmax_p);
+  // a captured sequence is replayed at later counts: its grid and stripe size
+  // cover every count the capacity allows (the pre-replay reserve keeps the
+  // count at the start of a step within cap - refill_max)
+  const long nsize = h->capturing ? std::max(nmax_p, h->cap) : nmax_p;
+  const long gc = move_cell_grid(h, nsize);
+  const long qreg = walk_qreg(h, nsize);
   if (qreg * kQStripes > h->qcap) {
     if (h->capturing) return fail(PFEM2GPU_ERR_STATE, "walk queues too small during graph capture");
     CHK(grow_queues(h, qreg * kQStripes));
@@ -1238,6 +1249,8 @@ int pfem2gpu_download(pfem2gpu_handle* h, double* x, double* xi, double* v, int3
       HIPCHK(hipMemcpyAsync(xi + (size_t)k * n, h->tx[k], n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   }
   HIPCHK(hipStreamSynchronize(h->stream));
+  if (cell)  // the stored words also carry the placed flag
+    for (long i = 0; i < n; ++i) cell[i] &= kCellMask;
   return PFEM2GPU_OK;
 }
 
@@ -1310,6 +1323,7 @@ int pfem2gpu_step(pfem2gpu_handle* h, const double* const* V, double* const* V_p
   }
   if (!same) {
     drop_graph(h);
+    CHK(grow_queues(h, walk_qreg(h, h->cap) * kQStripes));  // the captured stripe size (see rebin_or_move)
     const long n_ub0 = h->n_ub;
     h->capturing = true;
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_replay(DMesh m, const double* __
   constexpr int NV = 1 << D;
   if (blockIdx.x == 0 && threadIdx.x == 0) *count = n_end;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n_end; q += gridDim.x * blockDim.x) {
-    const int s = src[q], c = end_cell[q];
+    const int s = src[q], cw = end_cell[q], c = cell_of(cw);
     double x[D], v[D];
     if (s >= 0) {
 #pragma unroll
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_replay(DMesh m, const double* __
       P.x[k][q] = x[k];
       P.v[k][q] = v[k];
     }
-    P.cell[q] = c;
+    P.cell[q] = cw;  // refills carry the placed flag
   }
 }
 
@@ -167,7 +167,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_init(DMesh m, const int* __restr
   double* vt[3] = {vt0, vt1, vt2};
   const int n = *n_ptr;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const int c = Pt.cell[q];
+    const int w = Pt.cell[q];
+    const int c = cell_of(w);
     const Prm& pr = m.prm[inst_of(m, c)];
     double x[D], v[D], xi[D], N[NV], dN[NV][3], X0[D], RH[D];
 #pragma unroll
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_init(DMesh m, const int* __restr
       x[k] = Pt.x[k][q];
       v[k] = Pt.v[k][q];
     }
-    recover_xi_c<D>(m, pr, c, x, xi);
+    recover_xi_c<D>(m, pr, c, placed_of(w), x, xi);
     shape_and_grad<D>(xi, N, dN);
     box_geom<D>(m, c, X0, RH);
     int dof[NV];
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_unmap(DMesh m, int pshift, int n
         vs[k][s] = ve[k][q];
       }
     } else if (q < n_end) {
-      const int c = end_cell[q];
+      const int c = cell_of(end_cell[q]);
       const int inst = inst_of(m, c);
       const Prm& pr = m.prm[inst];
       const int code = m.ptab[(inst << pshift) + (-1 - s)];

@@ -79,7 +79,12 @@ struct DWalk {
   int* cell;
   int* s;
   int* src;
+  int* err;  // device error flags: kErrQueue is set when a stripe would overflow
 };
+
+// Device-side error flags (one int per handle, read back by the host).
+constexpr int kErrLookback = 1;  // a redistribution look-back wait timed out
+constexpr int kErrQueue = 2;     // a walk-queue stripe was too small (particles dropped)
 
 // Optional record of every particle's path through a move: the cell and the
 // reference position at the start of each substep, [s * stride + p] for the
@@ -91,6 +96,19 @@ struct DPath {
 };
 
 __device__ __forceinline__ int inst_of(const DMesh& m, int c) { return m.cell_inst ? m.cell_inst[c] : 0; }
+
+// Stored particle cell word: the cell id, plus kPlaced when the particle's
+// reference position is the exact centre of its sub-cell part because
+// seeding, a refill or an insertion at F(centre) put it there and no locate
+// has run on it since (src/pfem2particlehandler.cpp:88-91,390-394: those
+// store xi = centre, while every locate stores xi = F^-1(x),
+// :420-436,458-478). Cell ids stay below 2^28 (int32 indexing of
+// cell * 2^dim), so bit 30 is free; -1 (no cell) keeps every bit set.
+constexpr int kPlaced = 1 << 30;
+constexpr int kCellMask = kPlaced - 1;
+
+__device__ __forceinline__ int cell_of(int word) { return word & kCellMask; }
+__device__ __forceinline__ bool placed_of(int word) { return (word & kPlaced) != 0; }
 
 template <int D>
 __device__ __forceinline__ void load_ids(const int* __restrict__ base, int c, int* out) {
@@ -307,45 +325,40 @@ __device__ __forceinline__ int part_lin(const Prm& p, const double* xi, bool* va
   return lin;
 }
 
-// Reference position of a located particle from (cell, x): the inverse map,
-// snapped to the exact sub-cell centre when x is bitwise the forward map of
-// that centre (particles placed by seeding or refill keep the exact centre,
-// src/pfem2particlehandler.cpp:88-91,390-394).
-//
-// The forward map is only evaluated when xi is within 1e-9 of the centre:
-// x == F(centre) bitwise implies |F^-1(x) - centre| is a few ulps, so a
-// larger distance already rules the snap out.
+// Exact centre of the sub-cell part containing xi: the stored reference
+// position of a placed particle (kPlaced), whose inverse-mapped position lies
+// within a few ulps of that centre, well inside the part. The centre is
+// (idx + 1/2) * h_k as seeding and refill compute it
+// (src/pfem2particlehandler.cpp:88-91,390-394).
 template <int D>
-__device__ __forceinline__ void snap_centre(const DMesh& m, const Prm& p, const int* vid, const double* x,
-                                            double* xi) {
-  int pi[D];
-  bool valid = true;
+__device__ __forceinline__ void snap_part(const Prm& p, double* xi) {
+#pragma unroll
+  for (int k = 0; k < D; ++k) xi[k] = ((double)part_index(p, k, xi[k]) + 1.0 / 2) * p.h[k];
+}
+
+// Whether x is bitwise the forward map of the centre of the part that holds
+// xi = F^-1(x): an insertion there is a placed particle, like a seeded one.
+// The forward map is only evaluated when xi is within 1e-9 of the centre
+// (x == F(centre) bitwise puts F^-1(x) within a few ulps of it).
+template <int D>
+__device__ __forceinline__ bool at_part_centre(const DMesh& m, const Prm& p, const int* vid, const double* x,
+                                               const double* xi) {
+  bool valid = true, near = true;
   double xic[D];
-  bool near = true;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    pi[k] = part_index(p, k, xi[k]);
-    valid = valid && (pi[k] >= 0) && (pi[k] < p.q[k]);
-    xic[k] = ((double)pi[k] + 1.0 / 2) * p.h[k];
+    const int pk = part_index(p, k, xi[k]);
+    valid = valid && (pk >= 0) && (pk < p.q[k]);
+    xic[k] = ((double)pk + 1.0 / 2) * p.h[k];
     near = near && (fabs(xi[k] - xic[k]) <= 1e-9);
   }
-  if (!valid || !near) return;
+  if (!valid || !near) return false;
   double xc[D];
   fwd<D>(m, vid, xic, xc);
   bool eq = true;
 #pragma unroll
   for (int k = 0; k < D; ++k) eq = eq && (xc[k] == x[k]);
-  if (eq) {
-#pragma unroll
-    for (int k = 0; k < D; ++k) xi[k] = xic[k];
-  }
-}
-
-template <int D>
-__device__ __forceinline__ void recover_xi(const DMesh& m, const Prm& p, int c, const int* vid, const double* x,
-                                           double* xi) {
-  inv_map<D>(m, c, vid, x, xi);
-  snap_centre<D>(m, p, vid, x, xi);
+  return eq;
 }
 
 // Per-lane copy of what the substep loop reads about the particle's current
@@ -438,43 +451,12 @@ __device__ __forceinline__ void fwd_box(const CellCache<D>& cc, const double* xi
     for (int k = 0; k < D; ++k) x[k] = x[k] + N[i] * (((i >> k) & 1) ? cc.X1[k] : cc.X0[k]);
 }
 
-// snap_centre for a cached cell (forward map only when xi is near a centre;
-// box cells evaluate it from their corners without further loads)
+// Stored reference position of a particle in cell c from (c, x) and its
+// placed flag: the inverse map from the geometry table (box cells: no vertex
+// loads at all), or the exact part centre for a placed particle.
 template <int D>
-__device__ __forceinline__ void snap_cached(const DMesh& m, const Prm& p, const CellCache<D>& cc, const double* x,
-                                            double* xi) {
-  int pi[D];
-  bool valid = true, near = true;
-  double xic[D];
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    pi[k] = part_index(p, k, xi[k]);
-    valid = valid && (pi[k] >= 0) && (pi[k] < p.q[k]);
-    xic[k] = ((double)pi[k] + 1.0 / 2) * p.h[k];
-    near = near && (fabs(xi[k] - xic[k]) <= 1e-9);
-  }
-  if (!valid || !near) return;
-  double xc[D];
-  if (cc.box) {
-    fwd_box<D>(cc, xic, xc);
-  } else {
-    int vid[1 << D];
-    load_ids<D>(m.cv, cc.c, vid);
-    fwd<D>(m, vid, xic, xc);
-  }
-  bool eq = true;
-#pragma unroll
-  for (int k = 0; k < D; ++k) eq = eq && (xc[k] == x[k]);
-  if (eq) {
-#pragma unroll
-    for (int k = 0; k < D; ++k) xi[k] = xic[k];
-  }
-}
-
-// Reference position of a located particle in cell c from the geometry table
-// (box cells: no vertex loads at all).
-template <int D>
-__device__ __forceinline__ void recover_xi_c(const DMesh& m, const Prm& p, int c, const double* x, double* xi) {
+__device__ __forceinline__ void recover_xi_c(const DMesh& m, const Prm& p, int c, bool placed, const double* x,
+                                             double* xi) {
   CellCache<D> cc;
   cc.c = c;
   const double* g = m.geom + (size_t)c * 3 * D;
@@ -486,7 +468,7 @@ __device__ __forceinline__ void recover_xi_c(const DMesh& m, const Prm& p, int c
   }
   cc.box = (cc.X0[0] == cc.X0[0]);
   inv_cached<D>(m, cc, x, xi);
-  snap_cached<D>(m, p, cc, x, xi);
+  if (placed) snap_part<D>(p, xi);
 }
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {

@@ -184,7 +184,7 @@ __global__ void k_spill_count(const int* __restrict__ spill_n, long spill_base, 
                               int cshift, int* __restrict__ scount) {
   const int n = *spill_n;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    atomicAdd(&scount[tcell[spill_base + i] >> cshift], 1);
+    atomicAdd(&scount[cell_of(tcell[spill_base + i]) >> cshift], 1);
 }
 
 template <int D>
@@ -193,8 +193,8 @@ __global__ void k_spill_scatter(const int* __restrict__ spill_n, long spill_base
   const int n = *spill_n;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const long src = spill_base + i;
-    const int c = T.cell[src];
-    const int ch = c >> cshift;
+    const int c = T.cell[src];  // cell word (placed flag kept)
+    const int ch = cell_of(c) >> cshift;
     const long dst = spill2_base + sstart[ch] + atomicAdd(&sfill[ch], 1);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -233,7 +233,7 @@ constexpr int kQStripes = 1024;
 // index in T.x / T.v instead of a bucket. The source index is queued only
 // when the move needs it (keep_src: source maps, paths or a replay).
 template <int D>
-__device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, int src, bool keep_src,
+__device__ __forceinline__ void move_emit(int key, bool walk, int cell, int tflag, int s, int src, bool keep_src,
                                           const double* x, const double* xo, const double* v, int capc,
                                           int* __restrict__ bcount, long spill_base, int* __restrict__ spill_n,
                                           DTmp& T, DWalk& Q, int* __restrict__ qn, long qreg) {
@@ -250,7 +250,9 @@ __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, i
   if (capc >= 0) slot = wave_group_add(bcount, key);
   if (wm) {
     qb = __shfl(qb, qleader, 64);
-    if (walk) {
+    if (qb + __popcll(wm) > qreg) {  // the stripe is full: report, never write past it
+      if (lane == qleader) atomicOr(Q.err, kErrQueue);
+    } else if (walk) {
       const long q = (long)stripe * qreg + qb + __popcll(wm & below);
 #pragma unroll
       for (int k = 0; k < D; ++k) {
@@ -281,7 +283,7 @@ __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int s, i
       T.x[k][dst] = x[k];
       T.v[k][dst] = v[k];
     }
-    T.cell[dst] = cell;
+    T.cell[dst] = cell | tflag;
     if (T.src) T.src[dst] = src;
   }
 }
@@ -332,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const
   const bool keep_src = T.src || capc < 0 || path.c;
   for (long base = (long)blockIdx.x * blockDim.x; base < n; base += (long)gridDim.x * blockDim.x) {
     const int p = (int)(base + threadIdx.x);
-    int key = -1, s = 0;
+    int key = -1, s = 0, tflag = 0;
     bool walk = false;
     CellCache<D> cc;
     cc.c = -1;
@@ -345,8 +347,9 @@ __global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const
       u[k] = 0.0;
     }
     if (p < n) {
-      const int c = P.cell[p];
-      if (c >= 0) {
+      const int w = P.cell[p];
+      if (w >= 0) {
+        const int c = cell_of(w);
         load_cell<D>(m, Vc, c, cc, advect != 0);
         const Prm& pr = m.prm[inst_of(m, c)];
 #pragma unroll
@@ -355,7 +358,10 @@ __global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const
           v[k] = P.v[k][p];
         }
         inv_cached<D>(m, cc, x, xi);
-        snap_cached<D>(m, pr, cc, x, xi);
+        if (placed_of(w)) snap_part<D>(pr, xi);
+        // a move locates every particle at least once (n_sub >= 1); a
+        // re-binning keeps the flag
+        if (!advect) tflag = w & kPlaced;
         if (use_dv) {
           double corr[D];
           gather_table<D>(dVc, c, xi, corr);
@@ -366,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const
         if (!walk) key = c >> cshift;
       }
     }
-    move_emit<D>(key, walk, cc.c, s, p, keep_src, x, xi, v, capc, bcount, spill_base, spill_n, T, Q, qn, qreg);
+    move_emit<D>(key, walk, cc.c, tflag, s, p, keep_src, x, xi, v, capc, bcount, spill_base, spill_n, T, Q, qn, qreg);
   }
 }
 
@@ -461,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __r
         if (!walk) key = c2 >> cshift;
       }
     }
-    move_emit<D>(key, walk, cc.c, s, src, keep_src, x, xi, v, capc, bcount, spill_base, spill_n, T, Qout, qn_out,
+    move_emit<D>(key, walk, cc.c, 0, s, src, keep_src, x, xi, v, capc, bcount, spill_base, spill_n, T, Qout, qn_out,
                  qreg);
   }
 }
@@ -586,7 +592,7 @@ __device__ __forceinline__ unsigned lb_prefix(unsigned long long* lb, int ch, in
     const unsigned st = (unsigned)(w >> 32);
     if (st == 0) {
       if (++spins > (1L << 26)) {
-        atomicOr(err, 1);
+        atomicOr(err, kErrLookback);
         break;
       }
       __builtin_amdgcn_s_sleep(1);
@@ -657,7 +663,7 @@ __device__ __forceinline__ unsigned lb_prefix_wave(unsigned long long* lb, int c
     const unsigned long long need = first2 >= 63 ? ~0ULL : ((1ULL << (first2 + 1)) - 1ULL);
     if (m0 & need) {
       if (++spins > (1L << 26)) {
-        if (lane == 0) atomicOr(err, 1);
+        if (lane == 0) atomicOr(err, kErrLookback);
         break;
       }
       __builtin_amdgcn_s_sleep(1);
@@ -673,11 +679,11 @@ __device__ __forceinline__ unsigned lb_prefix_wave(unsigned long long* lb, int c
   return prefix;
 }
 
-// Reference position of x in chunk cell c from the LDS geometry (box) or
-// the generic path, snapped as in recover_xi_c.
+// Stored reference position of x in chunk cell c from the LDS geometry (box)
+// or the generic path, the part centre for a placed particle (recover_xi_c).
 template <int D>
-__device__ __forceinline__ void settle_xi(const DMesh& m, const Prm& pr, int c, const double* __restrict__ geo,
-                                          const double* x, double* xi) {
+__device__ __forceinline__ void settle_xi(const DMesh& m, const Prm& pr, int c, bool placed,
+                                          const double* __restrict__ geo, const double* x, double* xi) {
   CellCache<D> cc;
   cc.c = c;
 #pragma unroll
@@ -688,7 +694,7 @@ __device__ __forceinline__ void settle_xi(const DMesh& m, const Prm& pr, int c, 
   }
   cc.box = (cc.X0[0] == cc.X0[0]);
   inv_cached<D>(m, cc, x, xi);
-  snap_cached<D>(m, pr, cc, x, xi);
+  if (placed) snap_part<D>(pr, xi);
 }
 
 // Particles per thread a staged chunk holds in registers between the
@@ -701,6 +707,9 @@ struct SettlePre {
 // Settle workgroup: four wavefronts per chunk of 2^kSettleShift cells.
 constexpr int kSettleBlock = 256;
 constexpr int kSettleShift = 4;
+// the final-order cell byte holds the chunk-local cell in bits 0-6 and the
+// placed flag in bit 7
+static_assert(kSettleShift <= 7, "chunk-local cell ids must fit seven bits");
 
 // Cell data of a chunk (box geometry, advecting field, instance and its
 // parameters), carried in registers from its loads to its stores into an LDS
@@ -888,12 +897,12 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       const int j = threadIdx.x + u * blockDim.x;
       ppl[u] = 0;
       if (j < n) {
-        const int lc = pc[u] - c0;
+        const int lc = cell_of(pc[u]) - c0;
         const Prm& pr = s_prm[lc];
         double xi[D];
-        settle_xi<D>(m, pr, pc[u], s_geo + lc * 3 * D, px[u], xi);
+        settle_xi<D>(m, pr, cell_of(pc[u]), placed_of(pc[u]), s_geo + lc * 3 * D, px[u], xi);
         const int part = part_lin<D>(pr, xi, nullptr);
-        ppl[u] = (lc << 16) | part;
+        ppl[u] = (lc << 16) | (placed_of(pc[u]) ? 0x8000 : 0) | part;  // part < 2^15 (LDS budget)
         atomicAdd(&s_cnt[(lc << pshift) + part], 1);
         atomicAdd(&s_ccnt[lc], 1);
         if (staged_in) {
@@ -914,13 +923,14 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     }
     for (int j = threadIdx.x + PRE * blockDim.x; j < n; j += blockDim.x) {  // beyond the registers: not staged
       const long i = chunk_index(j, nb, bucket0, spill0);
-      const int c = T.cell[i];
+      const int cw = T.cell[i];
+      const int c = cell_of(cw);
       const int lc = c - c0;
       const Prm& pr = s_prm[lc];
       double x[D], xi[D];
 #pragma unroll
       for (int k = 0; k < D; ++k) x[k] = T.x[k][i];
-      settle_xi<D>(m, pr, c, s_geo + lc * 3 * D, x, xi);
+      settle_xi<D>(m, pr, c, placed_of(cw), s_geo + lc * 3 * D, x, xi);
       const int part = part_lin<D>(pr, xi, nullptr);
       atomicAdd(&s_cnt[(lc << pshift) + part], 1);
       atomicAdd(&s_ccnt[lc], 1);
@@ -981,19 +991,20 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       for (int u = 0; u < PRE; ++u) {
         const int j = threadIdx.x + u * blockDim.x;
         if (j < n) {
-          const int t = ((ppl[u] >> 16) << pshift) + (ppl[u] & 0xffff);
+          const int t = ((ppl[u] >> 16) << pshift) + (ppl[u] & 0x7fff);
           s_ord[atomicAdd(&s_roff[t], 1)] = j;
         }
       }
     } else {
       for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const long i = chunk_index(j, nb, bucket0, spill0);
-        const int c = T.cell[i];
+        const int cw = T.cell[i];
+        const int c = cell_of(cw);
         const int lc = c - c0;
         double x[D], xi[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) x[k] = T.x[k][i];
-        settle_xi<D>(m, s_prm[lc], c, s_geo + lc * 3 * D, x, xi);
+        settle_xi<D>(m, s_prm[lc], c, placed_of(cw), s_geo + lc * 3 * D, x, xi);
         const int t = (lc << pshift) + part_lin<D>(s_prm[lc], xi, nullptr);
         gord[chunk_index(atomicAdd(&s_roff[t], 1), nb, bucket0, spill0)] = j;
       }
@@ -1008,7 +1019,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         pf[u] = -1;
         if (j < n) {
           const int lc = ppl[u] >> 16;
-          const int t = (lc << pshift) + (ppl[u] & 0xffff);
+          const int t = (lc << pshift) + (ppl[u] & 0x7fff);
           const unsigned long long key = s_key[j];
           const int b = s_roff[t], a = b - s_cnt[t];
           int r = 0;
@@ -1038,7 +1049,8 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     } else {
       for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const long i = chunk_index(j, nb, bucket0, spill0);
-        const int c = T.cell[i];
+        const int cw = T.cell[i];
+        const int c = cell_of(cw);
         const int lc = c - c0;
         double xj[D], vj[D], xi[D];
 #pragma unroll
@@ -1046,7 +1058,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
           xj[k] = T.x[k][i];
           vj[k] = T.v[k][i];
         }
-        settle_xi<D>(m, s_prm[lc], c, s_geo + lc * 3 * D, xj, xi);
+        settle_xi<D>(m, s_prm[lc], c, placed_of(cw), s_geo + lc * 3 * D, xj, xi);
         const int t = (lc << pshift) + part_lin<D>(s_prm[lc], xi, nullptr);
         const unsigned long long key = particle_key<D>(xj);
         const int b = s_roff[t], a = b - s_cnt[t];
@@ -1071,7 +1083,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
           P.x[k][fpos] = xj[k];
           P.v[k][fpos] = vj[k];
         }
-        P.cell[fpos] = c;
+        P.cell[fpos] = cw;
         if (P.src) P.src[fpos] = T.src[i];
       }
     }
@@ -1100,7 +1112,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
             s_fv[k * nslots + f] = qv[u][k];
             s_fxi[k * nslots + f] = qxi[u][k];
           }
-          s_fcell[f] = (unsigned char)(ppl[u] >> 16);
+          s_fcell[f] = (unsigned char)((ppl[u] >> 16) | ((ppl[u] & 0x8000) ? 0x80 : 0));
           if (P.src) s_fsrc[f] = T.src[chunk_index(threadIdx.x + u * blockDim.x, nb, bucket0, spill0)];
         }
       }
@@ -1150,14 +1162,14 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
             s_fxi[k * nslots + f] = xi[k];
           }
           s_fsrc[f] = -1 - pp;
-          s_fcell[f] = (unsigned char)lc;
+          s_fcell[f] = (unsigned char)(lc | 0x80);  // placed at the part centre
         } else {
 #pragma unroll
           for (int k = 0; k < D; ++k) {
             P.x[k][o0n + f] = x[k];
             P.v[k][o0n + f] = v[k];
           }
-          P.cell[o0n + f] = c;
+          P.cell[o0n + f] = c | kPlaced;
           if (P.src) P.src[o0n + f] = -1 - pp;
         }
       }
@@ -1240,7 +1252,8 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
           P.x[k][o0 + t] = s_fx[k * nslots + t];
           P.v[k][o0 + t] = s_fv[k * nslots + t];
         }
-        P.cell[o0 + t] = c0 + s_fcell[t];
+        const int fc = s_fcell[t];
+        P.cell[o0 + t] = (c0 + (fc & 0x7f)) | ((fc & 0x80) ? kPlaced : 0);
         if (P.src) P.src[o0 + t] = s_fsrc[t];
       }
     }
@@ -1370,7 +1383,7 @@ __device__ __forceinline__ void contrib_one(const DMesh& m, int c, int i, const 
     double x[D], xi[D], N[NV];
 #pragma unroll
     for (int k = 0; k < D; ++k) x[k] = P.x[k][p];
-    recover_xi_c<D>(m, pr, c, x, xi);
+    recover_xi_c<D>(m, pr, c, placed_of(P.cell[p]), x, xi);
     shape<D>(xi, N);
     double Ni = N[0];
 #pragma unroll
@@ -1451,12 +1464,13 @@ __global__ __launch_bounds__(kBlock) void k_correct(DMesh m, const int* __restri
                                                    const double* __restrict__ dVc) {
   const int n = *n_ptr;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-    const int c = P.cell[p];
+    const int w = P.cell[p];
+    const int c = cell_of(w);
     const Prm pr = m.prm[inst_of(m, c)];
     double x[D], xi[D], corr[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) x[k] = P.x[k][p];
-    recover_xi_c<D>(m, pr, c, x, xi);
+    recover_xi_c<D>(m, pr, c, placed_of(w), x, xi);
     gather_table<D>(dVc, c, xi, corr);
 #pragma unroll
     for (int k = 0; k < D; ++k) P.v[k][p] = P.v[k][p] + corr[k];
@@ -1469,15 +1483,15 @@ __global__ __launch_bounds__(kBlock) void k_setvel(DMesh m, const int* __restric
   constexpr int NV = 1 << D;
   const int n = *n_ptr;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-    const int c = P.cell[p];
+    const int w = P.cell[p];
+    const int c = cell_of(w);
     const Prm pr = m.prm[inst_of(m, c)];
     double x[D], xi[D], u[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) x[k] = P.x[k][p];
-    int vid[NV], dof[NV];
-    load_ids<D>(m.cv, c, vid);
+    int dof[NV];
     load_ids<D>(m.cd, c, dof);
-    recover_xi_c<D>(m, pr, c, x, xi);
+    recover_xi_c<D>(m, pr, c, placed_of(w), x, xi);
     gather<D>(V, dof, xi, u);
 #pragma unroll
     for (int k = 0; k < D; ++k) P.v[k][p] = u[k];
@@ -1492,14 +1506,13 @@ __global__ __launch_bounds__(kBlock) void k_xi(DMesh m, const int* __restrict__ 
   double* out[3] = {xi0, xi1, xi2};
   const int n = *n_ptr;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-    const int c = P.cell[p];
+    const int w = P.cell[p];
+    const int c = cell_of(w);
     const Prm pr = m.prm[inst_of(m, c)];
     double x[D], xi[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) x[k] = P.x[k][p];
-    int vid[NV];
-    load_ids<D>(m.cv, c, vid);
-    recover_xi_c<D>(m, pr, c, x, xi);
+    recover_xi_c<D>(m, pr, c, placed_of(w), x, xi);
 #pragma unroll
     for (int k = 0; k < D; ++k) out[k][p] = xi[k];
   }
@@ -1527,6 +1540,7 @@ __global__ __launch_bounds__(kBlock) void k_append(DMesh m, int nc, const int* _
       int vid[NV];
       load_ids<D>(m.cv, c, vid);
       ok = inv_map<D>(m, c, vid, x, xi) && inside<D>(xi);
+      if (ok && at_part_centre<D>(m, m.prm[inst_of(m, c)], vid, x, xi)) c |= kPlaced;
     }
     if (!ok) {
       atomicAdd(rejected, 1ULL);

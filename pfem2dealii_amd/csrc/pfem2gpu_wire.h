@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kWireTile) void k_wire_pack(DMesh m, int L, const i
         double x[D], xi[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) x[k] = P.x[k][p];
-        recover_xi_c<D>(m, m.prm[inst_of(m, c)], c, x, xi);
+        recover_xi_c<D>(m, m.prm[inst_of(m, c)], c, placed_of(P.cell[p]), x, xi);
         unsigned long long* w = s_rec + threadIdx.x * W;
         w[0] = wire_pair((unsigned int)p, (unsigned int)c);
 #pragma unroll
@@ -174,7 +174,8 @@ __global__ __launch_bounds__(kBlock) void k_wire_compact(int nc, const int* __re
                                                         const unsigned char* __restrict__ flag, DParts P, DTmp T) {
   const int n = cs[nc];
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-    const int c = P.cell[p];
+    const int w = P.cell[p];
+    const int c = cell_of(w);
     if (flag[c]) continue;
     const int q = ncs[c] + (p - cs[c]);
 #pragma unroll
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_wire_compact(int nc, const int* __re
       T.x[k][q] = P.x[k][p];
       T.v[k][q] = P.v[k][p];
     }
-    T.cell[q] = c;
+    T.cell[q] = w;
   }
 }
 

@@ -96,6 +96,51 @@ def test_graph_step_matches_eager(gpu_available):
         assert hs[1].count() == hs[0].count()
 
 
+def test_graph_replays_while_the_count_grows(gpu_available):
+    """Particles start in the first column of cells of a 64x32 channel and
+    flow right at up to one cell per step (CFL 1), so every step fills a new
+    column (the count grows by ~500 per step) and most particles cross a face
+    in some substep. The graph is captured right after an exact count was
+    read (~1000 particles) and then replayed seven times with no host reads
+    between the replays; the capacity is reserved so that no replay moves a
+    buffer. The captured launch shape and walk-queue stripes must hold every
+    count the capacity allows: the replays must equal eager steps."""
+    _say("hipGraph replays with a growing count, channel 64x32 CFL 1, 8 steps")
+    import torch
+    from pfem2dealii_amd import ParticleHandler
+    mesh = W.structured_mesh([64, 32], [0.0, 0.0], [2.0, 1.0])
+    V = W.poiseuille(mesh)
+    params = dict(n_sub=5, dt=W.cfl_dt(mesh, V, cfl=1.0), max_per_part=5, particles_per_cell=[4, 4])
+    x, cells = W.seed_jittered_lattice(mesh, [4, 4], 0.1, seed=41)
+    sel = cells % 64 == 0
+    x, cells = x[:, sel], cells[sel]
+    hs = []
+    for _ in range(2):
+        h = ParticleHandler(mesh, params)
+        h.initialize_maps()
+        assert h.insert_particles(x, cells) == 0
+        h.set_velocity_from_field(V)
+        h.reserve_particles(6000)
+        hs.append(h)
+    Vd = torch.tensor(V, dtype=torch.float64, device="cuda")
+    outs = [torch.empty_like(Vd), torch.empty_like(Vd)]
+    torch.cuda.synchronize()
+    counts = []
+    for s in range(8):
+        hs[0].step(Vd, outs[0], use_graph=False)
+        hs[1].step(Vd, outs[1], use_graph=True)
+        if s == 1:  # what the capture saw
+            counts.append(hs[0].count())
+    hs[0].synchronize()
+    hs[1].synchronize()
+    counts.append(hs[0].count())
+    assert counts[-1] > 3 * counts[0], "the count did not grow: %s" % counts
+    assert _same(outs[0].cpu().numpy(), outs[1].cpu().numpy())
+    da, db = hs[0].download(), hs[1].download()
+    assert np.array_equal(da["cell"], db["cell"]) and np.array_equal(da["x"], db["x"])
+    assert np.array_equal(da["v"], db["v"])
+
+
 def test_tuning_does_not_change_results(gpu_available):
     _say("launch tunings give bitwise identical steps")
     from pfem2dealii_amd import ParticleHandler

@@ -155,6 +155,56 @@ def test_seed_lattice(gpu_available):
             assert_particles_match(h.download(), o.download())
 
 
+def _placed_meshes():
+    """Box meshes whose part centres do not map back exactly (non-dyadic
+    spacing and origin) and a perturbed (Newton) mesh."""
+    a = W.structured_mesh([10, 7], [0.1, 0.0], [1.3, 0.7])
+    b = W.structured_mesh([12, 6], [0.0, 0.0], [2.2, 0.41])
+    c = perturbed_mesh(W.structured_mesh([10, 7], [0.1, 0.0], [1.3, 0.7]), 0.2, seed=31)
+    return {"box_10x7": a, "box_12x6": b, "perturbed_10x7": c}
+
+
+@pytest.mark.parametrize("name", ["box_10x7", "box_12x6", "perturbed_10x7"])
+def test_seeded_particles_relocated_by_a_step_at_rest(gpu_available, name):
+    """The reference's cases start from rest: the lattice is seeded at part
+    centres (xi = centre exactly), the first move under a zero field leaves
+    every position unchanged, and its locate stores xi = F^-1(x), which can
+    differ from the centre by an ulp; the next step's gathers and corrections
+    use that xi. Positions, reference positions and velocities are compared
+    bitwise with the oracle after seeding, after the step at rest and after
+    two steps of flow."""
+    _say("seed -> step at rest -> 2 flow steps on %s" % name)
+    from oracle import Oracle
+    from pfem2dealii_amd import ParticleHandler
+    mesh = _placed_meshes()[name]
+    V = W.poiseuille(mesh)
+    V0 = np.zeros_like(V)
+    params = dict(n_sub=5, dt=W.cfl_dt(mesh, V), max_per_part=5, particles_per_cell=[2, 2])
+    o = Oracle(mesh, params)
+    o.initialize_maps()
+    o.seed_lattice(V0)
+    h = ParticleHandler(mesh, params)
+    h.initialize_maps()
+    h.seed_particles(V0)
+    assert_particles_match(h.download(), o.download())
+    prev = V0
+    for s, F in enumerate((V0, V, V)):
+        o.correct(F, prev)
+        h.correct_particle_velocity(F, prev)
+        o.move(F)
+        h.move_particles(F)
+        Po = o.project()
+        Pg = h.project_particle_fields()
+        assert_fields_match(Pg, Po)
+        d_o = o.download()
+        assert_particles_match(h.download(), d_o)
+        if s == 0:  # the case is live: some relocated xi are not the centres
+            q = np.array(params["particles_per_cell"], dtype=float)[:, None]
+            centre = (np.floor(d_o["xi"] * q) + 0.5) / q
+            assert np.any(d_o["xi"] != centre), "no xi moved off its centre: the case tests nothing"
+        prev = Po
+
+
 def test_insert_rejects_outside(gpu_available):
     _say("insert with particles outside their cell")
     cfg = W.config("channel2d", scale=0.25)

@@ -174,17 +174,24 @@ def test_constant_field_advection_exact_sum():
 
 
 def test_bilinear_field_interpolated_exactly():
+    """Q1 interpolation is isoparametric: on any Q1 cell, perturbed or not, a
+    nodal field affine in the physical coordinates, V_i = a + B X_i, is
+    reproduced at x = F(xi) (sum_i N_i (a + B X_i) = a + B F(xi)); on a box
+    mesh a field bilinear in x and y is reproduced too."""
     m = perturbed_mesh(_mesh2(4, 4, hi=(1.0, 1.0)), 0.2, seed=7)
     X = m["X"]
     V = np.stack([1.0 + 2.0 * X[0] - 0.5 * X[1], -0.5 + 0.25 * X[0] + 3.0 * X[1]])
     o = Oracle(m, _params(2))
     o.initialize_maps()
     x, cells = seed_in_cells(m, 4, seed=8)
-    o.insert(x, cells)
+    assert not all(o.is_box(int(c)) for c in np.unique(cells))  # non-affine cells present
+    assert o.insert(x, cells) == 0
     o.set_velocity_from_field(V)
     d = o.download()
-    # on a box mesh a field linear in x is reproduced; here vertices move, so
-    # only affine fields in xi are exact: check against the cell's own map
+    xx, yy = d["x"]
+    ex = np.stack([1.0 + 2.0 * xx - 0.5 * yy, -0.5 + 0.25 * xx + 3.0 * yy])
+    # Newton stops at |dxi| <= 1e-12, so the located xi maps to x within ~1e-12
+    assert np.abs(d["v"] - ex).max() < 5e-11
     m2 = _mesh2(4, 4, hi=(1.0, 1.0))
     X2 = m2["X"]
     V2 = np.stack([1.0 + 2.0 * X2[0] - 0.5 * X2[1] + 0.75 * X2[0] * X2[1], 0.5 * X2[0] * X2[1]])
@@ -196,7 +203,6 @@ def test_bilinear_field_interpolated_exactly():
     xx, yy = d2["x"]
     ex = np.stack([1.0 + 2.0 * xx - 0.5 * yy + 0.75 * xx * yy, 0.5 * xx * yy])
     assert np.abs(d2["v"] - ex).max() < 1e-14
-    assert d["v"].shape == x.shape
 
 
 def test_correction_zero_and_constant_increment():
@@ -386,18 +392,46 @@ def test_projection_linear_field_on_symmetric_lattice():
 
 
 def test_projection_weights_sum_to_particle_count():
-    """sum_j W_j = N by partition of unity: project v = 1 per particle, the
-    weights are recovered as S/V with V == 1, and the canonical and reference
-    orders agree to rounding."""
+    """sum_j W_j = N by partition of unity (a dropped vertex or a wrong DoF
+    loses weight); projecting v = 1 gives exactly 1 at every touched node;
+    the canonical and reference orders agree to rounding."""
     cfg = W.config("taylor_green2d", scale=1.0 / 32)
     outs = []
+    n = len(cfg["cells"])
     for mode in (MODE_CANONICAL, MODE_REFERENCE):
         o = Oracle(cfg["mesh"], cfg["params"], mode)
         o.initialize_maps()
-        o.insert(cfg["x"], cfg["cells"], np.stack([np.ones(len(cfg["cells"])), np.zeros(len(cfg["cells"]))]))
-        Vp = o.project()
-        assert np.array_equal(Vp[0], np.ones_like(Vp[0]))
+        assert o.insert(cfg["x"], cfg["cells"], np.stack([np.ones(n), np.zeros(n)])) == 0
+        Vp, Wn = o.project_weights()
+        assert abs(Wn.sum() - n) <= 1e-12 * n
+        assert (Wn >= 0).all() and (Wn > 0).sum() > 0.9 * Wn.shape[0]
+        assert np.array_equal(Vp[0][Wn > 0], np.ones(int((Wn > 0).sum())))
         o.set_velocity_from_field(cfg["V"])
         outs.append(o.project())
     fin = ~np.isnan(outs[1])
     assert np.abs(outs[0][fin] - outs[1][fin]).max() <= 1e-13 * np.abs(outs[1][fin]).max()
+
+
+def test_step_at_rest_relocates_seeded_particles():
+    """Seeding stores xi = the exact part centre; a move under a zero field
+    leaves x = F(centre) unchanged and its locate stores xi = F^-1(x) (the
+    reference's current-cell test, src/pfem2particlehandler.cpp:420-436).
+    On a mesh whose spacing is not dyadic that differs from the centre for
+    some particles, and always equals the oracle's own inverse map."""
+    m = W.structured_mesh([10, 7], [0.1, 0.0], [1.3, 0.7])
+    p = _params(2, q=2)
+    o = Oracle(m, p)
+    o.initialize_maps()
+    V0 = np.zeros_like(m["X"])
+    o.seed_lattice(V0)
+    d0 = o.download()
+    q = 2.0
+    centre = (np.floor(d0["xi"] * q) + 0.5) / q
+    assert np.array_equal(d0["xi"], centre)
+    o.move(V0)
+    d1 = o.download()
+    assert np.array_equal(d1["x"], d0["x"]) and np.array_equal(d1["cell"], d0["cell"])
+    for i in range(d1["cell"].shape[0]):
+        code, xi = o.inv(int(d1["cell"][i]), d1["x"][:, i])
+        assert code == 1 and np.array_equal(xi, d1["xi"][:, i])
+    assert (d1["xi"] != d0["xi"]).any(axis=0).sum() > 0

@@ -190,6 +190,22 @@ int pfem2gpu_pack_particles(pfem2gpu_handle* h, int32_t n_dest, const int32_t* d
 int pfem2gpu_unpack_particles(pfem2gpu_handle* h, const void* data, int64_t n_bytes, int where,
                               int64_t* n_rejected);
 
+/* pfem2gpu_pack_particles with device output that allocates nothing (no
+ * counterpart in the reference): for callers that cannot allocate, such as
+ * code under HIP graph capture, tight per-substep exchange loops and
+ * harnesses that forbid hipMalloc. Same arguments, results and errors as
+ * pfem2gpu_pack_particles with where = PFEM2GPU_DEVICE (`out` is device
+ * memory, 8-byte aligned). Its per-call buffers live in a fixed 32 MiB array
+ * of the library's device memory, one per GPU: a call needing more (about
+ * 4 bytes per destination and 12 per listed cell, plus 0.25 per mesh cell and
+ * with remove != 0 another 8 per mesh cell) fails with PFEM2GPU_ERR_ARG
+ * before any device work. The array is shared by every call on one GPU and
+ * keeps the last call's contents: calls that overlap on one GPU (from several
+ * host threads or streams) race. With timing off (pfem2gpu_set_timing) it
+ * creates no events either. Synchronises. */
+int pfem2gpu_pack_particles_noalloc(pfem2gpu_handle* h, int32_t n_dest, const int32_t* dest_offsets,
+                                    const int32_t* cells, void* out, int64_t capacity, int64_t* bytes, int remove);
+
 /* ---- Batched form (no counterpart in the reference) ----
  * Many independent flow instances in one handle: meshes[i] and params[i]
  * describe instance i; their cells, vertices and DoFs are concatenated in

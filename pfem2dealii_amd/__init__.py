@@ -60,7 +60,7 @@ EXPORTS = [
     "pfem2gpu_adjoint", "pfem2gpu_move_particles_traced",
     "pfem2gpu_output_particle_solution", "pfem2gpu_write_particles_vtu", "pfem2gpu_write_particles_pvtu",
     "pfem2gpu_vertex_to_cells", "pfem2gpu_particle_record_size", "pfem2gpu_pack_particles",
-    "pfem2gpu_unpack_particles",
+    "pfem2gpu_unpack_particles", "pfem2gpu_pack_particles_noalloc",
 ]
 
 
@@ -115,6 +115,8 @@ def load_library(path=None):
     lib.pfem2gpu_pack_particles.argtypes = [_vp, ctypes.c_int32, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int, _vp,
                                             ctypes.c_int]
     lib.pfem2gpu_unpack_particles.argtypes = [_vp, _vp, ctypes.c_int64, ctypes.c_int, _i64p]
+    lib.pfem2gpu_pack_particles_noalloc.argtypes = [_vp, ctypes.c_int32, _vp, _vp, _vp, ctypes.c_int64, _vp,
+                                                    ctypes.c_int]
     if path is None:
         _lib = lib
     return lib

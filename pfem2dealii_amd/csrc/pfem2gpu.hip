@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -108,7 +109,9 @@ struct pfem2gpu_handle {
   bool pending_corr = false;
   double* stageF[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   unsigned long long* d_rejected = nullptr;
-  int* h_pinned = nullptr;  // [0..1] synchronous reads, [2..3] adjoint loss, [4..5] count and error after a settle
+  int* h_pinned = nullptr;  // [0..1] synchronous reads, [2..3] adjoint loss, [4..5] count and error after a
+                            // settle, [6..9] the no-allocation pack's duplicate flag and total
+  int* scratch = nullptr;   // this device's no-allocation scratch (shared by every handle on the device)
   hipEvent_t settle_ev = nullptr;  // recorded after [4..5] are copied, following each eager settle
   bool settle_ev_valid = false;    // [4] is the current count once settle_ev completes
   long n_ub = 0;  // host-side upper bound of the particle count
@@ -624,6 +627,40 @@ int append_staged(pfem2gpu_handle* h, long n, const double* const* sx, const dou
 
 }  // namespace
 
+// ------------------------------------------------------ no-allocation scratch
+// Device scratch of pfem2gpu_pack_particles_noalloc: a fixed array in the
+// library's static device memory, one copy per GPU, held until the process
+// ends and shared by every call on that GPU (the last call's contents stay;
+// calls that overlap on one GPU race). Host code never passes the array's
+// name to a kernel or a copy (on the host it only stands for the symbol): it
+// uses the per-device address from hipGetSymbolAddress, taken once per device
+// when the first handle is created there with that device current.
+constexpr long kNoAllocScratchInts = 1L << 23;  // 32 MiB
+static __device__ __align__(256) int scratch[kNoAllocScratchInts];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's scratch address (looked up on its first use).
+int scratch_for_current_device(int** out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(PFEM2GPU_ERR_HIP, "device ordinal out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<int*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PFEM2GPU_OK;
+}
+
+}  // namespace
+
 // =================================================================== C ABI
 
 static int validate_mesh(const pfem2gpu_mesh* mesh, const pfem2gpu_params* params) {
@@ -843,7 +880,7 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
     if ((rc = dalloc(&h->stageF[0][k], h->ndofs)) ||
         (rc = dalloc(&h->stageF[1][k], h->ndofs)))
       return bail(rc);
-  if (hipHostMalloc(reinterpret_cast<void**>(&h->h_pinned), sizeof(int) * 8, 0) != hipSuccess)
+  if (hipHostMalloc(reinterpret_cast<void**>(&h->h_pinned), sizeof(int) * 16, 0) != hipSuccess)
     return bail(fail(PFEM2GPU_ERR_MEMORY, "hipHostMalloc"));
   if (hipEventCreateWithFlags(&h->settle_ev, hipEventDisableTiming) != hipSuccess)
     return bail(fail(PFEM2GPU_ERR_HIP, "hipEventCreate"));
@@ -852,6 +889,7 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
       hipStreamSynchronize(h->stream) != hipSuccess)
     return bail(fail(PFEM2GPU_ERR_HIP, "init"));
   if ((rc = grow(h, 1024, 0, 256))) return bail(rc);
+  if ((rc = scratch_for_current_device(&h->scratch))) return bail(rc);
   *out = h;
   return PFEM2GPU_OK;
 }
@@ -1828,6 +1866,126 @@ int pfem2gpu_unpack_particles(pfem2gpu_handle* h, const void* data, int64_t n_by
   CHK(flush(h));
   return h->dim == 2 ? unpack_impl<2>(h, data, n_bytes, where, n_rejected)
                      : unpack_impl<3>(h, data, n_bytes, where, n_rejected);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ no-allocation pack
+
+namespace {
+
+#define NA_LAUNCH(kernel, grid, block, ...)                                              \
+  do {                                                                                   \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, h->stream, __VA_ARGS__);      \
+    HIPCHK(hipGetLastError());                                                           \
+  } while (0)
+
+// pack_impl's steps with every per-call buffer carved from the device's
+// scratch array (offsets in ints, each piece 256-byte aligned) and launches
+// without timing events; validation of duplicates runs on the device.
+template <int D>
+int pack_noalloc_impl(pfem2gpu_handle* h, int32_t n_dest, const int32_t* dest_offsets, const int32_t* cells,
+                      void* out, int64_t capacity, int64_t* bytes, int remove) {
+  const long L = dest_offsets[n_dest];
+  const long rec = WireLayout<D>::kSize;
+  const int nc = h->nc;
+  long o = 0;
+  auto take = [&](long n) {
+    const long at = o;
+    o += (std::max(n, 1L) + 63) & ~63L;
+    return at;
+  };
+  const long i_ctl = take(4), i_doff = take(n_dest + 1), i_bytes = take(2L * (n_dest + 1));
+  const long i_cells = take(L), i_cnt = take(L), i_offs = take(L + 1), i_flag = take((nc + 3) / 4);
+  const long i_kcnt = remove ? take(nc) : 0, i_ncs = remove ? take(nc + 1L) : 0;
+  if (o > kNoAllocScratchInts)
+    return fail(PFEM2GPU_ERR_ARG, "pack_particles_noalloc: this call needs " + std::to_string(o * 4) +
+                                      " bytes of scratch; the limit is " + std::to_string(kNoAllocScratchInts * 4));
+  if (L == 0) return PFEM2GPU_OK;
+  int* S = h->scratch;
+  int* dcells = S + i_cells;
+  int* cnt = S + i_cnt;
+  int* offs = S + i_offs;
+  unsigned char* flag = reinterpret_cast<unsigned char*>(S + i_flag);
+  long long* dbytes = reinterpret_cast<long long*>(S + i_bytes);
+  HIPCHK(hipMemcpyAsync(dcells, cells, L * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(S + i_doff, dest_offsets, (n_dest + 1L) * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  // the array keeps the previous call's contents: clear the flags and the
+  // duplicate word (a fill kernel, so that the sequence can be captured)
+  NA_LAUNCH(k_fill_int, 1, kBlock, S + i_ctl, 4L, 0);
+  NA_LAUNCH(k_fill_int, grid_for((nc + 3) / 4), kBlock, S + i_flag, (long)((nc + 3) / 4), 0);
+  NA_LAUNCH(k_wire_flag_unique, grid_for(L), kBlock, (int)L, dcells, reinterpret_cast<unsigned int*>(flag), S + i_ctl);
+  NA_LAUNCH(k_wire_count, grid_for(L), kBlock, (int)L, dcells, h->cell_start, cnt);
+  const int nb = (int)((L + kScanChunk - 1) / kScanChunk);
+  NA_LAUNCH(k_scan_reduce, nb, kBlock, cnt, (int)L, h->scan_blk);
+  NA_LAUNCH(k_scan_blocks, 1, kBlock, h->scan_blk, nb, offs + L);
+  NA_LAUNCH(k_scan_down, nb, kBlock, cnt, (int)L, h->scan_blk, offs);
+  NA_LAUNCH(k_wire_dest_bytes, grid_for(n_dest + 1L), kBlock, n_dest, S + i_doff, offs, rec, dbytes);
+  if (n_dest > 0)
+    HIPCHK(hipMemcpyAsync(bytes, dbytes, n_dest * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(h->h_pinned + 8, dbytes + n_dest, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(h->h_pinned + 6, S + i_ctl, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->h_pinned[6] != 0)
+    return fail(PFEM2GPU_ERR_ARG, "pack_particles_noalloc: cell " + std::to_string(h->h_pinned[6] - 1) +
+                                      " listed twice");
+  int64_t total_bytes = 0;
+  std::memcpy(&total_bytes, h->h_pinned + 8, sizeof(int64_t));
+  if (total_bytes > capacity)
+    return fail(PFEM2GPU_ERR_ARG, "pack_particles_noalloc: output holds " + std::to_string(capacity) + " bytes, " +
+                                      std::to_string(total_bytes) + " needed");
+  if (total_bytes == 0) return PFEM2GPU_OK;
+  const long groups = (L + kWireCells - 1) / kWireCells;
+  NA_LAUNCH(k_wire_pack<D>, (int)std::min<long>(groups, 65536), kWireTile, dmesh(h), (int)L, dcells, offs,
+            h->cell_start, dparts(h), static_cast<unsigned long long*>(out));
+  if (remove) {
+    int* kcnt = S + i_kcnt;
+    int* ncs = S + i_ncs;
+    const int nb2 = (nc + kScanChunk - 1) / kScanChunk;
+    NA_LAUNCH(k_wire_keep, grid_for(nc), kBlock, nc, h->cell_start, flag, kcnt);
+    NA_LAUNCH(k_scan_reduce, nb2, kBlock, kcnt, nc, h->scan_blk);
+    NA_LAUNCH(k_scan_blocks, 1, kBlock, h->scan_blk, nb2, ncs + nc);
+    NA_LAUNCH(k_scan_down, nb2, kBlock, kcnt, nc, h->scan_blk, ncs);
+    const DTmp T = dtmp(h);
+    NA_LAUNCH(k_wire_compact<D>, grid_for(h->n_ub), kBlock, nc, h->cell_start, ncs, flag, dparts(h), T);
+    NA_LAUNCH(k_copy_state, grid_for(h->n_ub), kBlock, ncs + nc, T.x[0], T.x[1], D > 2 ? T.x[2] : nullptr, T.v[0],
+              T.v[1], D > 2 ? T.v[2] : nullptr, T.cell, h->px[0], h->px[1], D > 2 ? h->px[2] : nullptr, h->pv[0],
+              h->pv[1], D > 2 ? h->pv[2] : nullptr, h->pcell);
+    HIPCHK(hipMemcpyAsync(h->cell_start, ncs, (nc + 1L) * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
+    h->contrib_valid = false;
+    h->settle_ev_valid = false;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));  // the scratch is shared with the next call on this device
+  return PFEM2GPU_OK;
+}
+
+#undef NA_LAUNCH
+
+}  // namespace
+
+extern "C" {
+
+int pfem2gpu_pack_particles_noalloc(pfem2gpu_handle* h, int32_t n_dest, const int32_t* dest_offsets,
+                                    const int32_t* cells, void* out, int64_t capacity, int64_t* bytes, int remove) {
+  if (!h || !dest_offsets || !bytes) return fail(PFEM2GPU_ERR_ARG, "pack_particles_noalloc: null argument");
+  if (n_dest < 0) return fail(PFEM2GPU_ERR_ARG, "pack_particles_noalloc: n_dest < 0");
+  if (dest_offsets[0] != 0) return fail(PFEM2GPU_ERR_ARG, "pack_particles_noalloc: dest_offsets[0] must be 0");
+  for (int i = 0; i < n_dest; ++i)
+    if (dest_offsets[i + 1] < dest_offsets[i])
+      return fail(PFEM2GPU_ERR_ARG, "pack_particles_noalloc: dest_offsets must be non-decreasing");
+  for (int i = 0; i < n_dest; ++i) bytes[i] = 0;
+  const long L = dest_offsets[n_dest];
+  if (L > 0 && !cells) return fail(PFEM2GPU_ERR_ARG, "pack_particles_noalloc: null cell list");
+  if (L > h->nc) return fail(PFEM2GPU_ERR_ARG, "pack_particles_noalloc: more listed cells than the mesh has");
+  for (long i = 0; i < L; ++i)
+    if (cells[i] < 0 || cells[i] >= h->nc) return fail(PFEM2GPU_ERR_ARG, "pack_particles_noalloc: cell id out of range");
+  if (capacity < 0 || (capacity > 0 && !out)) return fail(PFEM2GPU_ERR_ARG, "pack_particles_noalloc: bad output buffer");
+  if (reinterpret_cast<uintptr_t>(out) & 7)
+    return fail(PFEM2GPU_ERR_ARG, "pack_particles_noalloc: device output must be 8-byte aligned");
+  if (!h->scratch) return fail(PFEM2GPU_ERR_STATE, "pack_particles_noalloc: no scratch address for this device");
+  CHK(flush(h));
+  return h->dim == 2 ? pack_noalloc_impl<2>(h, n_dest, dest_offsets, cells, out, capacity, bytes, remove)
+                     : pack_noalloc_impl<3>(h, n_dest, dest_offsets, cells, out, capacity, bytes, remove);
 }
 
 }  // extern "C"

@@ -162,6 +162,27 @@ __global__ void k_wire_flag(int L, const int* __restrict__ cells, unsigned char*
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L; i += gridDim.x * blockDim.x) flag[cells[i]] = 1;
 }
 
+// The flag of every listed cell, set with a word-wide atomic OR on the byte
+// array (4-byte aligned); a cell whose byte was already set is listed twice
+// and is reported through *dup (cell + 1).
+__global__ void k_wire_flag_unique(int L, const int* __restrict__ cells, unsigned int* __restrict__ flag_words,
+                                   int* __restrict__ dup) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L; i += gridDim.x * blockDim.x) {
+    const int c = cells[i];
+    const unsigned int bit = 1u << (8 * (c & 3));
+    const unsigned int old = atomicOr(&flag_words[c >> 2], bit);
+    if (old & bit) atomicExch(dup, c + 1);
+  }
+}
+
+// Byte counts per destination from the record offsets of the listed cells:
+// out[i] = (offs[doff[i + 1]] - offs[doff[i]]) * rec, out[n_dest] = offs[L] * rec.
+__global__ void k_wire_dest_bytes(int n_dest, const int* __restrict__ doff, const int* __restrict__ offs, long rec,
+                                  long long* __restrict__ out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= n_dest; i += gridDim.x * blockDim.x)
+    out[i] = i < n_dest ? (long long)(offs[doff[i + 1]] - offs[doff[i]]) * rec : (long long)offs[doff[n_dest]] * rec;
+}
+
 __global__ void k_wire_keep(int nc, const int* __restrict__ cs, const unsigned char* __restrict__ flag,
                             int* __restrict__ cnt) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)

@@ -134,6 +134,13 @@ int pfem2gpu_particle_count_async(pfem2gpu_handle* h, int32_t* dst);
  * Replaces iteration over begin()/end(), src/pfem2particlehandler.cpp:621-631 */
 int pfem2gpu_download(pfem2gpu_handle* h, double* x, double* xi, double* v, int32_t* cell);
 
+/* remove_particle() for n particles at once (src/pfem2particlehandler.cpp:
+ * 332-336, a std::vector::erase): index[i] (host array) is a position in the
+ * handle's current order, as pfem2gpu_download lists the particles; the other
+ * particles keep their order. An index listed twice is removed once.
+ * Synchronises. */
+int pfem2gpu_remove_particles(pfem2gpu_handle* h, int64_t n, const int64_t* index);
+
 /* Particles per cell ([n_cells], host). Synchronises. */
 int pfem2gpu_cell_counts(pfem2gpu_handle* h, int32_t* counts);
 

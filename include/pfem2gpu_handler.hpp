@@ -10,13 +10,26 @@
 // the mesh and parameters are given at construction and the nodal fields are
 // passed to the calls that use them, as std::array<std::vector<double>, dim>
 // indexed by DoF (the reference's std::array<TrilinosWrappers::MPI::Vector,
-// dim>). Failures throw std::runtime_error with the library's message (the
-// reference throws deal.II exceptions; it has no status codes).
+// dim>), or as `dim` raw component pointers (host memory by default, device
+// memory with where = PFEM2GPU_DEVICE), which a deal.II host takes from its
+// Trilinos vectors without a copy. Failures throw std::runtime_error with the
+// library's message (the reference throws deal.II exceptions; it has no
+// status codes).
+//
+// The reference times its four calls in TimerOutput sections named
+// "Particle seeding", "Particle velocity correction", "Particle transport" and
+// "Particle fields projection" (src/pfem2particlehandler.cpp:45,114,144,176)
+// and prints the summary every time step (src/pfem2solver.cpp:173). With
+// enable_timer(true) the library brackets the same calls with HIP events
+// under the same names; timer_sections() and print_timer_summary() report
+// them.
 #ifndef PFEM2GPU_HANDLER_HPP
 #define PFEM2GPU_HANDLER_HPP
 
 #include <array>
 #include <cstdint>
+#include <cstdio>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -46,6 +59,18 @@ struct Parameters {
 
 template <int dim>
 using NodalField = std::array<std::vector<double>, dim>;
+
+// The reference's TimerOutput section names of the handler's calls.
+inline const std::array<const char*, 4>& timer_section_names() {
+  static const std::array<const char*, 4> names{{"Particle seeding", "Particle velocity correction",
+                                                  "Particle transport", "Particle fields projection"}};
+  return names;
+}
+
+struct TimerSection {
+  int64_t calls = 0;
+  double total_ms = 0.0;
+};
 
 template <int dim>
 struct Particle {
@@ -88,10 +113,16 @@ class pfem2ParticleHandler {
   // initialize_maps() (src/pfem2particlehandler.cpp:319-324)
   void initialize_maps() { check(pfem2gpu_initialize_maps(h_)); }
 
-  // seed_particles() (src/pfem2particlehandler.cpp:42-109)
+  // seed_particles() (src/pfem2particlehandler.cpp:42-109). It needs only
+  // the mesh, so it may come before initialize_maps() as in
+  // src/pfem2solver.cpp:154-155.
   void seed_particles(const NodalField<dim>& V) {
     flush_inserts();
     check(pfem2gpu_seed_particles(h_, ptrs(V), PFEM2GPU_HOST));
+  }
+  void seed_particles(const std::array<const double*, dim>& V, int where = PFEM2GPU_HOST) {
+    flush_inserts();
+    check(pfem2gpu_seed_particles(h_, raw(V), where));
   }
 
   // insert_particle() (src/pfem2particlehandler.cpp:338-344): buffered and
@@ -103,6 +134,14 @@ class pfem2ParticleHandler {
       pending_x_[k].push_back(location[k]);
       pending_v_[k].push_back(velocity[k]);
     }
+  }
+
+  // remove_particle() (src/pfem2particlehandler.cpp:332-336): the particle
+  // at position `index` of particles(); the others keep their order.
+  void remove_particle(std::size_t index) { remove_particles(std::vector<int64_t>{(int64_t)index}); }
+  void remove_particles(const std::vector<int64_t>& index) {
+    flush_inserts();
+    check(pfem2gpu_remove_particles(h_, (int64_t)index.size(), index.data()));
   }
 
   // clear_particles() (src/pfem2particlehandler.cpp:326-330)
@@ -128,11 +167,26 @@ class pfem2ParticleHandler {
     check_sizes(V_old);
     check(pfem2gpu_correct_particle_velocity(h_, a, b, PFEM2GPU_HOST));
   }
+  void correct_particle_velocity(const std::array<const double*, dim>& V,
+                                 const std::array<const double*, dim>& V_old, int where = PFEM2GPU_HOST) {
+    flush_inserts();
+    const double* a[3] = {nullptr, nullptr, nullptr};
+    const double* b[3] = {nullptr, nullptr, nullptr};
+    for (int k = 0; k < dim; ++k) {
+      a[k] = V[k];
+      b[k] = V_old[k];
+    }
+    check(pfem2gpu_correct_particle_velocity(h_, a, b, where));
+  }
 
   // move_particles() (src/pfem2particlehandler.cpp:141-171)
   void move_particles(const NodalField<dim>& V) {
     flush_inserts();
     check(pfem2gpu_move_particles(h_, ptrs(V), PFEM2GPU_HOST));
+  }
+  void move_particles(const std::array<const double*, dim>& V, int where = PFEM2GPU_HOST) {
+    flush_inserts();
+    check(pfem2gpu_move_particles(h_, raw(V), where));
   }
 
   // project_particle_fields() (src/pfem2particlehandler.cpp:173-215): the
@@ -147,6 +201,41 @@ class pfem2ParticleHandler {
     }
     check(pfem2gpu_project_particle_fields(h_, o, PFEM2GPU_HOST));
     return out;
+  }
+  // ... into caller-owned component arrays of n_dofs entries each
+  void project_particle_fields(const std::array<double*, dim>& out, int where = PFEM2GPU_HOST) {
+    flush_inserts();
+    double* o[3] = {nullptr, nullptr, nullptr};
+    for (int k = 0; k < dim; ++k) o[k] = out[k];
+    check(pfem2gpu_project_particle_fields(h_, o, where));
+  }
+
+  // The reference's timer sections (HIP event time of everything each call
+  // enqueues). Turning the timer on or off clears the totals.
+  void enable_timer(bool on) { check(pfem2gpu_set_timing(h_, on ? 1 : 0)); }
+  std::map<std::string, TimerSection> timer_sections() {
+    const int cap = 128;
+    const char* names[cap];
+    int64_t calls[cap];
+    double ms[cap];
+    int n = 0;
+    check(pfem2gpu_timings(h_, cap, names, calls, ms, &n));
+    std::map<std::string, TimerSection> all;
+    for (int i = 0; i < n && i < cap; ++i) all[names[i]] = TimerSection{calls[i], ms[i]};
+    std::map<std::string, TimerSection> out;
+    for (const char* s : timer_section_names())
+      if (all.count(s)) out[s] = all[s];
+    return out;
+  }
+  // A TimerOutput-style table of the sections (wall time of the device work)
+  void print_timer_summary(std::FILE* f = stdout) {
+    const auto sec = timer_sections();
+    std::fprintf(f, "+---------------------------------+-----------+--------------+\n");
+    std::fprintf(f, "| Section                         | no. calls | device time  |\n");
+    std::fprintf(f, "+---------------------------------+-----------+--------------+\n");
+    for (const auto& e : sec)
+      std::fprintf(f, "| %-31s | %9lld | %10.4f ms |\n", e.first.c_str(), (long long)e.second.calls, e.second.total_ms);
+    std::fprintf(f, "+---------------------------------+-----------+--------------+\n");
   }
 
   // output_particle_solution() (src/pfem2particlehandler.cpp:217-317)
@@ -199,6 +288,10 @@ class pfem2ParticleHandler {
   const double* const* ptrs(const NodalField<dim>& V) {
     check_sizes(V);
     for (int k = 0; k < 3; ++k) fptr_[k] = k < dim ? V[k].data() : nullptr;
+    return fptr_;
+  }
+  const double* const* raw(const std::array<const double*, dim>& V) {
+    for (int k = 0; k < 3; ++k) fptr_[k] = k < dim ? V[k] : nullptr;
     return fptr_;
   }
   void flush_inserts() {

@@ -696,6 +696,18 @@ void orc_project_weights(void* h, double* Vout, double* W) {
   o->project(Vp, W);
 }
 
+// remove_particle (src/pfem2particlehandler.cpp:332-336, std::vector::erase)
+// of the particles at the listed positions; the others keep their order.
+void orc_remove(void* h, int64_t n, const int64_t* index) {
+  Oracle* o = static_cast<Oracle*>(h);
+  std::vector<char> drop(o->P.size(), 0);
+  for (int64_t i = 0; i < n; ++i) drop[(size_t)index[i]] = 1;
+  size_t w = 0;
+  for (size_t i = 0; i < o->P.size(); ++i)
+    if (!drop[i]) o->P[w++] = o->P[i];
+  o->P.resize(w);
+}
+
 int64_t orc_count(void* h) { return (int64_t) static_cast<Oracle*>(h)->P.size(); }
 
 // Download the particles in the stored order: x, xi, v as [dim][n], cell [n].

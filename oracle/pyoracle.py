@@ -48,6 +48,7 @@ def _load():
         lib.orc_move.argtypes = [ctypes.c_void_p, _d]
         lib.orc_project.argtypes = [ctypes.c_void_p, _d]
         lib.orc_project_weights.argtypes = [ctypes.c_void_p, _d, _d]
+        lib.orc_remove.argtypes = [ctypes.c_void_p, ctypes.c_int64, _l]
         lib.orc_count.restype = ctypes.c_int64
         lib.orc_count.argtypes = [ctypes.c_void_p]
         lib.orc_download.argtypes = [ctypes.c_void_p, _d, _d, _d, _i]
@@ -143,6 +144,10 @@ class Oracle:
         w = np.empty(self.ndofs, dtype=np.float64)
         self.lib.orc_project_weights(self.h, _dp(out), _dp(w))
         return out, w
+
+    def remove(self, index):
+        idx = np.ascontiguousarray(index, dtype=np.int64).ravel()
+        self.lib.orc_remove(self.h, idx.size, idx.ctypes.data_as(_l))
 
     def count(self):
         return int(self.lib.orc_count(self.h))

@@ -60,7 +60,7 @@ EXPORTS = [
     "pfem2gpu_adjoint", "pfem2gpu_move_particles_traced",
     "pfem2gpu_output_particle_solution", "pfem2gpu_write_particles_vtu", "pfem2gpu_write_particles_pvtu",
     "pfem2gpu_vertex_to_cells", "pfem2gpu_particle_record_size", "pfem2gpu_pack_particles",
-    "pfem2gpu_unpack_particles", "pfem2gpu_pack_particles_noalloc",
+    "pfem2gpu_unpack_particles", "pfem2gpu_pack_particles_noalloc", "pfem2gpu_remove_particles",
 ]
 
 
@@ -115,6 +115,7 @@ def load_library(path=None):
     lib.pfem2gpu_pack_particles.argtypes = [_vp, ctypes.c_int32, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int, _vp,
                                             ctypes.c_int]
     lib.pfem2gpu_unpack_particles.argtypes = [_vp, _vp, ctypes.c_int64, ctypes.c_int, _i64p]
+    lib.pfem2gpu_remove_particles.argtypes = [_vp, ctypes.c_int64, _vp]
     lib.pfem2gpu_pack_particles_noalloc.argtypes = [_vp, ctypes.c_int32, _vp, _vp, _vp, ctypes.c_int64, _vp,
                                                     ctypes.c_int]
     if path is None:
@@ -266,6 +267,14 @@ class ParticleHandler:
                "insert_particles")
         del keep
         return int(rej.value)
+
+    def remove_particles(self, index):
+        """remove_particle for the particles at the given positions of the
+        handle's order (src/pfem2particlehandler.cpp:332-336); the others keep
+        their order."""
+        idx = np.ascontiguousarray(index, dtype=np.int64).ravel()
+        _check(self.lib, self.lib.pfem2gpu_remove_particles(self.h, idx.size, idx.ctypes.data if idx.size else None),
+               "remove_particles")
 
     def clear_particles(self):
         _check(self.lib, self.lib.pfem2gpu_clear_particles(self.h), "clear_particles")

@@ -210,6 +210,25 @@ int t_collect(pfem2gpu_handle* h) {
   return PFEM2GPU_OK;
 }
 
+// One of the reference's TimerOutput sections around a handler call
+// (src/pfem2particlehandler.cpp:45,114,144,176): with timing on, a pair of
+// events on the handle's stream brackets everything the call enqueues.
+struct Section {
+  pfem2gpu_handle* h;
+  long idx = -1;
+  Section(pfem2gpu_handle* hh, const char* name) : h(hh) {
+    if (!h || !h->timing) return;
+    Timed t{name, take_event(h), take_event(h)};
+    if (t.a && t.b && hipEventRecord(t.a, h->stream) == hipSuccess) {
+      h->pending.push_back(t);
+      idx = (long)h->pending.size() - 1;
+    }
+  }
+  ~Section() {
+    if (idx >= 0) (void)hipEventRecord(h->pending[idx].b, h->stream);
+  }
+};
+
 #define LAUNCH_B(h, name, kernel, grid, block, smem, ...)                         \
   do {                                                                            \
     t_begin(h, name);                                                             \
@@ -1134,6 +1153,7 @@ int pfem2gpu_insert_particles(pfem2gpu_handle* h, int64_t n, const double* x, co
 
 int pfem2gpu_seed_particles(pfem2gpu_handle* h, const double* const* V, int where) {
   if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  Section sec(h, "Particle seeding");
   DField F;
   CHK(field_in(h, V, where, 0, &F));
   CHK(flush(h));
@@ -1194,6 +1214,7 @@ int pfem2gpu_set_velocity_from_field(pfem2gpu_handle* h, const double* const* V,
 int pfem2gpu_correct_particle_velocity(pfem2gpu_handle* h, const double* const* V, const double* const* V_old,
                                        int where) {
   if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  Section sec(h, "Particle velocity correction");
   CHK(flush(h));
   DField A, B;
   CHK(field_in(h, V, where, 0, &A));
@@ -1209,6 +1230,7 @@ int pfem2gpu_correct_particle_velocity(pfem2gpu_handle* h, const double* const* 
 int pfem2gpu_move_particles(pfem2gpu_handle* h, const double* const* V, int where) {
   if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
   if (!h->maps_ready) return fail(PFEM2GPU_ERR_STATE, "initialize_maps() must be called before move_particles()");
+  Section sec(h, "Particle transport");
   DField F;
   CHK(field_in(h, V, where, 0, &F));
   CHK(reserve(h, h->refill_max, 0));
@@ -1219,6 +1241,7 @@ int pfem2gpu_project_particle_fields(pfem2gpu_handle* h, double* const* V_out, i
   if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
   if (!V_out) return fail(PFEM2GPU_ERR_ARG, "null output");
   if (where != PFEM2GPU_HOST && where != PFEM2GPU_DEVICE) return fail(PFEM2GPU_ERR_ARG, "bad where");
+  Section sec(h, "Particle fields projection");
   CHK(flush(h));
   const long ne = (long)h->nc * h->nvc;
   double* out[3] = {nullptr, nullptr, nullptr};
@@ -1986,6 +2009,62 @@ int pfem2gpu_pack_particles_noalloc(pfem2gpu_handle* h, int32_t n_dest, const in
   CHK(flush(h));
   return h->dim == 2 ? pack_noalloc_impl<2>(h, n_dest, dest_offsets, cells, out, capacity, bytes, remove)
                      : pack_noalloc_impl<3>(h, n_dest, dest_offsets, cells, out, capacity, bytes, remove);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ particle removal
+
+namespace {
+
+template <int D>
+int remove_impl(pfem2gpu_handle* h, long n_rm, const int64_t* index, long n) {
+  DevBuf mem(h);
+  long long* drm = nullptr;
+  int *keep = nullptr, *pos = nullptr, *blk = nullptr, *cs_new = nullptr;
+  const int nb = (int)((n + kScanChunk - 1) / kScanChunk);
+  CHK(mem.get(&drm, n_rm));
+  CHK(mem.get(&keep, n));
+  CHK(mem.get(&pos, n + 1));
+  CHK(mem.get(&blk, nb + 1));
+  CHK(mem.get(&cs_new, h->nc + 1L));
+  HIPCHK(hipMemcpyAsync(drm, index, n_rm * sizeof(long long), hipMemcpyHostToDevice, h->stream));
+  const int* nptr = h->cell_start + h->nc;
+  LAUNCH(h, "k_remove", k_keep_flags, grid_for(n), 0, nptr, keep);
+  LAUNCH(h, "k_remove", k_drop_listed, grid_for(n_rm), 0, n_rm, drm, keep);
+  LAUNCH(h, "scan", k_scan_reduce, nb, 0, keep, (int)n, blk);
+  LAUNCH(h, "scan", k_scan_blocks, 1, 0, blk, nb, pos + n);
+  LAUNCH(h, "scan", k_scan_down, nb, 0, keep, (int)n, blk, pos);
+  const DTmp T = dtmp(h);
+  LAUNCH(h, "k_remove", k_compact_kept<D>, grid_for(std::max(n, (long)h->nc + 1)), 0, nptr, h->nc, keep, pos,
+         h->cell_start, cs_new, dparts(h), T);
+  LAUNCH(h, "k_copy", k_copy_state, grid_for(n), 0, cs_new + h->nc, T.x[0], T.x[1], D > 2 ? T.x[2] : nullptr, T.v[0],
+         T.v[1], D > 2 ? T.v[2] : nullptr, T.cell, h->px[0], h->px[1], D > 2 ? h->px[2] : nullptr, h->pv[0], h->pv[1],
+         D > 2 ? h->pv[2] : nullptr, h->pcell);
+  HIPCHK(hipMemcpyAsync(h->cell_start, cs_new, (h->nc + 1L) * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
+  h->contrib_valid = false;
+  h->settle_ev_valid = false;
+  HIPCHK(hipStreamSynchronize(h->stream));  // the pool buffers are reused after return
+  long c = 0;
+  return read_count(h, &c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfem2gpu_remove_particles(pfem2gpu_handle* h, int64_t n, const int64_t* index) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  if (n < 0 || (n > 0 && !index)) return fail(PFEM2GPU_ERR_ARG, "remove_particles: bad index array");
+  if (n == 0) return PFEM2GPU_OK;
+  CHK(flush(h));
+  long count = 0;
+  CHK(read_count(h, &count));
+  for (int64_t i = 0; i < n; ++i)
+    if (index[i] < 0 || index[i] >= count)
+      return fail(PFEM2GPU_ERR_ARG, "remove_particles: index " + std::to_string(index[i]) + " out of range [0, " +
+                                        std::to_string(count) + ")");
+  return h->dim == 2 ? remove_impl<2>(h, n, index, count) : remove_impl<3>(h, n, index, count);
 }
 
 }  // extern "C"

@@ -1458,6 +1458,40 @@ __global__ __launch_bounds__(kBlock) void k_node(int ndofs, const int* __restric
 
 // ------------------------------------------------------ per-particle kernels
 
+// Removal of listed particles (remove_particle, an erase that keeps the
+// order of the others, src/pfem2particlehandler.cpp:332-336): keep flags,
+// then each kept particle moves to the exclusive prefix of the flags; the
+// cell segments follow because the particles are sorted by cell.
+__global__ void k_keep_flags(const int* __restrict__ n_ptr, int* __restrict__ keep) {
+  const int n = *n_ptr;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) keep[i] = 1;
+}
+
+__global__ void k_drop_listed(long n_rm, const long long* __restrict__ rm, int* __restrict__ keep) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n_rm; i += (long)gridDim.x * blockDim.x)
+    keep[rm[i]] = 0;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_compact_kept(const int* __restrict__ n_ptr, int nc,
+                                                        const int* __restrict__ keep, const int* __restrict__ pos,
+                                                        const int* __restrict__ cs, int* __restrict__ cs_new,
+                                                        DParts P, DTmp T) {
+  const int n = *n_ptr;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i <= (long)nc; i += (long)gridDim.x * blockDim.x)
+    cs_new[i] = pos[cs[i]];
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    if (!keep[p]) continue;
+    const int q = pos[p];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      T.x[k][q] = P.x[k][p];
+      T.v[k][q] = P.v[k][p];
+    }
+    T.cell[q] = P.cell[p];
+  }
+}
+
 // v += interp(dV) (flush of a pending correction outside move)
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_correct(DMesh m, const int* __restrict__ n_ptr, DParts P,

@@ -5,12 +5,20 @@
 //
 //   handler_run <input> <output>
 //
-// Input (little-endian): int32 dim, int32 steps; int64 n_vertices, n_cells,
-// n_dofs, n_particles; uint32 n_sub; double dt; uint32 max_per_part;
-// uint32 q[3]; double X[dim][n_vertices]; int32 cell_vertices[n_cells][2^dim];
-// int32 cell[n_particles]; double x[dim][n_particles]; double V[dim][n_dofs].
-// Steps: insert_particle for each particle (zero velocity), then per step
-// move_particles(V), P = project_particle_fields(), correct_particle_velocity(V, P).
+// Input (little-endian): int32 dim, int32 steps, int32 order; int64
+// n_vertices, n_cells, n_dofs, n_particles; uint32 n_sub; double dt; uint32
+// max_per_part; uint32 q[3]; double X[dim][n_vertices]; int32
+// cell_vertices[n_cells][2^dim]; int32 cell[n_particles]; double
+// x[dim][n_particles]; double V[dim][n_dofs].
+// order 0: initialize_maps, insert_particle for each particle (zero
+// velocity), then per step move_particles(V), P = project_particle_fields(),
+// correct_particle_velocity(V, P).
+// order 1, the reference solver's order (src/pfem2solver.cpp:151-166) with
+// the FEM step replaced by a prescribed field: seed_particles(V) before
+// initialize_maps(), V_old = 0 (the FEM's initial old solution,
+// src/pfem2fem.cpp:586-594), then per step correct_particle_velocity(V,
+// V_old), move_particles(V), P = project_particle_fields(), V_old = P; the
+// reference's four timer sections are printed at the end.
 // Output: int64 n, rejected; double x[dim][n], xi[dim][n], v[dim][n];
 // int32 cell[n]; double P[dim][n_dofs].
 #include <cstdio>
@@ -44,7 +52,7 @@ void put(FILE* f, const T* p, std::size_t n) {
 }
 
 template <int dim>
-int run(Reader& in, int steps, FILE* out) {
+int run(Reader& in, int steps, int order, FILE* out) {
   const int64_t nv = in.get<int64_t>(), nc = in.get<int64_t>(), nd = in.get<int64_t>(), np = in.get<int64_t>();
   pfem2gpu::Parameters prm;
   prm.particle_integration_steps = in.get<uint32_t>();
@@ -62,17 +70,32 @@ int run(Reader& in, int steps, FILE* out) {
   for (int k = 0; k < dim; ++k) V[k] = in.vec<double>((std::size_t)nd);
 
   pfem2gpu::pfem2ParticleHandler<dim> h(mesh, prm);
-  h.initialize_maps();
-  for (int64_t p = 0; p < np; ++p) {
-    std::array<double, dim> loc;
-    for (int k = 0; k < dim; ++k) loc[k] = x[k * np + p];
-    h.insert_particle(loc, cell[p]);
-  }
   pfem2gpu::NodalField<dim> P;
-  for (int s = 0; s < steps; ++s) {
-    h.move_particles(V);
-    P = h.project_particle_fields();
-    h.correct_particle_velocity(V, P);
+  if (order == 0) {
+    h.initialize_maps();
+    for (int64_t p = 0; p < np; ++p) {
+      std::array<double, dim> loc;
+      for (int k = 0; k < dim; ++k) loc[k] = x[k * np + p];
+      h.insert_particle(loc, cell[p]);
+    }
+    for (int s = 0; s < steps; ++s) {
+      h.move_particles(V);
+      P = h.project_particle_fields();
+      h.correct_particle_velocity(V, P);
+    }
+  } else {
+    h.enable_timer(true);
+    h.seed_particles(V);
+    h.initialize_maps();
+    pfem2gpu::NodalField<dim> V_old;
+    for (int k = 0; k < dim; ++k) V_old[k].assign((std::size_t)nd, 0.0);
+    for (int s = 0; s < steps; ++s) {
+      h.correct_particle_velocity(V, V_old);
+      h.move_particles(V);
+      P = h.project_particle_fields();
+      V_old = P;
+    }
+    h.print_timer_summary(stdout);
   }
   const std::vector<pfem2gpu::Particle<dim>> parts = h.particles();
   const int64_t n = (int64_t)parts.size(), rej = h.rejected_insertions();
@@ -113,9 +136,9 @@ int main(int argc, char** argv) {
   int rc = 1;
   try {
     Reader in{fi};
-    const int dim = in.get<int32_t>(), steps = in.get<int32_t>();
-    if (dim == 2) rc = run<2>(in, steps, fo);
-    else if (dim == 3) rc = run<3>(in, steps, fo);
+    const int dim = in.get<int32_t>(), steps = in.get<int32_t>(), order = in.get<int32_t>();
+    if (dim == 2) rc = run<2>(in, steps, order, fo);
+    else if (dim == 3) rc = run<3>(in, steps, order, fo);
     else throw std::runtime_error("dim must be 2 or 3");
   } catch (const std::exception& e) {
     std::fprintf(stderr, "handler_run: %s\n", e.what());

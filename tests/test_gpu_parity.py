@@ -342,3 +342,38 @@ def test_dense_chunks_exceed_lds_stage(gpu_available):
     xi = rng.random((2, cells.shape[0]))
     x = W.seed_from_unit(mesh, cells, xi)
     _run_pair(cfg, 2, x=x, cells=cells)
+
+
+def test_remove_particles_keeps_the_order(gpu_available):
+    """remove_particle (src/pfem2particlehandler.cpp:332-336, an erase): the
+    particles at listed positions leave the handle, the others keep their
+    order, cell segments and placed flags; the next steps equal the oracle's
+    after the same erase."""
+    _say("remove particles after a step, channel 32x16, then 2 steps")
+    cfg = W.config("channel2d", scale=0.5)
+    o = OracleStepper(cfg)
+    g = GpuStepper(cfg)
+    o.insert(cfg["x"], cfg["cells"], cfg["V"])
+    g.insert(cfg["x"], cfg["cells"], cfg["V"])
+    o.step(cfg["V"])
+    g.step(cfg["V"])
+    before = g.download()
+    n = before["cell"].shape[0]
+    rng = np.random.default_rng(17)
+    idx = rng.choice(n, size=n // 3, replace=False)
+    idx = np.concatenate([idx, idx[:5]])  # repeats are removed once
+    g.h.remove_particles(idx)
+    o.o.remove(np.unique(idx))
+    keep = np.ones(n, dtype=bool)
+    keep[idx] = False
+    after = g.download()
+    for k in ("x", "xi", "v"):
+        assert np.array_equal(after[k], before[k][:, keep]), k
+    assert np.array_equal(after["cell"], before["cell"][keep])
+    assert np.array_equal(g.h.cell_counts(), np.bincount(before["cell"][keep], minlength=g.h.n_cells))
+    assert_particles_match(after, o.download())
+    for _ in range(2):
+        assert_fields_match(g.step(cfg["V"]), o.step(cfg["V"]))
+        assert_particles_match(g.download(), o.download())
+    with pytest.raises(Exception):
+        g.h.remove_particles([g.h.count()])

@@ -740,9 +740,10 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
   if (hip_stream) {
     h->stream = (hipStream_t)hip_stream;
   } else {
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    const hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
       delete h;
-      return fail(PFEM2GPU_ERR_HIP, "hipStreamCreate failed");
+      return fail(PFEM2GPU_ERR_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
     }
     h->own_stream = true;
   }

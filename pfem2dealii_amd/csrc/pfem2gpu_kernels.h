@@ -505,7 +505,7 @@ struct SettleLayout {
   int off_fin;                  // [nslots] final order: x[D], v[D], xi[D] (doubles), src (int), chunk-local
                                 // cell (byte); before
                                 // the final placement the same bytes hold key (u64), ord (int), xi[D]
-                                // and v[D] (doubles) in arrival order
+                                // and v[D] (doubles) in arrival order and the keys in group order (u64)
   int off_cnt, off_base, off_roff;  // [cpw * pmax] per (cell, part)
   int off_cbuf[2];              // cell data of the current / next chunk
   int off_geo, off_vc, off_prm, off_inst;  // offsets inside a cell data buffer
@@ -710,6 +710,12 @@ constexpr int kSettleShift = 4;
 // the final-order cell byte holds the chunk-local cell in bits 0-6 and the
 // placed flag in bit 7
 static_assert(kSettleShift <= 7, "chunk-local cell ids must fit seven bits");
+// the arrival-order arrays (key, order, xi, v, group-order key) share the
+// bytes of the final-order ones (x, v, xi, src, cell byte)
+static_assert(8 + 4 + 16 * 2 + 8 <= 24 * 2 + 5 && 8 + 4 + 16 * 3 + 8 <= 24 * 3 + 5, "arrival arrays exceed the slot");
+// the projection sums take one thread per (cell, vertex) of a chunk on the
+// wavefronts after the first, and the write-out at least the first
+static_assert((1 << kSettleShift) * 8 <= kSettleBlock - 64, "too few threads for the projection sums");
 
 // Cell data of a chunk (box geometry, advecting field, instance and its
 // parameters), carried in registers from its loads to its stores into an LDS
@@ -732,7 +738,10 @@ template <int D>
 __device__ __forceinline__ void cells_load(const DMesh& m, const double* __restrict__ Vc, int do_refill, int c0,
                                            int ncell, CellRegs<D>& r) {
   constexpr int NV = 1 << D;
-  const int t = threadIdx.x;
+  int t = threadIdx.x;
+  // the per-lane addresses below are rebuilt at every call rather than kept
+  // across the settle loop (where they would be spilled)
+  asm volatile("" : "+v"(t));
   r.geo = t < ncell * 3 * D ? m.geom[(size_t)c0 * 3 * D + t] : 0.0;
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
@@ -785,17 +794,19 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
   const int cpw = 1 << cshift;
   const int pmax = 1 << pshift;  // part buckets per cell, padded to a power of two
   const SettleLayout L = settle_layout(D, cpw, pmax, nslots);
-  double* s_fx = reinterpret_cast<double*>(smem_raw + L.off_fin);  // [D][nslots]
-  double* s_fv = s_fx + D * nslots;                                // [D][nslots]
-  double* s_fxi = s_fv + D * nslots;                               // [D][nslots]
-  int* s_fsrc = reinterpret_cast<int*>(s_fxi + D * nslots);        // [nslots]
-  unsigned char* s_fcell = reinterpret_cast<unsigned char*>(s_fsrc + nslots);  // [nslots]
+  const int ns = nslots;  // row stride of the per-slot arrays
+  double* s_fx = reinterpret_cast<double*>(smem_raw + L.off_fin);  // [D][ns]
+  double* s_fv = s_fx + D * ns;                                    // [D][ns]
+  double* s_fxi = s_fv + D * ns;                                   // [D][ns]
+  int* s_fsrc = reinterpret_cast<int*>(s_fxi + D * ns);            // [ns]
+  unsigned char* s_fcell = reinterpret_cast<unsigned char*>(s_fsrc + ns);  // [ns]
   // before the final placement: key, group order, reference position and
   // velocity of the staged particles, in arrival order (same bytes)
   unsigned long long* s_key = reinterpret_cast<unsigned long long*>(smem_raw + L.off_fin);  // [nmax]
-  int* s_ord = reinterpret_cast<int*>(s_key + nslots);                                    // [nmax]
-  double* s_sxi = reinterpret_cast<double*>(s_ord + nslots);                              // [D][nslots]
-  double* s_sv = s_sxi + D * nslots;                                                      // [D][nslots]
+  int* s_ord = reinterpret_cast<int*>(s_key + ns);                                        // [nmax]
+  double* s_sxi = reinterpret_cast<double*>(s_ord + ns);                                  // [D][ns]
+  double* s_sv = s_sxi + D * ns;                                                          // [D][ns]
+  unsigned long long* s_gkey = reinterpret_cast<unsigned long long*>(s_sv + D * ns);      // [ns] keys in group order
   int* s_cnt = reinterpret_cast<int*>(smem_raw + L.off_cnt);
   int* s_base = reinterpret_cast<int*>(smem_raw + L.off_base);
   int* s_roff = reinterpret_cast<int*>(smem_raw + L.off_roff);
@@ -908,7 +919,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         if (staged_in) {
           s_key[j] = particle_key<D>(px[u]);
 #pragma unroll
-          for (int k = 0; k < D; ++k) s_sxi[k * nslots + j] = xi[k];
+          for (int k = 0; k < D; ++k) s_sxi[k * ns + j] = xi[k];
         }
       }
     }
@@ -918,7 +929,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         const int j = threadIdx.x + u * blockDim.x;
         if (j < n)
 #pragma unroll
-          for (int k = 0; k < D; ++k) s_sv[k * nslots + j] = pv[u][k];
+          for (int k = 0; k < D; ++k) s_sv[k * ns + j] = pv[u][k];
       }
     }
     for (int j = threadIdx.x + PRE * blockDim.x; j < n; j += blockDim.x) {  // beyond the registers: not staged
@@ -992,7 +1003,9 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         const int j = threadIdx.x + u * blockDim.x;
         if (j < n) {
           const int t = ((ppl[u] >> 16) << pshift) + (ppl[u] & 0x7fff);
-          s_ord[atomicAdd(&s_roff[t], 1)] = j;
+          const int g = atomicAdd(&s_roff[t], 1);
+          s_ord[g] = j;
+          s_gkey[g] = s_key[j];  // the ranking reads the group's keys without the indirection
         }
       }
     } else {
@@ -1024,12 +1037,12 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
           const int b = s_roff[t], a = b - s_cnt[t];
           int r = 0;
           for (int q = a; q < b; ++q) {
-            const int o = s_ord[q];
-            if (o == j) continue;
-            const unsigned long long kq = s_key[o];
+            const unsigned long long kq = s_gkey[q];
             if (kq < key) {
               ++r;
             } else if (kq == key) {
+              const int o = s_ord[q];
+              if (o == j) continue;
               double xq[D], vq[D];
               const long iq = chunk_index(o, nb, bucket0, spill0);
 #pragma unroll
@@ -1039,7 +1052,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
               }
               double vj[D];
 #pragma unroll
-              for (int k = 0; k < D; ++k) vj[k] = s_sv[k * nslots + j];
+              for (int k = 0; k < D; ++k) vj[k] = s_sv[k * ns + j];
               if (tie_less<D>(xq, vq, px[u], vj, o, j)) ++r;
             }
           }
@@ -1094,11 +1107,17 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         const int j = threadIdx.x + u * blockDim.x;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          qxi[u][k] = pf[u] >= 0 ? s_sxi[k * nslots + j] : 0.0;
-          qv[u][k] = pf[u] >= 0 ? s_sv[k * nslots + j] : 0.0;
+          qxi[u][k] = pf[u] >= 0 ? s_sxi[k * ns + j] : 0.0;
+          qv[u][k] = pf[u] >= 0 ? s_sv[k * ns + j] : 0.0;
         }
       }
     }
+    // C. output offset of a staged chunk by decoupled look-back on the first
+    //    wavefront: its first round of status loads is issued here and
+    //    waited for after the wavefront's placement and refills
+    const bool lb_on = staged && threadIdx.x < 64 && ch > 0;
+    unsigned long long lbw = 0;
+    if (lb_on) lbw = lb_word(lb, ch - 1 - (int)threadIdx.x);
     __syncthreads();  // the arrival-order arrays are dead from here on
     // final placement of the staged survivors
     if (staged) {
@@ -1108,9 +1127,9 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         if (f >= 0) {
 #pragma unroll
           for (int k = 0; k < D; ++k) {
-            s_fx[k * nslots + f] = px[u][k];
-            s_fv[k * nslots + f] = qv[u][k];
-            s_fxi[k * nslots + f] = qxi[u][k];
+            s_fx[k * ns + f] = px[u][k];
+            s_fv[k * ns + f] = qv[u][k];
+            s_fxi[k * ns + f] = qxi[u][k];
           }
           s_fcell[f] = (unsigned char)((ppl[u] >> 16) | ((ppl[u] & 0x8000) ? 0x80 : 0));
           if (P.src) s_fsrc[f] = T.src[chunk_index(threadIdx.x + u * blockDim.x, nb, bucket0, spill0)];
@@ -1157,9 +1176,9 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         if (staged) {
 #pragma unroll
           for (int k = 0; k < D; ++k) {
-            s_fx[k * nslots + f] = x[k];
-            s_fv[k * nslots + f] = v[k];
-            s_fxi[k * nslots + f] = xi[k];
+            s_fx[k * ns + f] = x[k];
+            s_fv[k * ns + f] = v[k];
+            s_fxi[k * ns + f] = xi[k];
           }
           s_fsrc[f] = -1 - pp;
           s_fcell[f] = (unsigned char)(lc | 0x80);  // placed at the part centre
@@ -1174,61 +1193,19 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         }
       }
     }
-    // C. output offset of a staged chunk by decoupled look-back: its first
-    //    round of status loads is issued here and waited for only after the
-    //    projection sums, which do not depend on it
-    const bool lb_on = staged && threadIdx.x < 64 && ch > 0;
-    unsigned long long lbw = 0;
-    if (lb_on) lbw = lb_word(lb, ch - 1 - (int)threadIdx.x);
-    __syncthreads();  // final-order arrays complete
-    if (staged) {
-      // E1. per (cell, vertex) sums, sequential in final order: one thread
-      //     per (cell, vertex, component), component D being W
-      for (int t = threadIdx.x; t < ncell * NV * (D + 1); t += blockDim.x) {
-        const int q = t / (D + 1), comp = t - q * (D + 1);
-        const int lc = q / NV, iv = q - lc * NV;
-        const int c = c0 + lc;
-        const int a = s_cfin[lc], b = (lc + 1 < ncell) ? s_cfin[lc + 1] : nout;
-        const double* vv = s_fv + (comp < D ? comp : 0) * nslots;
-        double S = 0.0;
-        int f = a;
-        for (; f + 4 <= b; f += 4) {  // loads of four particles issued together
-          double N[4], w[4];
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-              const double xk = s_fxi[k * nslots + f + e];
-              const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
-              N[e] = (k == 0) ? l : N[e] * l;
-            }
-            w[e] = vv[f + e];
-          }
-#pragma unroll
-          for (int e = 0; e < 4; ++e) S = comp < D ? S + N[e] * w[e] : S + N[e];
-        }
-        for (; f < b; ++f) {
-          double N = 1.0;
-#pragma unroll
-          for (int k = 0; k < D; ++k) {
-            const double xk = s_fxi[k * nslots + f];
-            const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
-            N = (k == 0) ? l : N * l;
-          }
-          S = comp < D ? S + N * vv[f] : S + N;
-        }
-        contrib[((size_t)c * NV + iv) * (D + 1) + comp] = S;
-      }
-    }
-    if (lb_on) {  // (before the next chunk's loads: a wavefront's loads complete in order)
+    if (lb_on) {  // (its loads complete before the next chunk's, issued below)
       const unsigned prefix = lb_prefix_wave(lb, ch, lbw, err);
       if (threadIdx.x == 0) {
         lb_publish(lb, ch, 2u, prefix + (unsigned)nout);
         s_misc[8] = (int)prefix;
       }
     }
-    // the next chunk: particles into the registers, cell data into the other
-    // buffer (both used from the next iteration on)
+    __syncthreads();  // final-order arrays complete, the output offset known
+    const int o0 = s_misc[8];
+    // the next chunk's loads first, so that they arrive during the sums and
+    // the write-out: particles into the registers (free since the
+    // placement), cell data into registers and from there into the other
+    // buffer at the end
     const int nch = s_misc[4];
     const int nc0 = nch << cshift;
     const int nncell = nch < nchunks ? min(nc, nc0 + cpw) - nc0 : 0;
@@ -1238,25 +1215,79 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       const int nnb = min(s_misc[5], capc);
       load_parts(nnb + s_misc[7] - s_misc[6], nnb, (long)nch * capc, spill2_base + s_misc[6]);
     }
-    __syncthreads();
-    const int o0 = s_misc[8];
-    for (int i = threadIdx.x; i < cpw * pmax; i += blockDim.x) s_cnt[i] = 0;  // for the next chunk
-    for (int i = threadIdx.x; i < cpw; i += blockDim.x) s_ccnt[i] = 0;
-    for (int lc = threadIdx.x; lc < ncell; lc += blockDim.x) cs_out[c0 + lc] = o0 + s_cfin[lc];
-    if (c1 == nc && threadIdx.x == 0) cs_out[nc] = o0 + nout;
-    if (staged) {
+    // E1 on the wavefronts after the first (one thread per (cell, vertex)),
+    // E2 on the others
+    const int e1_waves = (cpw * NV + 63) >> 6;
+    const int wave = threadIdx.x >> 6;
+    if (staged && wave >= 1 && wave <= e1_waves) {
+      // E1. per (cell, vertex) sums S_k = sum N_i v_k and W = sum N_i over
+      //     the cell's particles in final order, each component a sequential
+      //     sum
+      constexpr int G = 4;  // the loads of a group of particles issued together
+      const int e1 = (int)threadIdx.x - 64;
+      if (e1 < ncell * NV) {
+        const int lc = e1 >> D, iv = e1 & (NV - 1);
+        const int a = s_cfin[lc], b = (lc + 1 < ncell) ? s_cfin[lc + 1] : nout;
+        double S[D], W = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) S[k] = 0.0;
+        int f = a;
+        for (; f + G <= b; f += G) {
+          double N[G], w[G][D];
+#pragma unroll
+          for (int e = 0; e < G; ++e) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+              const double xk = s_fxi[k * ns + f + e];
+              const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
+              N[e] = (k == 0) ? l : N[e] * l;
+              w[e][k] = s_fv[k * ns + f + e];
+            }
+          }
+#pragma unroll
+          for (int e = 0; e < G; ++e) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) S[k] = S[k] + N[e] * w[e][k];
+            W = W + N[e];
+          }
+        }
+        for (; f < b; ++f) {
+          double N = 1.0;
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            const double xk = s_fxi[k * ns + f];
+            const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
+            N = (k == 0) ? l : N * l;
+          }
+#pragma unroll
+          for (int k = 0; k < D; ++k) S[k] = S[k] + N * s_fv[k * ns + f];
+          W = W + N;
+        }
+        double* co = contrib + ((size_t)(c0 + lc) * NV + iv) * (D + 1);
+#pragma unroll
+        for (int k = 0; k < D; ++k) co[k] = S[k];
+        co[D] = W;
+      }
+    } else if (staged) {
       // E2. coalesced write-out in final order
-      for (int t = threadIdx.x; t < nout; t += blockDim.x) {
+      const int e2 = wave == 0 ? (int)threadIdx.x : (int)threadIdx.x - 64 * e1_waves;
+      const int ne2 = (int)blockDim.x - 64 * e1_waves;
+      for (int t = e2; t < nout; t += ne2) {
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          P.x[k][o0 + t] = s_fx[k * nslots + t];
-          P.v[k][o0 + t] = s_fv[k * nslots + t];
+          P.x[k][o0 + t] = s_fx[k * ns + t];
+          P.v[k][o0 + t] = s_fv[k * ns + t];
         }
         const int fc = s_fcell[t];
         P.cell[o0 + t] = (c0 + (fc & 0x7f)) | ((fc & 0x80) ? kPlaced : 0);
         if (P.src) P.src[o0 + t] = s_fsrc[t];
       }
     }
+    for (int lc = threadIdx.x; lc < ncell; lc += blockDim.x) cs_out[c0 + lc] = o0 + s_cfin[lc];
+    if (c1 == nc && threadIdx.x == 0) cs_out[nc] = o0 + nout;
+    // the counts of the next chunk start at zero (read last by the refills)
+    for (int i = threadIdx.x; i < cpw * pmax; i += blockDim.x) s_cnt[i] = 0;
+    for (int i = threadIdx.x; i < cpw; i += blockDim.x) s_ccnt[i] = 0;
     if (nch < nchunks) cells_store<D>(cr, nncell, cbuf_geo(buf ^ 1), cbuf_vc(buf ^ 1), cbuf_prm(buf ^ 1), cbuf_inst(buf ^ 1));
     if (threadIdx.x == 0) {
       dirty[ch] = staged ? 0 : 1;

@@ -13,11 +13,13 @@ on the handle's stream around the K steps; particles counted at the start of
 each step). Run B repeats the same W + K steps from the same state with HIP
 events around every launch, for the per-kernel durations and the roofline of
 the dominant kernel (algorithmic bytes per launch / average launch duration;
-the traffic figure comes from the committed counter summary). Then the
-256-instance batched form against plain launches, the adjoint's cost
-against its forward, the pack and unpack of the reference's particle records
-for half the workload's cells, and a CPU baseline: the oracle in reference-order mode on
-a bounded sample of the same workload, timed on this host. Prints one JSON
+the traffic figure comes from the committed counter summary). The same two
+runs on the 128^3 ABC configuration (16.8M particles, 3D) give the `abc3d`
+object. Then the 256-instance batched form against plain launches (20
+steps), the adjoint's cost against its forward, the pack and unpack of the
+reference's particle records for half the workload's cells, and the CPU
+baseline: the oracle in reference-order mode, one thread, one full step of
+each configuration, timed on this host (CPU model reported). Prints one JSON
 line.
 """
 import argparse
@@ -71,38 +73,48 @@ def hbm_traffic(workload, scale, kernel):
     return None
 
 
-def cpu_baseline(workload, budget_s=12.0):
-    """Oracle in reference-order mode (AoS vector, std::map redistribution,
-    vector-order projection), 1 thread, on a 1/4-per-direction sample of the
-    workload; steps until the budget is spent."""
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform string."""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(workload, steps=1):
+    """The oracle in reference-order mode (AoS vector, swap-with-back
+    deletion, std::map redistribution, vector-order projection), one thread,
+    on the full workload: `steps` full steps (move, project, correct) from
+    the seeded state, particles counted at the start of each step."""
     from oracle import Oracle, MODE_REFERENCE
     from pfem2dealii_amd import workloads as W
-    scale = 0.25 if workload != "abc3d" else 0.5
-    cfg = W.config(workload, scale=scale)
+    cfg = W.config(workload)
     o = Oracle(cfg["mesh"], cfg["params"], MODE_REFERENCE)
     o.initialize_maps()
     o.insert(cfg["x"], cfg["cells"])
+    n_seeded = len(cfg["cells"])
     V = cfg["V"]
     o.set_velocity_from_field(V)
-    done = 0
+    del cfg["x"], cfg["cells"]
     psteps = 0
     t0 = time.perf_counter()
-    while done < cfg["steps"]:
-        n = o.count()
+    for _ in range(steps):
+        psteps += o.count()
         o.move(V)
         Vp = o.project()
         o.correct(V, Vp)
-        psteps += n
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
     dt = time.perf_counter() - t0
+    o.close()
     m = cfg["mesh"]
     return {"value": psteps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "%s at 1/%d per direction (%s cells, %d particles), %d steps, %.1f s, oracle reference-order "
-                      "mode, g++ -O2 -ffp-contract=off" % (workload, int(round(1 / scale)),
-                                                           "x".join(str(a) for a in m["n"]),
-                                                           len(cfg["cells"]), done, dt)}
+            "sample": "%s full size (%s cells, %d particles seeded), %d full step(s) in %.1f s, oracle "
+                      "reference-order mode, g++ -O2 -ffp-contract=off, one thread"
+                      % (workload, "x".join(str(a) for a in m["n"]), n_seeded, steps, dt),
+            "cpu_model": cpu_model(), "host_nproc": os.cpu_count()}
 
 
 def batched_comparison(steps, warmup, n_inst=256, scale=1.0, log=None, force_tune=False):
@@ -354,6 +366,125 @@ def wire_cost(workload="taylor_green2d", scale=1.0, reps=3):
             "unpack_frac": round(unpack_b / (unpack_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def plain_run(workload, scale, steps, warmup, autotune=False):
+    """Run A: `warmup` + `steps` steps of the plain handle from the seeded
+    state, HIP events around the timed steps on the handle's stream
+    (particles counted on the device at the start of each step). Run B: the
+    same steps from the same state with HIP events around every launch, for
+    the per-kernel durations and the roofline of the dominant kernel
+    (algorithmic bytes per launch / average launch duration; `traffic` from
+    the committed counter summary)."""
+    import torch
+    from pfem2dealii_amd import ParticleHandler, workloads as W
+
+    t_setup = time.perf_counter()
+    cfg = W.config(workload, scale=scale)
+    mesh = cfg["mesh"]
+    dim = mesh["dim"]
+    V = torch.tensor(cfg["V"], dtype=torch.float64, device="cuda")
+    Vp = torch.empty_like(V)
+
+    def make():
+        h = ParticleHandler(mesh, cfg["params"])
+        h.initialize_maps()
+        assert h.insert_particles(cfg["x"], cfg["cells"]) == 0
+        h.set_velocity_from_field(V)
+        # capacity for the particle growth of the measured steps (refills),
+        # so that no step reallocates device arrays
+        h.reserve_particles(3 * len(cfg["cells"]))
+        return h
+
+    def step(h):
+        h.move_particles(V)
+        h.project_particle_fields(Vp)
+        h.correct_particle_velocity(V, Vp)
+
+    from pfem2dealii_amd.autotune import shipped, tune
+    sig = {"workload": workload, "scale": scale}
+    tuning = None if autotune else shipped(sig)
+    if tuning is None:  # on a scratch handle: tuning steps advance the simulation
+        ht = make()
+        tuning, _ = tune(ht, lambda: step(ht), signature=sig, force=True)
+        ht.close()
+    h = make()
+    h.set_tuning(**tuning)
+    n0 = h.count()
+    stream = torch.cuda.ExternalStream(h.stream())
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t_setup
+
+    # A. timed run: warmup steps, then `steps` steps between two HIP events
+    for _ in range(warmup):
+        step(h)
+    h.synchronize()
+    counts = torch.zeros(steps, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    h.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for s in range(steps):
+        h.count_async(counts.data_ptr() + 4 * s)
+        step(h)
+    ev1.record(stream)
+    h.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    psteps = int(counts.sum().item())
+    ms_per_step = gpu_ms / steps
+    value = psteps / (gpu_ms * 1e-3)
+    h.close()
+
+    # B. the same warmup + timed steps again from the same initial state with
+    #    HIP events around every launch on the handle's stream: per-kernel
+    #    durations over exactly the launches of run A (a kernel trace of this
+    #    program sees runs A and B, which launch identical work)
+    h = make()
+    h.set_tuning(**tuning)
+    nb_steps = warmup + steps
+    pcounts = torch.zeros(nb_steps + 1, dtype=torch.int32, device="cuda")
+    h.set_timing(True)
+    for s in range(nb_steps):
+        h.count_async(pcounts.data_ptr() + 4 * s)
+        step(h)
+    h.count_async(pcounts.data_ptr() + 4 * nb_steps)
+    tim = h.timings()
+    h.set_timing(False)
+    h.close()
+    pcl = pcounts.cpu().tolist()
+    nodes = mesh["X"].shape[1]
+    b_alg = alg_bytes_per_particle_step(dim, nodes, psteps / steps)
+    achieved_step = b_alg * psteps / steps / (ms_per_step * 1e-3) / 1e9
+    # kernels only: the reference's timer sections ("Particle ...") bracket whole calls
+    kern = {k: {"launches": n, "avg_ms": ms / n, "ms_per_step": ms / nb_steps} for k, (n, ms) in tim.items()
+            if not k.startswith("Particle ")}
+    dom = max(kern.items(), key=lambda kv: kv[1]["ms_per_step"]) if kern else ("none", {"avg_ms": 0.0, "launches": 0})
+    n_in = sum(pcl[:-1]) / float(nb_steps)
+    n_out = sum(pcl[1:]) / float(nb_steps)
+    alg = kernel_alg_bytes(dim, nodes, n_in, n_out)
+    dname = dom[0]
+    dbytes = alg.get(dname)
+    per_launch = dbytes / (dom[1]["launches"] / float(nb_steps)) if dbytes is not None else None
+    achieved = per_launch / (dom[1]["avg_ms"] * 1e-3) / 1e9 if per_launch else None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                "traffic": hbm_traffic(workload, scale, dname), "kernel": dname,
+                "alg_bytes_per_launch": per_launch, "avg_launch_ms": round(dom[1]["avg_ms"], 4),
+                "launches": dom[1]["launches"],
+                "step": {"achieved": round(achieved_step, 1), "frac": round(achieved_step / HBM_PEAK_GBS, 4),
+                         "alg_bytes_per_particle_step": round(b_alg, 2), "particles_per_step": psteps / steps}}
+    return {"value": value, "unit": "particle-steps/s", "ms_per_step": ms_per_step,
+            "config": {"workload": workload if scale == 1.0 else "%s@%g" % (workload, scale),
+                       "mesh": "x".join(str(a) for a in mesh["n"]), "particles_at_start": n0,
+                       "n_sub": cfg["params"]["n_sub"], "dt": cfg["params"]["dt"], "parallelism": "1 GPU",
+                       "form": "plain", "tuning": tuning},
+            "roofline": roofline,
+            "kernels_ms_per_step": {k: round(v["ms_per_step"], 4) for k, v in sorted(kern.items())},
+            "wall_s": round(wall, 3), "setup_s": round(setup_s, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -362,7 +493,8 @@ def main():
     ap.add_argument("--workload", default="taylor_green2d")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batched-steps", type=int, default=5, help="steps of the 256-instance comparison (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=20, help="steps of the 256-instance comparison (0: skip)")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 128^3 ABC leg")
     ap.add_argument("--adjoint-reps", type=int, default=3, help="repetitions of the adjoint cost measurement (0: skip)")
     ap.add_argument("--autotune", action="store_true", help="tune launch shapes instead of the shipped table")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -394,121 +526,18 @@ def main():
               flush=True)
         return
 
-    import torch
-    from pfem2dealii_amd import ParticleHandler, workloads as W
-
-    t_setup = time.perf_counter()
-    cfg = W.config(args.workload, scale=args.scale)
-    mesh = cfg["mesh"]
-    dim = mesh["dim"]
-    V = torch.tensor(cfg["V"], dtype=torch.float64, device="cuda")
-    Vp = torch.empty_like(V)
-
-    def make():
-        h = ParticleHandler(mesh, cfg["params"])
-        h.initialize_maps()
-        assert h.insert_particles(cfg["x"], cfg["cells"]) == 0
-        h.set_velocity_from_field(V)
-        # capacity for the particle growth of the measured steps (refills),
-        # so that no step reallocates device arrays
-        h.reserve_particles(3 * len(cfg["cells"]))
-        return h
-
-    def step(h):
-        h.move_particles(V)
-        h.project_particle_fields(Vp)
-        h.correct_particle_velocity(V, Vp)
-
-    from pfem2dealii_amd.autotune import shipped, tune
-    sig = {"workload": args.workload, "scale": args.scale}
-    tuning = None if args.autotune else shipped(sig)
-    if tuning is None:  # on a scratch handle: tuning steps advance the simulation
-        ht = make()
-        tuning, _ = tune(ht, lambda: step(ht), signature=sig, force=True)
-        ht.close()
-    h = make()
-    h.set_tuning(**tuning)
-    n0 = h.count()
-    stream = torch.cuda.ExternalStream(h.stream())
-    torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t_setup
-
-    # A. timed run: warmup steps, then `steps` steps between two HIP events
-    for _ in range(args.warmup):
-        step(h)
-    h.synchronize()
-    counts = torch.zeros(args.steps, dtype=torch.int32, device="cuda")
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    h.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for s in range(args.steps):
-        h.count_async(counts.data_ptr() + 4 * s)
-        step(h)
-    ev1.record(stream)
-    h.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = ev0.elapsed_time(ev1)
-    psteps = int(counts.sum().item())
-    ms_per_step = gpu_ms / args.steps
-    value = psteps / (gpu_ms * 1e-3)
-    h.close()
-
-    # B. the same warmup + timed steps again from the same initial state with
-    #    HIP events around every launch on the handle's stream: per-kernel
-    #    durations over exactly the launches of run A (a kernel trace of this
-    #    program sees runs A and B, which launch identical work)
-    h = make()
-    h.set_tuning(**tuning)
-    nb_steps = args.warmup + args.steps
-    pcounts = torch.zeros(nb_steps + 1, dtype=torch.int32, device="cuda")
-    h.set_timing(True)
-    for s in range(nb_steps):
-        h.count_async(pcounts.data_ptr() + 4 * s)
-        step(h)
-    h.count_async(pcounts.data_ptr() + 4 * nb_steps)
-    tim = h.timings()
-    h.set_timing(False)
-    h.close()
-    del cfg["x"], cfg["cells"]
-    pcl = pcounts.cpu().tolist()
-    nodes = mesh["X"].shape[1]
-    b_alg = alg_bytes_per_particle_step(dim, nodes, psteps / args.steps)
-    achieved_step = b_alg * psteps / args.steps / (ms_per_step * 1e-3) / 1e9
-    # kernels only: the reference's timer sections ("Particle ...") bracket whole calls
-    kern = {k: {"launches": n, "avg_ms": ms / n, "ms_per_step": ms / nb_steps} for k, (n, ms) in tim.items()
-            if not k.startswith("Particle ")}
-    dom = max(kern.items(), key=lambda kv: kv[1]["ms_per_step"]) if kern else ("none", {"avg_ms": 0.0, "launches": 0})
-    n_in = sum(pcl[:-1]) / float(nb_steps)
-    n_out = sum(pcl[1:]) / float(nb_steps)
-    alg = kernel_alg_bytes(dim, nodes, n_in, n_out)
-    dname = dom[0]
-    dbytes = alg.get(dname)
-    per_launch = dbytes / (dom[1]["launches"] / float(nb_steps)) if dbytes is not None else None
-    achieved = per_launch / (dom[1]["avg_ms"] * 1e-3) / 1e9 if per_launch else None
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                "traffic": hbm_traffic(args.workload, args.scale, dname), "kernel": dname,
-                "alg_bytes_per_launch": per_launch, "avg_launch_ms": round(dom[1]["avg_ms"], 4),
-                "launches": dom[1]["launches"],
-                "step": {"achieved": round(achieved_step, 1), "frac": round(achieved_step / HBM_PEAK_GBS, 4),
-                         "alg_bytes_per_particle_step": round(b_alg, 2), "particles_per_step": psteps / args.steps}}
+    plain = plain_run(args.workload, args.scale, args.steps, args.warmup, args.autotune)
     out = {
         "metric": "particle-steps/s plain vs batched-256 on 1 GPU at 1e-11 rel err; %HBM peak",
-        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": args.workload if args.scale == 1.0 else "%s@%g" % (args.workload, args.scale),
-                   "mesh": "x".join(str(a) for a in mesh["n"]), "particles_at_start": n0,
-                   "n_sub": cfg["params"]["n_sub"], "dt": cfg["params"]["dt"], "parallelism": "1 GPU",
-                   "form": "plain", "tuning": tuning},
-        "roofline": roofline,
-        "kernels_ms_per_step": {k: round(v["ms_per_step"], 4) for k, v in sorted(kern.items())},
-        "wall_s": round(wall, 3), "setup_s": round(setup_s, 1),
+        "value": plain["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": plain["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic", "config": plain["config"], "roofline": plain["roofline"],
+        "kernels_ms_per_step": plain["kernels_ms_per_step"], "wall_s": plain["wall_s"], "setup_s": plain["setup_s"],
     }
+    if args.workload == "taylor_green2d" and args.scale == 1.0 and not args.no_3d:
+        d3 = plain_run("abc3d", 1.0, args.steps, args.warmup, args.autotune)
+        out["abc3d"] = {k: d3[k] for k in ("value", "unit", "ms_per_step", "config", "roofline",
+                                            "kernels_ms_per_step")}
     if args.batched_steps > 0:
         out["batched"] = batched_comparison(args.batched_steps, 1, force_tune=args.autotune)
     if args.adjoint_reps > 0:
@@ -518,7 +547,8 @@ def main():
         out["wire"] = wire_cost(args.workload, args.scale, args.wire_reps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload)
-        out["cpu_baseline"]["host"] = {"nproc": os.cpu_count()}
+        if "abc3d" in out:
+            out["abc3d"]["cpu_baseline"] = cpu_baseline("abc3d")
     print(json.dumps(out), flush=True)
 
 

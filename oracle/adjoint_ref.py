@@ -133,6 +133,7 @@ class FrozenModel:
         if record:
             self.paths = []
         x, v, c = self.x0.copy(), np.array(v0, dtype=np.float64).copy(), self.c0.copy()
+        placed = None  # inserted particles: placed when at the exact image of their part centre
         for s, (src, ec) in enumerate(zip(self.srcs, self.end_cells)):
             alive = np.ones(c.shape[0], dtype=bool)
             cc = c.copy()
@@ -144,7 +145,7 @@ class FrozenModel:
                 X0, h = self._X0h(cs)
                 xi = (x - X0) / h
                 if k == 0:
-                    xi = self._snap(x, cs, xi)
+                    xi = self._snap(x, cs, xi) if placed is None else self._centre(xi, placed)
                 if record:
                     path.append((cc.copy(), xi.copy(), alive.copy()))
                 u = self._interp(V, cs, xi)
@@ -176,10 +177,19 @@ class FrozenModel:
                 xn[:, ref] = self._fwd(cr, xi)
                 vn[:, ref] = self._interp(V, cr, xi)
             x, v, c = xn, vn, ec.copy()
-        return x, v, c
+            placed = ~surv  # refills sit at their part centre until located
+        return x, v, c, placed
+
+    def _centre(self, xi, placed):
+        """the exact centre of the part holding xi where `placed` (a refill's
+        stored reference position), xi elsewhere"""
+        pk = [(xi[k] * self.q[k]).astype(np.int64) for k in range(self.D)]
+        xc = np.stack([(pk[k] + 0.5) * self.hp[k] for k in range(self.D)])
+        return np.where(placed, xc, xi)
 
     def _snap(self, x, c, xi):
-        """exact sub-cell centre for particles placed there (refills)"""
+        """exact sub-cell centre for inserted particles placed exactly at its
+        image (the insertion rule)"""
         D = self.D
         pk = [(xi[k] * self.q[k]).astype(np.int64) for k in range(D)]
         xc = np.stack([(pk[k] + 0.5) * self.hp[k] for k in range(D)])
@@ -189,11 +199,11 @@ class FrozenModel:
             hit &= (F[k] == x[k]) & (pk[k] >= 0) & (pk[k] < self.q[k])
         return np.where(hit, xc, xi)
 
-    def project(self, x, v, c):
+    def project(self, x, v, c, placed=None):
         D = self.D
         X0, h = self._X0h(c)
         xi = (x - X0) / h
-        xi = self._snap(x, c, xi)
+        xi = self._snap(x, c, xi) if placed is None else self._centre(xi, placed)
         N = _shape(xi)
         nd = self.X.shape[1]
         S = np.zeros((D, nd))
@@ -201,21 +211,21 @@ class FrozenModel:
         for i in range(self.NV):
             d = self.cv[c, i]
             for k in range(D):
-                np.add.at(S[k], d, N[i] * v[k])
-            np.add.at(W, d, N[i])
+                S[k] += np.bincount(d, weights=N[i] * v[k], minlength=nd)
+            W += np.bincount(d, weights=N[i], minlength=nd)
         return S / W, W, xi
 
     def loss(self, V, v0, target):
-        x, v, c = self.forward(V, v0, record=self.paths is None)
-        P, _, _ = self.project(x, v, c)
+        x, v, c, placed = self.forward(V, v0, record=self.paths is None)
+        P, _, _ = self.project(x, v, c, placed)
         r = P - target
         return 0.5 * float((r * r).sum())
 
     def grad(self, V, v0, target):
         """(L, dL/dv0, dL/dV) by the reverse sweep over the recorded path."""
         D = self.D
-        x, v, c = self.forward(V, v0, record=True)
-        P, W, xi = self.project(x, v, c)
+        x, v, c, placed = self.forward(V, v0, record=True)
+        P, W, xi = self.project(x, v, c, placed)
         r = P - target
         L = 0.5 * float((r * r).sum())
         N = _shape(xi)
@@ -250,7 +260,7 @@ class FrozenModel:
                 for i in range(self.NV):
                     dd = self.cv[cr, i]
                     for k in range(D):
-                        np.add.at(gV[k], dd, Nr[i] * av[k, ref])
+                        gV[k] += np.bincount(dd, weights=Nr[i] * av[k, ref], minlength=nd)
             alive_end = path[self.n_sub][2]
             a = np.where(alive_end, xs, 0.0)
             for k in range(self.n_sub - 1, -1, -1):
@@ -264,7 +274,7 @@ class FrozenModel:
                     dd = self.cv[cc, i]
                     m = alive_end
                     for kk in range(D):
-                        np.add.at(gV[kk], dd[m], self.dt * Nk[i][m] * a[kk][m])
+                        gV[kk] += np.bincount(dd[m], weights=self.dt * Nk[i][m] * a[kk][m], minlength=nd)
                     av_i = a[0] * V[0][dd]
                     for kk in range(1, D):
                         av_i = av_i + a[kk] * V[kk][dd]

@@ -465,6 +465,8 @@ int device_error(int flags) {
     return fail(PFEM2GPU_ERR_HIP, "redistribution look-back wait timed out; particle layout is invalid");
   if (flags & kErrQueue)
     return fail(PFEM2GPU_ERR_STATE, "a walk-queue stripe overflowed; particles were dropped");
+  if (flags & kErrRange)
+    return fail(PFEM2GPU_ERR_ARG, "an adjoint term was not finite or exceeded 2^46 in magnitude");
   return PFEM2GPU_OK;
 }
 
@@ -1595,17 +1597,20 @@ int adjoint_impl(pfem2gpu_handle* h, int n_steps, int K, const double* const* Vu
   }
   // final projection, loss and the projection adjoint
   double *Pn[3] = {nullptr, nullptr, nullptr}, *W = nullptr, *dloss = nullptr;
+  unsigned long long* floss = nullptr;  // the loss as a fixed-point sum
   for (int k = 0; k < D; ++k) CHK(mem.get(&Pn[k], nd));
   CHK(mem.get(&W, nd));
   CHK(mem.get(&dloss, 1));
+  CHK(mem.get(&floss, 2));
   const DMesh m = dmesh(h);
   LAUNCH(h, "k_contrib_dirty", k_contrib_dirty<D>, std::min((h->nchunks + kBlock - 1) / kBlock, 4096), 0, m, nc, h->cpw, h->dirty,
          h->cell_start, dparts(h), h->contrib);
   LAUNCH(h, "k_node_w", k_node_w<D>, grid_for(nd), 0, nd, h->d2c_off, h->d2c, h->contrib, Pn[0], Pn[1], Pn[2], W);
-  LAUNCH(h, "k_fill", k_fill_double, 1, 0, dloss, 1L);
+  LAUNCH(h, "k_fill", k_fill_u64, 1, 0, floss, 2L);
   DField Pf;
   for (int k = 0; k < 3; ++k) Pf.c[k] = Pn[k];
-  LAUNCH(h, "k_loss", k_loss<D>, std::min(grid_for(nd), 1024), 0, nd, Pf, Tg, dloss);
+  LAUNCH(h, "k_loss", k_loss<D>, std::min(grid_for(nd), 1024), 0, nd, Pf, Tg, floss, h->d_err);
+  LAUNCH(h, "k_loss", k_fix_to_double, 1, 0, 1L, floss, dloss);
   long nmax = 1;
   for (long c : cnt) nmax = std::max(nmax, c);
   double *xa[3] = {nullptr, nullptr, nullptr}, *va[3] = {nullptr, nullptr, nullptr};
@@ -1619,9 +1624,9 @@ int adjoint_impl(pfem2gpu_handle* h, int n_steps, int K, const double* const* Vu
   LAUNCH(h, "k_adj_init", k_adj_init<D>, grid_for(cnt[n_steps]), 0, m, h->cell_start + nc, dparts(h), Pf, W, Tg,
          xa[0], xa[1], xa[2], va[0], va[1], va[2]);
   for (int k = 0; k < D; ++k) LAUNCH(h, "k_fill", k_fill_double, grid_for(nd), 0, gV[k], (long)nd);
-  double* gcv = nullptr;  // dL/dV per (cell, vertex), reduced to the DoFs at the end
-  CHK(mem.get(&gcv, (size_t)nc * h->nvc * D));
-  LAUNCH(h, "k_fill", k_fill_double, grid_for((long)nc * h->nvc * D), 0, gcv, (long)nc * h->nvc * D);
+  unsigned long long* gcv = nullptr;  // dL/dV per (cell, vertex) in fixed point, reduced to the DoFs at the end
+  CHK(mem.get(&gcv, 2 * (size_t)nc * h->nvc * D));
+  LAUNCH(h, "k_fill", k_fill_u64, grid_for(2L * nc * h->nvc * D), 0, gcv, 2L * nc * h->nvc * D);
   // backward, one checkpoint segment at a time (latest first): recompute the
   // segment's moves from its checkpoint (all but the last segment, whose
   // paths the forward recorded), recording every particle's path, then
@@ -1638,9 +1643,9 @@ int adjoint_impl(pfem2gpu_handle* h, int n_steps, int K, const double* const* Vu
     for (int s = s1; s >= s0; --s) {
       LAUNCH(h, "k_fill", k_fill_adj, grid_for(cnt[s - 1]), 0, cnt[s - 1], xb[0], xb[1], xb[2], vb[0], vb[1], vb[2]);
       LAUNCH(h, "k_adj_unmap", k_adj_unmap<D>, grid_for(cnt[s]), 0, m, h->pshift, (int)cnt[s], srcs[s], ecell[s],
-             xa[0], xa[1], xa[2], va[0], va[1], va[2], xb[0], xb[1], xb[2], vb[0], vb[1], vb[2], gcv);
+             xa[0], xa[1], xa[2], va[0], va[1], va[2], xb[0], xb[1], xb[2], vb[0], vb[1], vb[2], gcv, h->d_err);
       LAUNCH(h, "k_adj_move", k_adj_move<D>, grid_for(cnt[s - 1]), 0, m, h->Vc, (int)cnt[s - 1], max_sub,
-             paths[s - s0], xb[0], xb[1], xb[2], gcv);
+             paths[s - s0], xb[0], xb[1], xb[2], gcv, h->d_err);
       for (int k = 0; k < D; ++k) {
         std::swap(xa[k], xb[k]);
         std::swap(va[k], vb[k]);

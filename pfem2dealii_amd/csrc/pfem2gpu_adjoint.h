@@ -10,8 +10,10 @@
 // W_j = sum_p N_j, and the loss L = 1/2 |P - target|^2.
 // Reverse: the projection's scatter becomes a gather (k_adj_init), the
 // interpolation's gather becomes a scatter into a per-(cell, vertex)
-// accumulator (k_adj_move, k_adj_unmap; FP64 atomics after aggregation over
-// each wavefront), gathered into dL/dV per DoF at the end (k_adj_gather).
+// accumulator (k_adj_move, k_adj_unmap; integer atomics on 128-bit fixed-point
+// values after aggregation over each wavefront), gathered into dL/dV per DoF
+// at the end (k_adj_gather). Integer sums do not depend on the order of the
+// additions, so the gradient and the loss are the same bits on every run.
 #pragma once
 
 #include "pfem2gpu_kernels.h"
@@ -19,6 +21,59 @@
 namespace pfem2gpu {
 
 constexpr int kAdjMaxSub = 8;
+
+// 128-bit two's-complement fixed-point accumulator of a double sum, as two
+// 64-bit words (low, high) of value (hi * 2^64 + lo) * 2^-kFixBits. A term is
+// rounded once (toward zero, at 2^-80) when it is converted; the additions
+// are exact integer additions, so the result does not depend on their order.
+// Range: |sum| < 2^(127 - kFixBits) = 2^47.
+constexpr int kFixBits = 80;
+
+// Returns false (and converts 0) for a non-finite term or one of magnitude
+// 2^46 or more, which the accumulator cannot hold.
+__device__ __forceinline__ bool fix_from(double x, unsigned long long& lo, unsigned long long& hi) {
+  double a = fabs(x) * 0x1p80;                     // exact (a power-of-two scale)
+  const bool ok = a < 0x1p126;                     // false for NaN too
+  if (!ok) a = 0.0;
+  const double hd = floor(a * 0x1p-64);            // exact
+  const double r = a - hd * 0x1p64;                // exact, in [0, 2^64)
+  lo = (unsigned long long)r;
+  hi = (unsigned long long)hd;
+  if (x < 0.0) {  // two's complement negation
+    lo = ~lo + 1ULL;
+    hi = ~hi + (lo == 0ULL ? 1ULL : 0ULL);
+  }
+  return ok;
+}
+
+__device__ __forceinline__ double fix_to(unsigned long long lo, unsigned long long hi) {
+  return ((double)(long long)hi * 0x1p64 + (double)lo) * 0x1p-80;
+}
+
+// acc[0..1] += (lo, hi): a wrap of the low word carries into the high word;
+// the number of wraps is that of the total sum, whatever the order.
+__device__ __forceinline__ void fix_add_words(unsigned long long* acc, unsigned long long lo, unsigned long long hi) {
+  const unsigned long long old = atomicAdd(&acc[0], lo);
+  const unsigned long long carry = (old + lo < old) ? 1ULL : 0ULL;
+  if (hi + carry != 0ULL) atomicAdd(&acc[1], hi + carry);
+}
+
+__device__ __forceinline__ bool fix_add(unsigned long long* acc, double x) {
+  unsigned long long lo, hi;
+  const bool ok = fix_from(x, lo, hi);
+  fix_add_words(acc, lo, hi);
+  return ok;
+}
+
+// dst[i] = the double value of the fixed-point accumulator src[2i..2i+1]
+__global__ void k_fix_to_double(long n, const unsigned long long* __restrict__ src, double* __restrict__ dst) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    dst[i] = fix_to(src[2 * i], src[2 * i + 1]);
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ a, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] = 0ULL;
+}
 
 __global__ void k_fill_double(double* __restrict__ a, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] = 0.0;
@@ -46,9 +101,11 @@ __global__ __launch_bounds__(kBlock) void k_node_w(int ndofs, const int* __restr
   }
 }
 
-// L = 1/2 sum_{k,j} (P_k[j] - T_k[j])^2 (block partial sums, one atomic each)
+// L = 1/2 sum_{k,j} (P_k[j] - T_k[j])^2: block partial sums in a fixed tree,
+// added to the fixed-point accumulator out[0..1]
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_loss(int ndofs, DField P, DField T, double* __restrict__ out) {
+__global__ __launch_bounds__(kBlock) void k_loss(int ndofs, DField P, DField T, unsigned long long* __restrict__ out,
+                                                int* __restrict__ err) {
   __shared__ double part[kBlock];
   double s = 0.0;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ndofs; j += gridDim.x * blockDim.x)
@@ -63,7 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_loss(int ndofs, DField P, DField T, 
     if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(out, 0.5 * part[0]);
+  if (threadIdx.x == 0 && !fix_add(out, 0.5 * part[0])) atomicOr(err, kErrRange);
 }
 
 // Post-step state of a replayed move from the maps the forward recorded:
@@ -210,30 +267,38 @@ __global__ __launch_bounds__(kBlock) void k_adj_init(DMesh m, const int* __restr
   }
 }
 
-// dL/dV[k][j] += sum over the (cell, vertex) entries of DoF j of the
-// per-(cell, vertex) accumulator (ascending cell, as the projection sums).
+// dL/dV[k][j] = the sum over the (cell, vertex) entries of DoF j of the
+// per-(cell, vertex) fixed-point accumulators (an exact integer sum,
+// converted once).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_adj_gather(int ndofs, const int* __restrict__ d2c_off,
-                                                      const int* __restrict__ d2c, const double* __restrict__ gcv,
-                                                      double* g0, double* g1, double* g2) {
+                                                      const int* __restrict__ d2c,
+                                                      const unsigned long long* __restrict__ gcv, double* g0,
+                                                      double* g1, double* g2) {
   double* g[3] = {g0, g1, g2};
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ndofs; j += gridDim.x * blockDim.x) {
-    double S[D];
+    unsigned long long lo[D], hi[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) S[k] = 0.0;
+    for (int k = 0; k < D; ++k) lo[k] = hi[k] = 0ULL;
     for (int e = d2c_off[j]; e < d2c_off[j + 1]; ++e)
 #pragma unroll
-      for (int k = 0; k < D; ++k) S[k] = S[k] + gcv[(size_t)d2c[e] * D + k];
+      for (int k = 0; k < D; ++k) {
+        const unsigned long long* a = gcv + 2 * ((size_t)d2c[e] * D + k);
+        const unsigned long long l = lo[k] + a[0];
+        hi[k] = hi[k] + a[1] + (l < lo[k] ? 1ULL : 0ULL);
+        lo[k] = l;
+      }
 #pragma unroll
-    for (int k = 0; k < D; ++k) g[k][j] = g[k][j] + S[k];
+    for (int k = 0; k < D; ++k) g[k][j] = fix_to(lo[k], hi[k]);
   }
 }
 
 // Scatter of per-lane contributions add[i][k] into the per-(cell, vertex)
-// accumulator gcv of the lane's cell (key < 0: nothing), aggregated over the
-// lanes of the wavefront that share the cell: up to kAdjGroups distinct
-// cells are summed in LDS and flushed with one global atomic per (cell,
-// vertex, component); lanes beyond that add directly. Wave-uniform.
+// fixed-point accumulator gcv (two words per value) of the lane's cell
+// (key < 0: nothing), aggregated over the lanes of the wavefront that share
+// the cell: up to kAdjGroups distinct cells are summed in LDS (fixed point
+// too) and flushed with one global addition per (cell, vertex, component);
+// lanes beyond that add directly. Wave-uniform.
 constexpr int kAdjGroups = 16;
 
 // Orders this wavefront's LDS accesses around the point (waits for its LDS
@@ -244,8 +309,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 template <int D>
-__device__ __forceinline__ void adj_scatter(int key, const double (*add)[3], double* __restrict__ gcv, int* s_gcell,
-                                            double* s_acc) {
+__device__ __forceinline__ void adj_scatter(int key, const double (*add)[3], unsigned long long* __restrict__ gcv,
+                                            int* s_gcell, unsigned long long* s_acc, int* __restrict__ err) {
+  bool ok = true;
   constexpr int NV = 1 << D;
   constexpr int W = NV * D;
   const int lane = threadIdx.x & 63;
@@ -260,23 +326,24 @@ __device__ __forceinline__ void adj_scatter(int key, const double (*add)[3], dou
     pending &= ~grp;
     ++ng;
   }
-  for (int l = lane; l < ng * W; l += 64) s_acc[l] = 0.0;
+  for (int l = lane; l < 2 * ng * W; l += 64) s_acc[l] = 0ULL;
   wave_lds_sync();
   if (gid >= 0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
-      for (int k = 0; k < D; ++k) atomicAdd(&s_acc[gid * W + i * D + k], add[i][k]);
+      for (int k = 0; k < D; ++k) ok = fix_add(&s_acc[2 * (gid * W + i * D + k)], add[i][k]) && ok;
   } else if (key >= 0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
-      for (int k = 0; k < D; ++k) atomicAdd(&gcv[((size_t)key * NV + i) * D + k], add[i][k]);
+      for (int k = 0; k < D; ++k) ok = fix_add(&gcv[2 * (((size_t)key * NV + i) * D + k)], add[i][k]) && ok;
   }
+  if (!ok) atomicOr(err, kErrRange);
   wave_lds_sync();
   for (int l = lane; l < ng * W; l += 64) {
     const int gi = l / W;
-    atomicAdd(&gcv[(size_t)s_gcell[gi] * W + (l - gi * W)], s_acc[l]);
+    fix_add_words(&gcv[2 * ((size_t)s_gcell[gi] * W + (l - gi * W))], s_acc[2 * l], s_acc[2 * l + 1]);
   }
   wave_lds_sync();
 }
@@ -293,10 +360,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_unmap(DMesh m, int pshift, int n
                                                      const double* xe1, const double* xe2, const double* ve0,
                                                      const double* ve1, const double* ve2, double* xs0, double* xs1,
                                                      double* xs2, double* vs0, double* vs1, double* vs2,
-                                                     double* __restrict__ gcv) {
+                                                     unsigned long long* __restrict__ gcv, int* __restrict__ err) {
   constexpr int NV = 1 << D;
   __shared__ int s_gcell[kBlock / 64][kAdjGroups];
-  __shared__ double s_acc[kBlock / 64][kAdjGroups * NV * D];
+  __shared__ unsigned long long s_acc[kBlock / 64][2 * kAdjGroups * NV * D];
   const int wave = threadIdx.x >> 6;
   const double* xe[3] = {xe0, xe1, xe2};
   const double* ve[3] = {ve0, ve1, ve2};
@@ -332,7 +399,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_unmap(DMesh m, int pshift, int n
         for (int k = 0; k < D; ++k) add[i][k] = N[i] * ve[k][q];
       key = c;
     }
-    adj_scatter<D>(key, add, gcv, s_gcell[wave], s_acc[wave]);
+    adj_scatter<D>(key, add, gcv, s_gcell[wave], s_acc[wave], err);
   }
 }
 
@@ -351,10 +418,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_unmap(DMesh m, int pshift, int n
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __restrict__ Vc, int n_start, int max_sub,
                                                     DPath path, double* xt0, double* xt1, double* xt2,
-                                                    double* __restrict__ gcv) {
+                                                    unsigned long long* __restrict__ gcv, int* __restrict__ err) {
   constexpr int NV = 1 << D;
   __shared__ int s_gcell[kBlock / 64][kAdjGroups];
-  __shared__ double s_acc[kBlock / 64][kAdjGroups * NV * D];
+  __shared__ unsigned long long s_acc[kBlock / 64][2 * kAdjGroups * NV * D];
   const int wave = threadIdx.x >> 6;
   double* xt[3] = {xt0, xt1, xt2};
   for (long base = (long)blockIdx.x * blockDim.x; base < n_start; base += (long)gridDim.x * blockDim.x) {
@@ -412,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __re
       // component) instead of contending on the same addresses
       const bool fl = act && pc[s] != acell && acell >= 0;
       if (__ballot(fl)) {
-        adj_scatter<D>(fl ? acell : -1, acc, gcv, s_gcell[wave], s_acc[wave]);
+        adj_scatter<D>(fl ? acell : -1, acc, gcv, s_gcell[wave], s_acc[wave], err);
         if (fl) {
 #pragma unroll
           for (int i = 0; i < NV; ++i)
@@ -444,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __re
         for (int d = 0; d < D; ++d) a[d] += dt * gx[d] * cc.RH[d];
       }
     }
-    adj_scatter<D>(acell, acc, gcv, s_gcell[wave], s_acc[wave]);
+    adj_scatter<D>(acell, acc, gcv, s_gcell[wave], s_acc[wave], err);
     if (p < n_start) {
 #pragma unroll
       for (int k = 0; k < D; ++k) xt[k][p] = a[k];

@@ -85,6 +85,7 @@ struct DWalk {
 // Device-side error flags (one int per handle, read back by the host).
 constexpr int kErrLookback = 1;  // a redistribution look-back wait timed out
 constexpr int kErrQueue = 2;     // a walk-queue stripe was too small (particles dropped)
+constexpr int kErrRange = 4;     // an adjoint term outside the fixed-point accumulator's range
 
 // Optional record of every particle's path through a move: the cell and the
 // reference position at the start of each substep, [s * stride + p] for the

@@ -69,14 +69,14 @@ def case(request):
                   [0.2 * np.cos(2 * x0[k]) for k in range(2, cfg["mesh"]["dim"])])
     # target: the projection of a perturbed run along the same frozen path
     m.forward(V, v0, record=True)
-    xt, vt, ct = m.forward(Vt, 1.02 * v0)
-    P, _, _ = m.project(xt, vt, ct)
+    xt, vt, ct, pt = m.forward(Vt, 1.02 * v0)
+    P, _, _ = m.project(xt, vt, ct, pt)
     return {"cfg": cfg, "V": V, "v0": v0, "target": P, "m": m}
 
 
 def test_reference_forward_reproduces_oracle_cells(case):
     m = case["m"]
-    x, v, c = m.forward(case["V"], case["v0"], record=True)
+    x, v, c, _ = m.forward(case["V"], case["v0"], record=True)
     assert np.array_equal(c, m.end_cells[-1])
 
 
@@ -96,8 +96,8 @@ def test_reference_gradient_matches_central_differences(case):
 
 def test_reference_gradient_of_zero_residual_is_zero(case):
     m, V, v0 = case["m"], case["V"], case["v0"]
-    x, v, c = m.forward(V, v0, record=True)
-    P, _, _ = m.project(x, v, c)
+    x, v, c, placed = m.forward(V, v0, record=True)
+    P, _, _ = m.project(x, v, c, placed)
     L, gv0, gV = m.grad(V, v0, P)
     assert L == 0.0 and not gv0.any() and not gV.any()
 
@@ -117,8 +117,8 @@ def test_reference_v0_gradient_is_projection_weight_without_refills():
     v0 = rng.standard_normal(x0.shape)
     T = rng.standard_normal((2, mesh["n_dofs"]))
     L, gv0, gV = m.grad(V, v0, T)
-    x, v, c = m.forward(V, v0)
-    P, Wt, xi = m.project(x, v, c)
+    x, v, c, placed = m.forward(V, v0)
+    P, Wt, xi = m.project(x, v, c, placed)
     r = P - T
     N = np.stack([(1 - xi[0]) * (1 - xi[1]), xi[0] * (1 - xi[1]), (1 - xi[0]) * xi[1], xi[0] * xi[1]])
     want = np.zeros_like(v)

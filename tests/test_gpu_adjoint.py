@@ -176,3 +176,20 @@ def test_adjoint_rejects_non_box_cells_and_too_many_substeps():
     with pytest.raises(Pfem2GpuError, match="substeps"):
         h.adjoint(2, 1, V, V)
     h.close()
+
+
+def test_adjoint_is_deterministic(adjoint_case):
+    """The gradient accumulators are integer (fixed-point) sums, so a second
+    call on the same state returns the same bits."""
+    import torch
+    c = adjoint_case
+    _say("%s: adjoint called again gives the same bits" % c["name"])
+    cfg = c["cfg"]
+    h = _handle(cfg, None, x=c["x0"], cells=c["c0"], v=c["v0"])
+    V = torch.tensor(c["V"], dtype=torch.float64, device="cuda")
+    T = torch.tensor(c["target"], dtype=torch.float64, device="cuda")
+    loss, gv0, gV = h.adjoint(c["steps"], c["ckpt"], V, T)
+    h.close()
+    assert loss == c["loss"]
+    assert np.array_equal(gv0.cpu().numpy().view(np.int64), c["gv0"].view(np.int64))
+    assert np.array_equal(gV.cpu().numpy().view(np.int64), c["gV"].view(np.int64))

@@ -278,6 +278,40 @@ int pfem2gpu_set_timing(pfem2gpu_handle* h, int enable);
 int pfem2gpu_timings(pfem2gpu_handle* h, int cap, const char** names, int64_t* launches, double* total_ms,
                      int* n_out);
 
+/* ---- FEM assembly (pfem2Fem, Q1 with the 2-point Gauss rule per direction
+ * and the Q1 mapping). The global systems are the unconstrained sums of the
+ * cell matrices and right-hand sides over the cells in ascending order, as
+ * AffineConstraints::distribute_local_to_global adds them for unconstrained
+ * DoFs; constrained rows and columns (Dirichlet conditions) are left to the
+ * caller. Matrices are CSR over the pattern pfem2gpu_fem_sparsity returns. */
+
+/* The pattern of the DoFs that share a cell (DoFTools::make_sparsity_pattern
+ * without constraints, src/pfem2fem.cpp:78-95): nnz, row_ptr [n_dofs + 1] and
+ * cols [nnz], ascending per row (host arrays; any may be NULL). Built by the
+ * first call of this function or of an assembly, which synchronises. */
+int pfem2gpu_fem_sparsity(pfem2gpu_handle* h, int64_t* nnz, int64_t* row_ptr, int32_t* cols);
+
+/* assemble_pressure_equation() without SCHEMEB (src/pfem2fem.cpp:392-458):
+ * coeff = rho / time_step,
+ *   A_ij = sum over cells and points of grad N_i . grad N_j JxW
+ *   b_i  = sum of coeff (sum_k V_pred_k(j) d_k N_i) N_j JxW
+ *          - sum over flagged faces of coeff N_i (V_pred . n) JxW.
+ * V_pred: dim arrays [n_dofs]. boundary_faces: [n_cells] bit f set for face f
+ * (deal.II numbering 2 k + side) at a boundary without a pressure Dirichlet
+ * condition, or NULL for none. matrix: [nnz] or NULL to skip the matrix (it
+ * depends on the mesh only). rhs: [n_dofs]. `where` applies to every array. */
+int pfem2gpu_assemble_pressure_equation(pfem2gpu_handle* h, const double* const* V_pred,
+                                        const uint8_t* boundary_faces, double rho, double time_step, double* matrix,
+                                        double* rhs, int where);
+
+/* assemble_velocity_correction() without SCHEMEB (src/pfem2fem.cpp:460-517):
+ *   M_ij  = sum of rho N_i N_j JxW (the matrix of every component)
+ *   b_i^k = sum of rho N_i N_j JxW V_pred_k(j) - time_step N_i JxW d_k N_j P(j).
+ * P: [n_dofs]. matrix: [nnz] or NULL. rhs: dim arrays [n_dofs]. */
+int pfem2gpu_assemble_velocity_correction(pfem2gpu_handle* h, const double* const* V_pred, const double* P,
+                                          double rho, double time_step, double* matrix, double* const* rhs,
+                                          int where);
+
 #ifdef __cplusplus
 }
 #endif

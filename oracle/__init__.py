@@ -3,4 +3,4 @@
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
 this package; the product library never does.
 """
-from .pyoracle import Oracle, build_oracle, MODE_CANONICAL, MODE_REFERENCE  # noqa: F401
+from .pyoracle import Oracle, FemOracle, build_oracle, MODE_CANONICAL, MODE_REFERENCE  # noqa: F401

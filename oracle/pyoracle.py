@@ -65,6 +65,15 @@ def _load():
         lib.orc_v2c.argtypes = [ctypes.c_void_p, ctypes.c_int, _i, _d]
         lib.orc_key.restype = ctypes.c_uint64
         lib.orc_key.argtypes = [_d, ctypes.c_int]
+        _u8 = ctypes.POINTER(ctypes.c_uint8)
+        lib.orc_fem_pattern.restype = ctypes.c_int64
+        lib.orc_fem_pattern.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _i, _l, _i]
+        lib.orc_fem_pressure.restype = ctypes.c_int
+        lib.orc_fem_pressure.argtypes = [ctypes.c_int, ctypes.c_int64, _d, ctypes.c_int64, _i, _i, ctypes.c_int64, _d,
+                                         _u8, ctypes.c_double, ctypes.c_double, _l, _i, _d, _d]
+        lib.orc_fem_vcorr.restype = ctypes.c_int
+        lib.orc_fem_vcorr.argtypes = [ctypes.c_int, ctypes.c_int64, _d, ctypes.c_int64, _i, _i, ctypes.c_int64, _d,
+                                      _d, ctypes.c_double, ctypes.c_double, _l, _i, _d, _d]
         _lib = lib
     return _lib
 
@@ -201,3 +210,57 @@ class Oracle:
 def key(x):
     x = _f64(x)
     return int(_load().orc_key(_dp(x), x.shape[0]))
+
+
+class FemOracle:
+    """pfem2Fem's pressure-equation and velocity-correction assembly on the
+    CPU (oracle/fem_oracle.cpp) over a mesh dict (dim, X [dim][n_vertices],
+    cell_vertices, cell_dofs, n_dofs). Matrices are CSR over pattern()."""
+
+    def __init__(self, mesh):
+        self.lib = _load()
+        self.dim = int(mesh["dim"])
+        self.X = _f64(mesh["X"])
+        self.cv = np.ascontiguousarray(mesh["cell_vertices"], dtype=np.int32)
+        self.cd = np.ascontiguousarray(mesh.get("cell_dofs", mesh["cell_vertices"]), dtype=np.int32)
+        self.nd = int(mesh["n_dofs"])
+        self.nc = self.cv.shape[0]
+        self.row_ptr = np.zeros(self.nd + 1, dtype=np.int64)
+        nnz = self.lib.orc_fem_pattern(self.dim, self.nc, self.nd, _ip(self.cd), self.row_ptr.ctypes.data_as(_l), None)
+        self.cols = np.empty(nnz, dtype=np.int32)
+        self.lib.orc_fem_pattern(self.dim, self.nc, self.nd, _ip(self.cd), self.row_ptr.ctypes.data_as(_l),
+                                 _ip(self.cols))
+
+    def pattern(self):
+        return self.row_ptr, self.cols
+
+    def pressure(self, V, rho, dt, faces=None):
+        V = _f64(V)
+        vals = np.empty(self.cols.shape[0])
+        rhs = np.empty(self.nd)
+        f = None
+        if faces is not None:
+            faces = np.ascontiguousarray(faces, dtype=np.uint8)
+            f = faces.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        rc = self.lib.orc_fem_pressure(self.dim, self.X.shape[1], _dp(self.X), self.nc, _ip(self.cv), _ip(self.cd),
+                                       self.nd, _dp(V), f, float(rho), float(dt), self.row_ptr.ctypes.data_as(_l),
+                                       _ip(self.cols), _dp(vals), _dp(rhs))
+        assert rc == 0
+        return vals, rhs
+
+    def velocity_correction(self, V, P, rho, dt):
+        V, P = _f64(V), _f64(P)
+        vals = np.empty(self.cols.shape[0])
+        rhs = np.empty((self.dim, self.nd))
+        rc = self.lib.orc_fem_vcorr(self.dim, self.X.shape[1], _dp(self.X), self.nc, _ip(self.cv), _ip(self.cd),
+                                    self.nd, _dp(V), _dp(P), float(rho), float(dt), self.row_ptr.ctypes.data_as(_l),
+                                    _ip(self.cols), _dp(vals), _dp(rhs))
+        assert rc == 0
+        return vals, rhs
+
+    def dense(self, vals):
+        A = np.zeros((self.nd, self.nd))
+        for r in range(self.nd):
+            a, b = self.row_ptr[r], self.row_ptr[r + 1]
+            A[r, self.cols[a:b]] = vals[a:b]
+        return A

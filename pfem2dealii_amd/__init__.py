@@ -61,6 +61,7 @@ EXPORTS = [
     "pfem2gpu_output_particle_solution", "pfem2gpu_write_particles_vtu", "pfem2gpu_write_particles_pvtu",
     "pfem2gpu_vertex_to_cells", "pfem2gpu_particle_record_size", "pfem2gpu_pack_particles",
     "pfem2gpu_unpack_particles", "pfem2gpu_pack_particles_noalloc", "pfem2gpu_remove_particles",
+    "pfem2gpu_fem_sparsity", "pfem2gpu_assemble_pressure_equation", "pfem2gpu_assemble_velocity_correction",
 ]
 
 
@@ -118,6 +119,11 @@ def load_library(path=None):
     lib.pfem2gpu_remove_particles.argtypes = [_vp, ctypes.c_int64, _vp]
     lib.pfem2gpu_pack_particles_noalloc.argtypes = [_vp, ctypes.c_int32, _vp, _vp, _vp, ctypes.c_int64, _vp,
                                                     ctypes.c_int]
+    lib.pfem2gpu_fem_sparsity.argtypes = [_vp, _i64p, _vp, _vp]
+    lib.pfem2gpu_assemble_pressure_equation.argtypes = [_vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _vp, _vp,
+                                                        ctypes.c_int]
+    lib.pfem2gpu_assemble_velocity_correction.argtypes = [_vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _vp,
+                                                          _vp, ctypes.c_int]
     if path is None:
         _lib = lib
     return lib
@@ -308,6 +314,95 @@ class ParticleHandler:
         _check(self.lib, self.lib.pfem2gpu_project_particle_fields(self.h, f.arr, f.where),
                "project_particle_fields")
         return out
+
+    # ---- FEM assembly (pfem2Fem, src/pfem2fem.cpp) ----
+
+    def fem_sparsity(self):
+        """CSR pattern of the DoFs that share a cell (src/pfem2fem.cpp:78-95):
+        (row_ptr int64 [n_dofs + 1], cols int32 [nnz], ascending per row)."""
+        nnz = ctypes.c_int64()
+        _check(self.lib, self.lib.pfem2gpu_fem_sparsity(self.h, ctypes.byref(nnz), None, None), "fem_sparsity")
+        rp = np.empty(self.n_dofs + 1, dtype=np.int64)
+        cols = np.empty(nnz.value, dtype=np.int32)
+        _check(self.lib, self.lib.pfem2gpu_fem_sparsity(self.h, None, rp.ctypes.data, cols.ctypes.data),
+               "fem_sparsity")
+        self._fem_nnz = nnz.value
+        return rp, cols
+
+    def _fem_out(self, dev, shape):
+        if dev is not None:
+            import torch
+            return torch.empty(shape, dtype=torch.float64, device=dev)
+        return np.empty(shape, dtype=np.float64)
+
+    def _fem_call(self, dev, fn, *args):
+        """Device arrays: the library reads and writes them on the handle's
+        stream, so torch's stream is drained first and the handle's stream
+        after (the results are complete when this returns)."""
+        if dev is not None:
+            import torch
+            torch.cuda.current_stream(dev).synchronize()
+        rc = fn(*args)
+        if dev is not None and rc == 0:
+            rc = self.lib.pfem2gpu_synchronize(self.h)
+        return rc
+
+    @staticmethod
+    def _ptr(a):
+        if a is None:
+            return None
+        return a.data_ptr() if _is_device(a) else a.ctypes.data
+
+    def assemble_pressure_equation(self, V_pred, rho, time_step, boundary_faces=None, matrix=True):
+        """assemble_pressure_equation (src/pfem2fem.cpp:392-458): returns
+        (values [nnz] over fem_sparsity() or None, rhs [n_dofs]), on the device
+        (torch) when V_pred is a device tensor. boundary_faces: [n_cells] uint8,
+        bit f for face f at a boundary without a pressure Dirichlet condition."""
+        if getattr(self, "_fem_nnz", None) is None:
+            self.fem_sparsity()
+        f = _FieldPtrs(V_pred, self.dim, self.n_dofs)
+        dev = V_pred.device if f.where == DEVICE else None
+        faces = None
+        if boundary_faces is not None:
+            if dev is not None:
+                import torch
+                faces = torch.as_tensor(np.asarray(boundary_faces.cpu() if _is_device(boundary_faces)
+                                                   else boundary_faces, dtype=np.uint8)).to(dev)
+            else:
+                faces = np.ascontiguousarray(boundary_faces, dtype=np.uint8)
+            if tuple(faces.shape) != (self.n_cells,):
+                raise ValueError("boundary_faces must have one entry per cell")
+        vals = self._fem_out(dev, (self._fem_nnz,)) if matrix else None
+        rhs = self._fem_out(dev, (self.n_dofs,))
+        _check(self.lib, self._fem_call(dev, self.lib.pfem2gpu_assemble_pressure_equation, self.h, f.arr,
+                                        self._ptr(faces), float(rho), float(time_step), self._ptr(vals),
+                                        self._ptr(rhs), f.where), "assemble_pressure_equation")
+        return vals, rhs
+
+    def assemble_velocity_correction(self, V_pred, P, rho, time_step, matrix=True):
+        """assemble_velocity_correction (src/pfem2fem.cpp:460-517): returns
+        (values [nnz] or None, rhs [dim, n_dofs]); the matrix is that of every
+        component."""
+        if getattr(self, "_fem_nnz", None) is None:
+            self.fem_sparsity()
+        f = _FieldPtrs(V_pred, self.dim, self.n_dofs)
+        dev = V_pred.device if f.where == DEVICE else None
+        if dev is not None:
+            if not (_is_device(P) and P.is_contiguous() and tuple(P.shape) == (self.n_dofs,)
+                    and str(P.dtype) == "torch.float64"):
+                raise ValueError("P must be a contiguous float64 device tensor [n_dofs] with a device V_pred")
+            Pk = P
+        else:
+            Pk = np.ascontiguousarray(P, dtype=np.float64)
+            if Pk.shape != (self.n_dofs,):
+                raise ValueError("P must be [n_dofs]")
+        vals = self._fem_out(dev, (self._fem_nnz,)) if matrix else None
+        rhs = self._fem_out(dev, (self.dim, self.n_dofs))
+        r = _FieldPtrs(rhs, self.dim, self.n_dofs, writable=True)
+        _check(self.lib, self._fem_call(dev, self.lib.pfem2gpu_assemble_velocity_correction, self.h, f.arr,
+                                        self._ptr(Pk), float(rho), float(time_step), self._ptr(vals), r.arr,
+                                        f.where), "assemble_velocity_correction")
+        return vals, rhs
 
     # ---- readback ----
     def count(self):

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpfem2gpu.so")
 SOURCES = ["pfem2gpu.hip"]
-HEADERS = ["pfem2gpu_device.h", "pfem2gpu_kernels.h", "pfem2gpu_adjoint.h", "pfem2gpu_output.h", "pfem2gpu_wire.h"]
+HEADERS = ["pfem2gpu_device.h", "pfem2gpu_kernels.h", "pfem2gpu_adjoint.h", "pfem2gpu_output.h", "pfem2gpu_wire.h",
+           "pfem2gpu_fem.h"]
 
 
 def hipcc():

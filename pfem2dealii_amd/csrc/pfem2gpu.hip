@@ -25,6 +25,7 @@
 
 #include "../../include/pfem2gpu.h"
 #include "pfem2gpu_adjoint.h"
+#include "pfem2gpu_fem.h"
 #include "pfem2gpu_output.h"
 #include "pfem2gpu_wire.h"
 
@@ -129,6 +130,19 @@ struct pfem2gpu_handle {
   long qcap = 0;
   int* qn = nullptr;  // [qn_len] queue lengths of the walk rounds (max n_sub + 1 used)
   int qn_len = 0;
+
+  // FEM assembly (built by the first assembly or sparsity call)
+  bool fem_ready = false;
+  std::vector<int64_t> fem_rowptr;  // [ndofs+1] CSR rows of the DoFs sharing a cell (host)
+  std::vector<int32_t> fem_col;     // [nnz] ascending columns per row (host)
+  long* fem_soff = nullptr;         // [nnz+1] contributions of each entry
+  int* fem_src = nullptr;           // [nc*nvc*nvc] cell-matrix offsets, ascending cell per entry
+  double* fem_A = nullptr;          // [nc][nvc][nvc] cell matrices
+  double* fem_b = nullptr;          // [nc][nvc][dim] cell right-hand sides
+  double* fem_vals = nullptr;       // [nnz] staging of a host matrix output
+  double* fem_rhs[3] = {nullptr, nullptr, nullptr};  // [ndofs] staging of host right-hand sides
+  double* fem_P = nullptr;          // [ndofs] staging of a host pressure input
+  unsigned char* fem_faces = nullptr;  // [nc] staging of host face flags
 
   // device buffers kept between adjoint calls (pointer, bytes)
   std::vector<std::pair<void*, size_t>> adj_pool;
@@ -1016,6 +1030,14 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
   dfree(h->contrib);
   dfree(h->d_rejected);
   dfree(h->dirty);
+  dfree(h->fem_soff);
+  dfree(h->fem_src);
+  dfree(h->fem_A);
+  dfree(h->fem_b);
+  dfree(h->fem_vals);
+  for (int k = 0; k < 3; ++k) dfree(h->fem_rhs[k]);
+  dfree(h->fem_P);
+  dfree(h->fem_faces);
   if (h->h_pinned) (void)hipHostFree(h->h_pinned);
   if (h->settle_ev) (void)hipEventDestroy(h->settle_ev);
   for (auto& t : h->pending) {
@@ -2071,6 +2093,190 @@ int pfem2gpu_remove_particles(pfem2gpu_handle* h, int64_t n, const int64_t* inde
       return fail(PFEM2GPU_ERR_ARG, "remove_particles: index " + std::to_string(index[i]) + " out of range [0, " +
                                         std::to_string(count) + ")");
   return h->dim == 2 ? remove_impl<2>(h, n, index, count) : remove_impl<3>(h, n, index, count);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- FEM assembly
+
+namespace {
+
+// The CSR pattern of the DoFs sharing a cell (DoFTools::make_sparsity_pattern
+// without constraints, src/pfem2fem.cpp:78-95) and, for every entry, the
+// offsets of its cell-matrix contributions in ascending cell order (the
+// order in which distribute_local_to_global adds them over the cells).
+int fem_prepare(pfem2gpu_handle* h) {
+  if (h->fem_ready) return PFEM2GPU_OK;
+  const int nvc = h->nvc, nc = h->nc, nd = h->ndofs;
+  if ((long)nc * nvc * nvc > INT32_MAX) return fail(PFEM2GPU_ERR_ARG, "mesh too large for FEM assembly");
+  std::vector<int> cd((size_t)nc * nvc);
+  HIPCHK(hipMemcpy(cd.data(), h->cd, cd.size() * 4, hipMemcpyDeviceToHost));
+  std::vector<int> off(nd + 1, 0), lst(cd.size());
+  for (size_t e = 0; e < cd.size(); ++e) off[cd[e] + 1]++;
+  for (int j = 0; j < nd; ++j) off[j + 1] += off[j];
+  {
+    std::vector<int> pos(off.begin(), off.end() - 1);
+    for (size_t e = 0; e < cd.size(); ++e) lst[pos[cd[e]]++] = (int)e;
+  }
+  std::vector<int64_t> rp(nd + 1, 0);
+  std::vector<int32_t> col;
+  col.reserve((size_t)nd * (nvc == 4 ? 9 : 27));
+  std::vector<int32_t> row;
+  for (int r = 0; r < nd; ++r) {
+    row.clear();
+    for (int e = off[r]; e < off[r + 1]; ++e) {
+      const int c = lst[e] / nvc;
+      for (int j = 0; j < nvc; ++j) row.push_back(cd[(size_t)c * nvc + j]);
+    }
+    std::sort(row.begin(), row.end());
+    row.erase(std::unique(row.begin(), row.end()), row.end());
+    col.insert(col.end(), row.begin(), row.end());
+    rp[r + 1] = (int64_t)col.size();
+  }
+  const long nnz = (long)col.size();
+  std::vector<long> soff(nnz + 1, 0);
+  std::vector<int> src((size_t)nc * nvc * nvc);
+  // contribution (p, (c*nvc + i)*nvc + j) for r = cd[c][i], in the order
+  // rows ascending, cells ascending: a stable counting sort by p (the
+  // position of p inside its row is kept between the two passes)
+  std::vector<uint16_t> cp;
+  cp.reserve(src.size());
+  for (int r = 0; r < nd; ++r) {
+    if (rp[r + 1] - rp[r] > 65535) return fail(PFEM2GPU_ERR_ARG, "FEM matrix row too long");
+    const int32_t* b = col.data() + rp[r];
+    const int32_t* e = col.data() + rp[r + 1];
+    for (int q = off[r]; q < off[r + 1]; ++q) {
+      const int c = lst[q] / nvc;
+      for (int j = 0; j < nvc; ++j) {
+        const uint16_t k = (uint16_t)(std::lower_bound(b, e, cd[(size_t)c * nvc + j]) - b);
+        cp.push_back(k);
+        soff[rp[r] + k + 1]++;
+      }
+    }
+  }
+  for (long p = 0; p < nnz; ++p) soff[p + 1] += soff[p];
+  {
+    std::vector<long> pos(soff.begin(), soff.end() - 1);
+    size_t t = 0;
+    for (int r = 0; r < nd; ++r)
+      for (int q = off[r]; q < off[r + 1]; ++q)
+        for (int j = 0; j < nvc; ++j) src[pos[rp[r] + cp[t++]]++] = lst[q] * nvc + j;
+  }
+  int rc;
+  if ((rc = dalloc(&h->fem_soff, soff.size())) || (rc = dalloc(&h->fem_src, src.size())) ||
+      (rc = dalloc(&h->fem_A, (size_t)nc * nvc * nvc)) || (rc = dalloc(&h->fem_b, (size_t)nc * nvc * h->dim)))
+    return rc;
+  HIPCHK(hipMemcpy(h->fem_soff, soff.data(), soff.size() * sizeof(long), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->fem_src, src.data(), src.size() * 4, hipMemcpyHostToDevice));
+  h->fem_rowptr.swap(rp);
+  h->fem_col.swap(col);
+  h->fem_ready = true;
+  return PFEM2GPU_OK;
+}
+
+int fem_grid(long nc, int nvc) {
+  const long cpb = kFemBlock / nvc;
+  long g = (nc + cpb - 1) / cpb;
+  return (int)std::max(1L, std::min(g, 65536L));
+}
+
+// Host-side staging of the optional host outputs and inputs.
+int fem_stage(pfem2gpu_handle* h, bool vals, int nrhs, bool P, bool faces) {
+  int rc;
+  if (vals && !h->fem_vals && (rc = dalloc(&h->fem_vals, h->fem_col.size()))) return rc;
+  for (int k = 0; k < nrhs; ++k)
+    if (!h->fem_rhs[k] && (rc = dalloc(&h->fem_rhs[k], h->ndofs))) return rc;
+  if (P && !h->fem_P && (rc = dalloc(&h->fem_P, h->ndofs))) return rc;
+  if (faces && !h->fem_faces && (rc = dalloc(&h->fem_faces, h->nc))) return rc;
+  return PFEM2GPU_OK;
+}
+
+template <int D>
+int fem_assemble(pfem2gpu_handle* h, bool pressure, const double* const* Vp, const double* P,
+                 const unsigned char* faces, double c0, double c1, double* matrix, double* const* rhs, int where) {
+  const bool host = where == PFEM2GPU_HOST;
+  const int nrhs = pressure ? 1 : D;
+  CHK(fem_prepare(h));
+  CHK(fem_stage(h, host && matrix, host ? nrhs : 0, host && P, host && faces));
+  DField Vf;
+  CHK(field_in(h, Vp, where, 0, &Vf));
+  const double* Pd = P;
+  const unsigned char* fd = faces;
+  if (host) {
+    if (P) HIPCHK(hipMemcpyAsync(h->fem_P, P, h->ndofs * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (faces) HIPCHK(hipMemcpyAsync(h->fem_faces, faces, h->nc, hipMemcpyHostToDevice, h->stream));
+    Pd = P ? h->fem_P : nullptr;
+    fd = faces ? h->fem_faces : nullptr;
+  }
+  const int grid = fem_grid(h->nc, h->nvc);
+  if (pressure) {
+    LAUNCH_B(h, "k_fem_pressure", (k_fem_cells<D, true>), grid, kFemBlock, 0, dmesh(h), h->nc, Vf, nullptr, c0, c1,
+             h->fem_A, h->fem_b);
+    if (fd)
+      LAUNCH(h, "k_fem_faces", k_fem_faces<D>, grid_for((long)h->nc * h->nvc), 0, dmesh(h), h->nc, Vf, c0, fd,
+             h->fem_b);
+  } else {
+    LAUNCH_B(h, "k_fem_vcorr", (k_fem_cells<D, false>), grid, kFemBlock, 0, dmesh(h), h->nc, Vf, Pd, c0, c1,
+             h->fem_A, h->fem_b);
+  }
+  const long nnz = (long)h->fem_col.size();
+  if (matrix) {
+    double* vd = host ? h->fem_vals : matrix;
+    LAUNCH(h, "k_fem_gather_matrix", k_fem_gather_matrix, grid_for(nnz), 0, nnz, h->fem_soff, h->fem_src, h->fem_A,
+           vd);
+  }
+  double* rd[3] = {nullptr, nullptr, nullptr};
+  for (int k = 0; k < nrhs; ++k) rd[k] = host ? h->fem_rhs[k] : rhs[k];
+  LAUNCH(h, "k_fem_gather_rhs", k_fem_gather_rhs, grid_for(h->ndofs), 0, h->ndofs, nrhs, h->d2c_off, h->d2c,
+         h->fem_b, rd[0], rd[1], rd[2]);
+  if (host) {
+    if (matrix) HIPCHK(hipMemcpyAsync(matrix, h->fem_vals, nnz * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    for (int k = 0; k < nrhs; ++k)
+      HIPCHK(hipMemcpyAsync(rhs[k], rd[k], h->ndofs * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  return PFEM2GPU_OK;
+}
+
+int fem_check(pfem2gpu_handle* h, const double* const* V, const void* rhs, int where) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  if (!V || !rhs) return fail(PFEM2GPU_ERR_ARG, "null field or right-hand side");
+  if (where != PFEM2GPU_HOST && where != PFEM2GPU_DEVICE) return fail(PFEM2GPU_ERR_ARG, "bad where");
+  return PFEM2GPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfem2gpu_fem_sparsity(pfem2gpu_handle* h, int64_t* nnz, int64_t* row_ptr, int32_t* cols) {
+  if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
+  CHK(fem_prepare(h));
+  if (nnz) *nnz = (int64_t)h->fem_col.size();
+  if (row_ptr) std::memcpy(row_ptr, h->fem_rowptr.data(), h->fem_rowptr.size() * sizeof(int64_t));
+  if (cols) std::memcpy(cols, h->fem_col.data(), h->fem_col.size() * sizeof(int32_t));
+  return PFEM2GPU_OK;
+}
+
+int pfem2gpu_assemble_pressure_equation(pfem2gpu_handle* h, const double* const* V_pred,
+                                        const uint8_t* boundary_faces, double rho, double time_step, double* matrix,
+                                        double* rhs, int where) {
+  CHK(fem_check(h, V_pred, rhs, where));
+  const double coeff = rho / time_step;
+  double* r[1] = {rhs};
+  return h->dim == 2 ? fem_assemble<2>(h, true, V_pred, nullptr, boundary_faces, coeff, 0.0, matrix, r, where)
+                     : fem_assemble<3>(h, true, V_pred, nullptr, boundary_faces, coeff, 0.0, matrix, r, where);
+}
+
+int pfem2gpu_assemble_velocity_correction(pfem2gpu_handle* h, const double* const* V_pred, const double* P,
+                                          double rho, double time_step, double* matrix, double* const* rhs,
+                                          int where) {
+  CHK(fem_check(h, V_pred, rhs, where));
+  if (!P) return fail(PFEM2GPU_ERR_ARG, "null pressure");
+  for (int k = 0; k < h->dim; ++k)
+    if (!rhs[k]) return fail(PFEM2GPU_ERR_ARG, "null right-hand side component");
+  return h->dim == 2 ? fem_assemble<2>(h, false, V_pred, P, nullptr, rho, time_step, matrix, rhs, where)
+                     : fem_assemble<3>(h, false, V_pred, P, nullptr, rho, time_step, matrix, rhs, where);
 }
 
 }  // extern "C"

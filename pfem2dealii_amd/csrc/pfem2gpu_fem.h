@@ -1,0 +1,340 @@
+// Finite-element assembly of the pressure equation and of the velocity
+// correction (src/pfem2fem.cpp:392-458 and :460-517) on the GPU: Q1 elements,
+// the 2-point Gauss rule per direction (QUAD_POINTS_PER_DIRECTION,
+// src/pfem2fem.h:24), the Q1 mapping, and the reference's arithmetic inside a
+// cell (quadrature point, then test function i, then trial function j).
+//
+// k_fem_cells: a workgroup takes kFemBlock / 2^dim cells, one thread per
+// (cell, r). Each thread first stages vertex r's coordinates and DoF r's field
+// values in the cell's LDS record, then computes the geometry at quadrature
+// point q = r (Jacobian, JxW and the real gradients of all 2^dim shape
+// functions) into the record, then row i = r of the cell matrix and entry i of
+// the cell right-hand side from the record. Cell matrices and right-hand
+// sides go to cell-major buffers.
+//
+// k_fem_gather_matrix: one thread per entry of the global CSR matrix sums
+// its contributions (a precomputed list of cell-matrix offsets, ascending
+// cell) and k_fem_gather_rhs one thread per DoF its cells' entries
+// (ascending cell): the sums of distribute_local_to_global over the cells in
+// order, without atomics, so results are the same bits on every run.
+// Constraints are not applied: the result is the unconstrained global system.
+#pragma once
+
+#include "pfem2gpu_kernels.h"
+
+namespace pfem2gpu {
+
+constexpr int kFemBlock = 128;
+
+// Points of QGauss<1>(2) on [0, 1]: (1 -+ 1/sqrt(3)) / 2, weight 1/2 each.
+__device__ __forceinline__ double gauss_pt(int b) {
+  return b ? 0.78867513459481288225 : 0.21132486540518711775;
+}
+
+// Layout of a cell's LDS record, in doubles. The size is odd so that the
+// records of the cells of one wavefront start in different LDS banks.
+template <int D>
+struct FemRec {
+  static constexpr int NV = 1 << D;
+  static constexpr int NQ = 1 << D;
+  static constexpr int G = 0;                 // [NQ][NV][D] real shape gradients at the points
+  static constexpr int J = G + NQ * NV * D;   // [NQ] JxW
+  static constexpr int X = J + NQ;            // [NV][D] vertex coordinates
+  static constexpr int V = X + NV * D;        // [NV][D] first field at the cell's DoFs
+  static constexpr int P = V + NV * D;        // [NV] second (scalar) field at the DoFs
+  static constexpr int size = (P + NV) | 1;
+};
+
+// Jacobian J[k][l] = sum_i X_i[k] dN_i/dxi_l of the Q1 map (sequential in i),
+// its determinant and its inverse (adjugate entries / det).
+template <int D>
+struct Q1Jac {
+  double det;
+  double inv[D][D];  // inv[l][k] = (J^-1)[l][k]
+};
+
+template <int D>
+__device__ __forceinline__ Q1Jac<D> q1_jacobian(const double* Xr, const double (*dN)[3]) {
+  constexpr int NV = 1 << D;
+  double J[D][D];
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+#pragma unroll
+    for (int l = 0; l < D; ++l) J[k][l] = 0.0;
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+      for (int l = 0; l < D; ++l) J[k][l] = J[k][l] + Xr[i * D + k] * dN[i][l];
+  Q1Jac<D> r;
+  if constexpr (D == 2) {
+    r.det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    r.inv[0][0] = J[1][1] / r.det;
+    r.inv[0][1] = -J[0][1] / r.det;
+    r.inv[1][0] = -J[1][0] / r.det;
+    r.inv[1][1] = J[0][0] / r.det;
+  } else {
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    r.det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    r.inv[0][0] = c00 / r.det;
+    r.inv[1][0] = c01 / r.det;
+    r.inv[2][0] = c02 / r.det;
+    r.inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / r.det;
+    r.inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / r.det;
+    r.inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / r.det;
+    r.inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / r.det;
+    r.inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / r.det;
+    r.inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / r.det;
+  }
+  return r;
+}
+
+// Real-space gradient of a shape function: g_k = sum_l dN/dxi_l (J^-1)[l][k].
+template <int D>
+__device__ __forceinline__ void q1_grad(const Q1Jac<D>& jac, const double* dNi, double* g) {
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    double s = dNi[0] * jac.inv[0][k];
+#pragma unroll
+    for (int l = 1; l < D; ++l) s = s + dNi[l] * jac.inv[l][k];
+    g[k] = s;
+  }
+}
+
+template <int D>
+__device__ __forceinline__ double dot_d(const double* a, const double* b) {
+  double s = a[0] * b[0];
+#pragma unroll
+  for (int k = 1; k < D; ++k) s = s + a[k] * b[k];
+  return s;
+}
+
+// Reference point of point q of face f = 2 k + side (xi_k = side; the other
+// coordinates Gauss points, the lower direction in the lower bit of q).
+template <int D>
+__device__ __forceinline__ void face_point(int f, int q, double* xi) {
+  const int kf = f >> 1;
+  int b = 0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    if (k == kf) {
+      xi[k] = (double)(f & 1);
+    } else {
+      xi[k] = gauss_pt((q >> b) & 1);
+      ++b;
+    }
+  }
+}
+
+// Boundary term of assemble_pressure_equation (src/pfem2fem.cpp:436-449),
+// after the cell term of the same right-hand side entry: for row i of every
+// cell with faces flagged in faces[c], face by face in increasing number,
+//   b_i -= sum_q coeff N_i (V_q . n) JxW,  V_q = sum_l N_l V(l),
+// n = J^-T n_ref / |J^-T n_ref| and JxW = det J |J^-T n_ref| w (Nanson).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_fem_faces(DMesh m, int nc, DField Vp, double coeff,
+                                                     const unsigned char* __restrict__ faces,
+                                                     double* __restrict__ bcell) {
+  constexpr int NV = 1 << D;
+  const long total = (long)nc * NV;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const long c = t / NV;
+    const int i = (int)(t % NV);
+    const int fm = faces[c];
+    if (!fm) continue;
+    int vid[NV], dof[NV];
+    load_ids<D>(m.cv, (int)c, vid);
+    load_ids<D>(m.cd, (int)c, dof);
+    double Xr[NV * D], Vr[NV * D];
+    for (int l = 0; l < NV; ++l)
+      for (int k = 0; k < D; ++k) {
+        Xr[l * D + k] = m.X[k][vid[l]];
+        Vr[l * D + k] = Vp.c[k][dof[l]];
+      }
+    double b = bcell[t];
+  for (int f = 0; f < 2 * D; ++f) {
+    if (!((fm >> f) & 1)) continue;
+    for (int q = 0; q < (1 << (D - 1)); ++q) {
+      double xi[D], w = 1.0;
+      face_point<D>(f, q, xi);
+      for (int k = 0; k < D - 1; ++k) w = w * 0.5;
+      double N[NV], dN[NV][3];
+      shape<D>(xi, N);
+      shape_grad<D>(xi, dN);
+      const Q1Jac<D> jac = q1_jacobian<D>(Xr, dN);
+      const int kf = f >> 1;
+      const double sgn = (f & 1) ? 1.0 : -1.0;
+      double nv[D], n2 = 0.0;
+      for (int k = 0; k < D; ++k) {
+        nv[k] = sgn * jac.inv[kf][k];
+        n2 = n2 + nv[k] * nv[k];
+      }
+      const double nn = sqrt(n2);
+      for (int k = 0; k < D; ++k) nv[k] = nv[k] / nn;
+      const double JxW = jac.det * nn * w;
+      double Vq[D];
+      for (int k = 0; k < D; ++k) Vq[k] = 0.0;
+      for (int l = 0; l < NV; ++l)
+        for (int k = 0; k < D; ++k) Vq[k] = Vq[k] + N[l] * Vr[l * D + k];
+      double tv[D];
+      for (int k = 0; k < D; ++k) tv[k] = coeff * N[i] * Vq[k];
+      b = b - dot_d<D>(tv, nv) * JxW;
+    }
+  }
+    bcell[t] = b;
+  }
+}
+
+// PRESSURE: assemble_pressure_equation (src/pfem2fem.cpp:392-458, SCHEMEB
+// off), c0 = coeff = rho / dt:
+//   A_ij = sum_q grad N_i . grad N_j JxW
+//   b_i  = sum_q sum_j coeff (sum_k V*_k(j) d_k N_i) N_j JxW
+// (the boundary term follows in k_fem_faces)
+// otherwise: assemble_velocity_correction (src/pfem2fem.cpp:460-517, SCHEMEB
+// off), c0 = rho, c1 = dt, the matrix common to the components:
+//   A_ij  = sum_q rho N_i N_j JxW
+//   b_ik  = sum_q sum_j [rho N_i N_j JxW V*_k(j) - dt N_i JxW d_k N_j P(j)]
+template <int D, bool PRESSURE>
+__global__ __launch_bounds__(kFemBlock) void k_fem_cells(DMesh m, int nc, DField Vp, const double* __restrict__ P,
+                                                        double c0, double c1, double* __restrict__ Acell, double* __restrict__ bcell) {
+  using R = FemRec<D>;
+  constexpr int NV = 1 << D;
+  constexpr int NQ = 1 << D;
+  constexpr int CPB = kFemBlock / NV;
+  __shared__ double s[CPB * R::size];
+  const int lc = threadIdx.x / NV, r = threadIdx.x % NV;
+  double* rec = s + lc * R::size;
+  for (long base = (long)blockIdx.x * CPB; base < nc; base += (long)gridDim.x * CPB) {
+    const long c = base + lc;
+    const bool on = c < nc;
+    if (on) {
+      const int vid = m.cv[c * NV + r], dof = m.cd[c * NV + r];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        rec[R::X + r * D + k] = m.X[k][vid];
+        rec[R::V + r * D + k] = Vp.c[k][dof];
+      }
+      if constexpr (!PRESSURE) rec[R::P + r] = P[dof];
+    }
+    __syncthreads();
+    if (on) {  // geometry at quadrature point q = r
+      double xi[D], w = 1.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        xi[k] = gauss_pt((r >> k) & 1);
+        w = w * 0.5;
+      }
+      double dN[NV][3];
+      shape_grad<D>(xi, dN);
+      const Q1Jac<D> jac = q1_jacobian<D>(rec + R::X, dN);
+      rec[R::J + r] = jac.det * w;
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        double g[D];
+        q1_grad<D>(jac, dN[j], g);
+#pragma unroll
+        for (int k = 0; k < D; ++k) rec[R::G + (r * NV + j) * D + k] = g[k];
+      }
+    }
+    __syncthreads();
+    if (on) {  // row i = r
+      const int i = r;
+      double A[NV];
+#pragma unroll
+      for (int j = 0; j < NV; ++j) A[j] = 0.0;
+      if constexpr (PRESSURE) {
+        const double coeff = c0;
+        double b = 0.0;
+#pragma unroll 1
+        for (int q = 0; q < NQ; ++q) {
+          double xi[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) xi[k] = gauss_pt((q >> k) & 1);
+          double N[NV];
+          shape<D>(xi, N);
+          const double JxW = rec[R::J + q];
+          double gi[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) gi[k] = rec[R::G + (q * NV + i) * D + k];
+#pragma unroll
+          for (int j = 0; j < NV; ++j) {
+            double gj[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) gj[k] = rec[R::G + (q * NV + j) * D + k];
+            A[j] = A[j] + dot_d<D>(gi, gj) * JxW;
+            double aux = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) aux = aux + rec[R::V + j * D + k] * gi[k];
+            b = b + coeff * aux * N[j] * JxW;
+          }
+        }
+        bcell[c * NV + i] = b;
+      } else {
+        const double rho = c0, dt = c1;
+        double b[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) b[k] = 0.0;
+#pragma unroll 1
+        for (int q = 0; q < NQ; ++q) {
+          double xi[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) xi[k] = gauss_pt((q >> k) & 1);
+          double N[NV];
+          shape<D>(xi, N);
+          const double JxW = rec[R::J + q];
+          const double aux2 = dt * N[i] * JxW;
+#pragma unroll
+          for (int j = 0; j < NV; ++j) {
+            const double aux = rho * N[i] * N[j] * JxW;
+            A[j] = A[j] + aux;
+            const double pj = rec[R::P + j];
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+              b[k] = b[k] + aux * rec[R::V + j * D + k];
+              b[k] = b[k] - aux2 * rec[R::G + (q * NV + j) * D + k] * pj;
+            }
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) bcell[(c * NV + i) * D + k] = b[k];
+      }
+#pragma unroll
+      for (int j = 0; j < NV; ++j) Acell[(c * NV + i) * NV + j] = A[j];
+    }
+    __syncthreads();  // the records are rewritten by the next cells
+  }
+}
+
+// Entry p of the global matrix: the sum of its cell-matrix contributions
+// src[src_off[p] .. src_off[p+1]) in ascending cell order.
+__global__ __launch_bounds__(kBlock) void k_fem_gather_matrix(long nnz, const long* __restrict__ src_off,
+                                                             const int* __restrict__ src,
+                                                             const double* __restrict__ Acell,
+                                                             double* __restrict__ values) {
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < nnz; p += (long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (long e = src_off[p]; e < src_off[p + 1]; ++e) s = s + Acell[src[e]];
+    values[p] = s;
+  }
+}
+
+// Right-hand side of DoF r, components k < ncomp: the sum of the entries of
+// its (cell, vertex) pairs d2c (ascending cell).
+__global__ __launch_bounds__(kBlock) void k_fem_gather_rhs(int ndofs, int ncomp, const int* __restrict__ d2c_off,
+                                                          const int* __restrict__ d2c,
+                                                          const double* __restrict__ bcell, double* o0, double* o1,
+                                                          double* o2) {
+  double* out[3] = {o0, o1, o2};
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < ndofs; r += gridDim.x * blockDim.x) {
+    for (int k = 0; k < ncomp; ++k) {
+      double s = 0.0;
+      for (int e = d2c_off[r]; e < d2c_off[r + 1]; ++e) s = s + bcell[(size_t)d2c[e] * ncomp + k];
+      out[k][r] = s;
+    }
+  }
+}
+
+}  // namespace pfem2gpu

@@ -366,6 +366,142 @@ def wire_cost(workload="taylor_green2d", scale=1.0, reps=3):
             "unpack_frac": round(unpack_b / (unpack_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+FP64_PEAK_TFLOPS = 78.6
+
+
+def fem_work_per_cell(dim):
+    """Algorithmic HBM bytes and FP64 operations per cell of the cell
+    kernels (k_fem_pressure, k_fem_vcorr) as written: reads of the cell's
+    vertex and DoF ids (2 * 4 * 2^d) and of the places of its matrix and
+    right-hand side entries (4 * 4^d + 4 * 2^d), one vertex's coordinates and
+    one DoF's field values per cell (8 d each; + 8 for the pressure of
+    k_fem_vcorr); writes of the cell matrix (8 * 4^d) and right-hand side
+    (8 * 2^d, times d for k_fem_vcorr). Operations: per quadrature point the Jacobian
+    (2^(d+1) d^2), its inverse (7 in 2D, 41 in 3D), JxW and the 2^d real
+    gradients (2^d d (2d - 1)); per (row, point, column) 4d + 5 for the
+    pressure (gradient product, matrix and right-hand side updates) and
+    3 + 4d for the velocity correction."""
+    nv = 1 << dim
+    geo = nv * (2 * nv * dim * dim + (7 if dim == 2 else 41) + 1 + nv * dim * (2 * dim - 1))
+    places = 4 * nv * nv + 4 * nv
+    p_bytes = 8 * nv + places + 8 * dim + 8 * dim + 8 * nv * nv + 8 * nv
+    v_bytes = 8 * nv + places + 8 * dim + 8 * dim + 8 + 8 * nv * nv + 8 * nv * dim
+    return {"k_fem_pressure": (p_bytes, geo + nv * nv * nv * (4 * dim + 5)),
+            "k_fem_vcorr": (v_bytes, geo + nv * nv * nv * (3 + 4 * dim))}
+
+
+def fem_cost(reps=5, cpu=True):
+    """FEM assembly (pressure equation with every boundary face flagged, and
+    velocity correction, matrices and right-hand sides) on the meshes of the
+    2D and 3D workloads at full size, device arrays: per-kernel HIP-event
+    durations (median over `reps` calls after one warm-up call), algorithmic
+    bytes and FP64 operations of the cell kernels (fem_work_per_cell) and of
+    the segmented sums (matrix: 8 bytes read per contribution, 8 bytes of
+    offset read and 8 written per entry; right-hand side: 8 bytes read per
+    (cell, vertex) and component, 4 bytes of offset read per DoF and 8 written
+    per DoF and component). CPU baseline: the
+    oracle's assembly (one thread) of the 2D mesh and of the 3D mesh at
+    half the cells per direction."""
+    import torch
+    from pfem2dealii_amd import ParticleHandler, workloads as W
+    out = {}
+    for name, n, lo, hi in (("taylor_green2d", [2048, 1024], [0.0, 0.0], [2.0 * np.pi, np.pi]),
+                            ("abc3d", [128, 128, 128], [0.0] * 3, [1.0] * 3)):
+        mesh = W.structured_mesh(n, lo, hi)
+        dim, nd = mesh["dim"], mesh["n_dofs"]
+        nc = mesh["cell_vertices"].shape[0]
+        nv = 1 << dim
+        prm = dict(n_sub=1, dt=0.01, max_per_part=4, particles_per_cell=[2] * dim)
+        h = ParticleHandler(mesh, prm)
+        rp, cols = h.fem_sparsity()
+        nnz = cols.shape[0]
+        rng = np.random.default_rng(1)
+        V = torch.tensor(rng.standard_normal((dim, nd)), device="cuda")
+        P = torch.tensor(rng.standard_normal(nd), device="cuda")
+        faces = torch.tensor(box_faces(n), device="cuda")
+        h.assemble_pressure_equation(V, 1.0, 0.01, boundary_faces=faces)
+        h.assemble_velocity_correction(V, P, 1.0, 0.01)
+        per = {}
+        walls = {"pressure": [], "velocity_correction": []}
+        for _ in range(reps):
+            h.set_timing(True)
+            t0 = time.perf_counter()
+            h.assemble_pressure_equation(V, 1.0, 0.01, boundary_faces=faces)
+            walls["pressure"].append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            h.assemble_velocity_correction(V, P, 1.0, 0.01)
+            walls["velocity_correction"].append(time.perf_counter() - t0)
+            for k, (cnt, ms) in h.timings().items():
+                if k.startswith("k_fem"):
+                    per.setdefault(k, []).append(ms / cnt)
+            h.set_timing(False)
+        h.close()
+        work = fem_work_per_cell(dim)
+        kern = {}
+        for k, v in sorted(per.items()):
+            ms = float(np.median(v))
+            e = {"ms": round(ms, 4)}
+            if k in work:
+                b, f = work[k][0] * nc, work[k][1] * nc
+            elif k in ("k_fem_gather_matrix", "k_fem_gather_mass"):
+                b, f = 8.0 * nc * nv * nv + 8.0 * nnz + 8.0 * (nnz + 1), float(nc * nv * nv)
+            elif k in ("k_fem_gather_rhs", "k_fem_gather_rhs_v"):
+                ncomp = 1 if k == "k_fem_gather_rhs" else dim
+                b, f = 8.0 * ncomp * nc * nv + 8.0 * ncomp * nd + 4.0 * (nd + 1), float(ncomp * nc * nv)
+            else:
+                b, f = None, None
+            if b is not None:
+                gbs = b / (ms * 1e-3) / 1e9
+                tf = f / (ms * 1e-3) / 1e12
+                e.update({"alg_bytes": b, "GBps": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                          "fp64_ops": f, "TFLOPs": round(tf, 2), "fp64_frac": round(tf / FP64_PEAK_TFLOPS, 4)})
+            kern[k] = e
+        res = {"mesh": "x".join(str(a) for a in n), "cells": nc, "dofs": nd, "nnz": nnz, "kernels": kern,
+               "pressure_call_ms": round(float(np.median(walls["pressure"])) * 1e3, 3),
+               "velocity_correction_call_ms": round(float(np.median(walls["velocity_correction"])) * 1e3, 3)}
+        if cpu:
+            from oracle import FemOracle
+            cs = 1 if dim == 2 else 2
+            cm = W.structured_mesh([a // cs for a in n], lo, hi)
+            fo = FemOracle(cm)
+            Vc = np.ones((dim, cm["n_dofs"]))
+            t0 = time.perf_counter()
+            fo.pressure(Vc, 1.0, 0.01, box_faces(cm["n"]))
+            tp = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            fo.velocity_correction(Vc, Vc[0], 1.0, 0.01)
+            tv = time.perf_counter() - t0
+            ccells = cm["cell_vertices"].shape[0]
+            res["cpu_baseline"] = {"pressure_cells_per_s": ccells / tp, "velocity_correction_cells_per_s": ccells / tv,
+                                   "cores": 1, "kind": "port",
+                                   "sample": "oracle assembly (matrix + right-hand side) of the %s mesh, one thread"
+                                             % cm_name(cm)}
+        gp = sum(kern[k]["ms"] for k in ("k_fem_pressure", "k_fem_faces", "k_fem_gather_matrix", "k_fem_gather_rhs")
+                 if k in kern)
+        gv = sum(kern[k]["ms"] for k in ("k_fem_vcorr", "k_fem_gather_mass", "k_fem_gather_rhs_v") if k in kern)
+        res["pressure_gpu_cells_per_s"] = nc / (gp * 1e-3)
+        res["velocity_correction_gpu_cells_per_s"] = nc / (gv * 1e-3)
+        out[name] = res
+    return out
+
+
+def box_faces(n):
+    """Boundary face flags of a box mesh (bit 2k + side on the box faces)."""
+    nc = int(np.prod(n))
+    fm = np.zeros(nc, dtype=np.uint8)
+    rem = np.arange(nc)
+    for k in range(len(n)):
+        ck = rem % n[k]
+        rem = rem // n[k]
+        fm |= np.where(ck == 0, 1 << (2 * k), 0).astype(np.uint8)
+        fm |= np.where(ck == n[k] - 1, 1 << (2 * k + 1), 0).astype(np.uint8)
+    return fm
+
+
+def cm_name(mesh):
+    return "x".join(str(a) for a in mesh["n"])
+
+
 def plain_run(workload, scale, steps, warmup, autotune=False):
     """Run A: `warmup` + `steps` steps of the plain handle from the seeded
     state, HIP events around the timed steps on the handle's stream
@@ -501,7 +637,9 @@ def main():
                     help="only run a plain 1 GiB device copy (counter calibration) and exit")
     ap.add_argument("--wire-reps", type=int, default=3,
                     help="repetitions of the particle-record pack/unpack measurement (0: skip)")
-    ap.add_argument("--section", choices=("batched", "adjoint", "wire"), default=None,
+    ap.add_argument("--fem-reps", type=int, default=5,
+                    help="repetitions of the FEM assembly measurement (0: skip)")
+    ap.add_argument("--section", choices=("batched", "adjoint", "wire", "fem"), default=None,
                     help="only run the batched comparison or the adjoint cost measurement (for profiling) and "
                          "print its result")
     args = ap.parse_args()
@@ -520,6 +658,10 @@ def main():
     if args.section == "adjoint":
         print(json.dumps({"section": "adjoint", "adjoint": adjoint_cost(max(1, args.adjoint_reps)),
                           "adjoint3d": adjoint_cost(max(1, args.adjoint_reps), 4.0, "adjoint3d")}), flush=True)
+        return
+    if args.section == "fem":
+        print(json.dumps({"section": "fem", "fem": fem_cost(max(1, args.fem_reps), cpu=not args.no_cpu)}),
+              flush=True)
         return
     if args.section == "wire":
         print(json.dumps({"section": "wire", "wire": wire_cost(args.workload, args.scale, max(1, args.wire_reps))}),
@@ -545,6 +687,8 @@ def main():
         out["adjoint3d"] = adjoint_cost(args.adjoint_reps, 4.0, "adjoint3d")
     if args.wire_reps > 0:
         out["wire"] = wire_cost(args.workload, args.scale, args.wire_reps)
+    if args.fem_reps > 0 and args.scale == 1.0:
+        out["fem"] = fem_cost(args.fem_reps, cpu=not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload)
         if "abc3d" in out:

@@ -25,7 +25,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
-#include <set>
 #include <vector>
 
 namespace {
@@ -180,18 +179,22 @@ extern "C" {
 // written when not null (call once with null for the size).
 int64_t orc_fem_pattern(int dim, int64_t nc, int64_t ndofs, const int* cd, int64_t* rowptr, int32_t* cols) {
   const int nv = 1 << dim;
-  std::vector<std::set<int>> rows(ndofs);
+  std::vector<std::vector<int>> rows(ndofs);
   for (long c = 0; c < nc; ++c)
     for (int i = 0; i < nv; ++i)
-      for (int j = 0; j < nv; ++j) rows[cd[c * nv + i]].insert(cd[c * nv + j]);
+      for (int j = 0; j < nv; ++j) rows[cd[c * nv + i]].push_back(cd[c * nv + j]);
   int64_t n = 0;
   rowptr[0] = 0;
   for (long r = 0; r < ndofs; ++r) {
-    for (int col : rows[r]) {
+    std::vector<int>& row = rows[r];
+    std::sort(row.begin(), row.end());
+    row.erase(std::unique(row.begin(), row.end()), row.end());
+    for (int col : row) {
       if (cols) cols[n] = col;
       ++n;
     }
     rowptr[r + 1] = n;
+    std::vector<int>().swap(row);
   }
   return n;
 }

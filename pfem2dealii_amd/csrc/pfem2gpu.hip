@@ -135,10 +135,11 @@ struct pfem2gpu_handle {
   bool fem_ready = false;
   std::vector<int64_t> fem_rowptr;  // [ndofs+1] CSR rows of the DoFs sharing a cell (host)
   std::vector<int32_t> fem_col;     // [nnz] ascending columns per row (host)
-  long* fem_soff = nullptr;         // [nnz+1] contributions of each entry
-  int* fem_src = nullptr;           // [nc*nvc*nvc] cell-matrix offsets, ascending cell per entry
-  double* fem_A = nullptr;          // [nc][nvc][nvc] cell matrices
-  double* fem_b = nullptr;          // [nc][nvc][dim] cell right-hand sides
+  long* fem_soff = nullptr;         // [nnz+1] contributions of each entry (runs in fem_A)
+  int* fem_apos = nullptr;          // [nc*nvc*nvc] place of cell-matrix entry (c, i, j) in fem_A
+  int* fem_bpos = nullptr;          // [nc*nvc] place of right-hand side entry (c, i) in fem_b
+  double* fem_A = nullptr;          // cell-matrix entries in contribution order (entry, then cell)
+  double* fem_b = nullptr;          // [dim][nc*nvc] cell right-hand sides in DoF -> (cell, vertex) order
   double* fem_vals = nullptr;       // [nnz] staging of a host matrix output
   double* fem_rhs[3] = {nullptr, nullptr, nullptr};  // [ndofs] staging of host right-hand sides
   double* fem_P = nullptr;          // [ndofs] staging of a host pressure input
@@ -1031,7 +1032,8 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
   dfree(h->d_rejected);
   dfree(h->dirty);
   dfree(h->fem_soff);
-  dfree(h->fem_src);
+  dfree(h->fem_apos);
+  dfree(h->fem_bpos);
   dfree(h->fem_A);
   dfree(h->fem_b);
   dfree(h->fem_vals);
@@ -2162,12 +2164,18 @@ int fem_prepare(pfem2gpu_handle* h) {
       for (int q = off[r]; q < off[r + 1]; ++q)
         for (int j = 0; j < nvc; ++j) src[pos[rp[r] + cp[t++]]++] = lst[q] * nvc + j;
   }
+  // the places of the cell entries in contribution order (inverse lists)
+  std::vector<int> apos(src.size()), bpos(lst.size());
+  for (size_t e = 0; e < src.size(); ++e) apos[src[e]] = (int)e;
+  for (size_t e = 0; e < lst.size(); ++e) bpos[lst[e]] = (int)e;
   int rc;
-  if ((rc = dalloc(&h->fem_soff, soff.size())) || (rc = dalloc(&h->fem_src, src.size())) ||
-      (rc = dalloc(&h->fem_A, (size_t)nc * nvc * nvc)) || (rc = dalloc(&h->fem_b, (size_t)nc * nvc * h->dim)))
+  if ((rc = dalloc(&h->fem_soff, soff.size())) || (rc = dalloc(&h->fem_apos, apos.size())) ||
+      (rc = dalloc(&h->fem_bpos, bpos.size())) || (rc = dalloc(&h->fem_A, (size_t)nc * nvc * nvc)) ||
+      (rc = dalloc(&h->fem_b, (size_t)nc * nvc * h->dim)))
     return rc;
   HIPCHK(hipMemcpy(h->fem_soff, soff.data(), soff.size() * sizeof(long), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->fem_src, src.data(), src.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->fem_apos, apos.data(), apos.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->fem_bpos, bpos.data(), bpos.size() * 4, hipMemcpyHostToDevice));
   h->fem_rowptr.swap(rp);
   h->fem_col.swap(col);
   h->fem_ready = true;
@@ -2211,24 +2219,24 @@ int fem_assemble(pfem2gpu_handle* h, bool pressure, const double* const* Vp, con
   const int grid = fem_grid(h->nc, h->nvc);
   if (pressure) {
     LAUNCH_B(h, "k_fem_pressure", (k_fem_cells<D, true>), grid, kFemBlock, 0, dmesh(h), h->nc, Vf, nullptr, c0, c1,
-             h->fem_A, h->fem_b);
+             h->fem_apos, h->fem_bpos, h->fem_A, h->fem_b);
     if (fd)
-      LAUNCH(h, "k_fem_faces", k_fem_faces<D>, grid_for((long)h->nc * h->nvc), 0, dmesh(h), h->nc, Vf, c0, fd,
+      LAUNCH(h, "k_fem_faces", k_fem_faces<D>, grid_for(h->nc), 0, dmesh(h), h->nc, Vf, c0, fd, h->fem_bpos,
              h->fem_b);
   } else {
     LAUNCH_B(h, "k_fem_vcorr", (k_fem_cells<D, false>), grid, kFemBlock, 0, dmesh(h), h->nc, Vf, Pd, c0, c1,
-             h->fem_A, h->fem_b);
+             h->fem_apos, h->fem_bpos, h->fem_A, h->fem_b);
   }
   const long nnz = (long)h->fem_col.size();
   if (matrix) {
     double* vd = host ? h->fem_vals : matrix;
-    LAUNCH(h, "k_fem_gather_matrix", k_fem_gather_matrix, grid_for(nnz), 0, nnz, h->fem_soff, h->fem_src, h->fem_A,
-           vd);
+    LAUNCH(h, pressure ? "k_fem_gather_matrix" : "k_fem_gather_mass", k_fem_segsum<long>, grid_for(nnz), 0, nnz,
+           h->fem_soff, 1, 0L, h->fem_A, vd, nullptr, nullptr);
   }
   double* rd[3] = {nullptr, nullptr, nullptr};
   for (int k = 0; k < nrhs; ++k) rd[k] = host ? h->fem_rhs[k] : rhs[k];
-  LAUNCH(h, "k_fem_gather_rhs", k_fem_gather_rhs, grid_for(h->ndofs), 0, h->ndofs, nrhs, h->d2c_off, h->d2c,
-         h->fem_b, rd[0], rd[1], rd[2]);
+  LAUNCH(h, pressure ? "k_fem_gather_rhs" : "k_fem_gather_rhs_v", k_fem_segsum<int>, grid_for(h->ndofs), 0,
+         (long)h->ndofs, h->d2c_off, nrhs, (long)h->nc * h->nvc, h->fem_b, rd[0], rd[1], rd[2]);
   if (host) {
     if (matrix) HIPCHK(hipMemcpyAsync(matrix, h->fem_vals, nnz * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     for (int k = 0; k < nrhs; ++k)

@@ -9,14 +9,17 @@
 // values in the cell's LDS record, then computes the geometry at quadrature
 // point q = r (Jacobian, JxW and the real gradients of all 2^dim shape
 // functions) into the record, then row i = r of the cell matrix and entry i of
-// the cell right-hand side from the record. Cell matrices and right-hand
-// sides go to cell-major buffers.
+// the cell right-hand side from the record. Each cell-matrix entry (c, i, j)
+// is stored at its place apos[(c 2^d + i) 2^d + j] in contribution order (by
+// global entry, then ascending cell), each right-hand side entry (c, i) at
+// bpos[c 2^d + i] in the order of the DoF -> (cell, vertex) lists (by DoF,
+// then ascending cell; one plane of nc 2^d entries per component).
 //
-// k_fem_gather_matrix: one thread per entry of the global CSR matrix sums
-// its contributions (a precomputed list of cell-matrix offsets, ascending
-// cell) and k_fem_gather_rhs one thread per DoF its cells' entries
-// (ascending cell): the sums of distribute_local_to_global over the cells in
-// order, without atomics, so results are the same bits on every run.
+// k_fem_gather_matrix: one thread per entry of the global CSR matrix sums its
+// contiguous run of contributions, and k_fem_gather_rhs one thread per DoF
+// its contiguous run of cell entries: the sums of distribute_local_to_global
+// over the cells in ascending order, without atomics, so results are the
+// same bits on every run.
 // Constraints are not applied: the result is the unconstrained global system.
 #pragma once
 
@@ -130,61 +133,77 @@ __device__ __forceinline__ void face_point(int f, int q, double* xi) {
 }
 
 // Boundary term of assemble_pressure_equation (src/pfem2fem.cpp:436-449),
-// after the cell term of the same right-hand side entry: for row i of every
-// cell with faces flagged in faces[c], face by face in increasing number,
-//   b_i -= sum_q coeff N_i (V_q . n) JxW,  V_q = sum_l N_l V(l),
+// after the cell term of the same right-hand side entries: one thread per
+// cell with faces flagged in faces[c]; face by face in increasing number and
+// point by point, the geometry once, then for every row i
+//   b_i -= coeff N_i (V_q . n) JxW,  V_q = sum_l N_l V(l),
 // n = J^-T n_ref / |J^-T n_ref| and JxW = det J |J^-T n_ref| w (Nanson).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_fem_faces(DMesh m, int nc, DField Vp, double coeff,
                                                      const unsigned char* __restrict__ faces,
-                                                     double* __restrict__ bcell) {
+                                                     const int* __restrict__ bpos, double* __restrict__ bcell) {
   constexpr int NV = 1 << D;
-  const long total = (long)nc * NV;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
-    const long c = t / NV;
-    const int i = (int)(t % NV);
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
     const int fm = faces[c];
     if (!fm) continue;
     int vid[NV], dof[NV];
-    load_ids<D>(m.cv, (int)c, vid);
-    load_ids<D>(m.cd, (int)c, dof);
-    double Xr[NV * D], Vr[NV * D];
-    for (int l = 0; l < NV; ++l)
+    load_ids<D>(m.cv, c, vid);
+    load_ids<D>(m.cd, c, dof);
+    double Xr[NV * D], Vr[NV * D], b[NV];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+#pragma unroll
       for (int k = 0; k < D; ++k) {
         Xr[l * D + k] = m.X[k][vid[l]];
         Vr[l * D + k] = Vp.c[k][dof[l]];
       }
-    double b = bcell[t];
-  for (int f = 0; f < 2 * D; ++f) {
-    if (!((fm >> f) & 1)) continue;
-    for (int q = 0; q < (1 << (D - 1)); ++q) {
-      double xi[D], w = 1.0;
-      face_point<D>(f, q, xi);
-      for (int k = 0; k < D - 1; ++k) w = w * 0.5;
-      double N[NV], dN[NV][3];
-      shape<D>(xi, N);
-      shape_grad<D>(xi, dN);
-      const Q1Jac<D> jac = q1_jacobian<D>(Xr, dN);
-      const int kf = f >> 1;
-      const double sgn = (f & 1) ? 1.0 : -1.0;
-      double nv[D], n2 = 0.0;
-      for (int k = 0; k < D; ++k) {
-        nv[k] = sgn * jac.inv[kf][k];
-        n2 = n2 + nv[k] * nv[k];
-      }
-      const double nn = sqrt(n2);
-      for (int k = 0; k < D; ++k) nv[k] = nv[k] / nn;
-      const double JxW = jac.det * nn * w;
-      double Vq[D];
-      for (int k = 0; k < D; ++k) Vq[k] = 0.0;
-      for (int l = 0; l < NV; ++l)
-        for (int k = 0; k < D; ++k) Vq[k] = Vq[k] + N[l] * Vr[l * D + k];
-      double tv[D];
-      for (int k = 0; k < D; ++k) tv[k] = coeff * N[i] * Vq[k];
-      b = b - dot_d<D>(tv, nv) * JxW;
+      b[l] = bcell[bpos[(size_t)c * NV + l]];
     }
-  }
-    bcell[t] = b;
+    for (int f = 0; f < 2 * D; ++f) {
+      if (!((fm >> f) & 1)) continue;
+      for (int q = 0; q < (1 << (D - 1)); ++q) {
+        double xi[D], w = 1.0;
+        face_point<D>(f, q, xi);
+#pragma unroll
+        for (int k = 0; k < D - 1; ++k) w = w * 0.5;
+        double N[NV], dN[NV][3];
+        shape<D>(xi, N);
+        shape_grad<D>(xi, dN);
+        const Q1Jac<D> jac = q1_jacobian<D>(Xr, dN);
+        const int kf = f >> 1;
+        const double sgn = (f & 1) ? 1.0 : -1.0;
+        double nv[D], n2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          double a = jac.inv[0][k];
+#pragma unroll
+          for (int l = 1; l < D; ++l)
+            if (l == kf) a = jac.inv[l][k];
+          nv[k] = sgn * a;
+          n2 = n2 + nv[k] * nv[k];
+        }
+        const double nn = sqrt(n2);
+#pragma unroll
+        for (int k = 0; k < D; ++k) nv[k] = nv[k] / nn;
+        const double JxW = jac.det * nn * w;
+        double Vq[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) Vq[k] = 0.0;
+#pragma unroll
+        for (int l = 0; l < NV; ++l)
+#pragma unroll
+          for (int k = 0; k < D; ++k) Vq[k] = Vq[k] + N[l] * Vr[l * D + k];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+          double tv[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) tv[k] = coeff * N[i] * Vq[k];
+          b[i] = b[i] - dot_d<D>(tv, nv) * JxW;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) bcell[bpos[(size_t)c * NV + i]] = b[i];
   }
 }
 
@@ -199,7 +218,9 @@ __global__ __launch_bounds__(kBlock) void k_fem_faces(DMesh m, int nc, DField Vp
 //   b_ik  = sum_q sum_j [rho N_i N_j JxW V*_k(j) - dt N_i JxW d_k N_j P(j)]
 template <int D, bool PRESSURE>
 __global__ __launch_bounds__(kFemBlock) void k_fem_cells(DMesh m, int nc, DField Vp, const double* __restrict__ P,
-                                                        double c0, double c1, double* __restrict__ Acell, double* __restrict__ bcell) {
+                                                        double c0, double c1, const int* __restrict__ apos,
+                                                        const int* __restrict__ bpos, double* __restrict__ Acell,
+                                                        double* __restrict__ bcell) {
   using R = FemRec<D>;
   constexpr int NV = 1 << D;
   constexpr int NQ = 1 << D;
@@ -271,7 +292,7 @@ __global__ __launch_bounds__(kFemBlock) void k_fem_cells(DMesh m, int nc, DField
             b = b + coeff * aux * N[j] * JxW;
           }
         }
-        bcell[c * NV + i] = b;
+        bcell[bpos[c * NV + i]] = b;
       } else {
         const double rho = c0, dt = c1;
         double b[D];
@@ -299,40 +320,45 @@ __global__ __launch_bounds__(kFemBlock) void k_fem_cells(DMesh m, int nc, DField
           }
         }
 #pragma unroll
-        for (int k = 0; k < D; ++k) bcell[(c * NV + i) * D + k] = b[k];
+        for (int k = 0; k < D; ++k) bcell[(size_t)k * nc * NV + bpos[c * NV + i]] = b[k];
       }
 #pragma unroll
-      for (int j = 0; j < NV; ++j) Acell[(c * NV + i) * NV + j] = A[j];
+      for (int j = 0; j < NV; ++j) Acell[apos[(c * NV + i) * NV + j]] = A[j];
     }
     __syncthreads();  // the records are rewritten by the next cells
   }
 }
 
-// Entry p of the global matrix: the sum of its cell-matrix contributions
-// src[src_off[p] .. src_off[p+1]) in ascending cell order.
-__global__ __launch_bounds__(kBlock) void k_fem_gather_matrix(long nnz, const long* __restrict__ src_off,
-                                                             const int* __restrict__ src,
-                                                             const double* __restrict__ Acell,
-                                                             double* __restrict__ values) {
-  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < nnz; p += (long)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (long e = src_off[p]; e < src_off[p + 1]; ++e) s = s + Acell[src[e]];
-    values[p] = s;
-  }
-}
+// Segmented sums for the global matrix and right-hand sides: segment p of
+// component k is src[k * plane + off[p] .. k * plane + off[p+1]), summed in
+// order (ascending cell) into out_k[p]. The segments of a workgroup's kBlock
+// consecutive p form one contiguous run of src, staged through LDS in tiles of
+// kFemTile doubles with coalesced loads; each thread then sums its own
+// segment's part of every tile in order.
+constexpr int kFemTile = 2048;
 
-// Right-hand side of DoF r, components k < ncomp: the sum of the entries of
-// its (cell, vertex) pairs d2c (ascending cell).
-__global__ __launch_bounds__(kBlock) void k_fem_gather_rhs(int ndofs, int ncomp, const int* __restrict__ d2c_off,
-                                                          const int* __restrict__ d2c,
-                                                          const double* __restrict__ bcell, double* o0, double* o1,
-                                                          double* o2) {
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void k_fem_segsum(long nseg, const OffT* __restrict__ off, int ncomp,
+                                                      long plane, const double* __restrict__ src, double* o0,
+                                                      double* o1, double* o2) {
+  __shared__ double tile[kFemTile];
   double* out[3] = {o0, o1, o2};
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < ndofs; r += gridDim.x * blockDim.x) {
+  for (long base = (long)blockIdx.x * kBlock; base < nseg; base += (long)gridDim.x * kBlock) {
+    const long p = base + threadIdx.x;
+    const long lo = off[base], hi = off[base + kBlock < nseg ? base + kBlock : nseg];
+    const long mlo = p < nseg ? (long)off[p] : hi, mhi = p < nseg ? (long)off[p + 1] : hi;
     for (int k = 0; k < ncomp; ++k) {
+      const double* sk = src + k * plane;
       double s = 0.0;
-      for (int e = d2c_off[r]; e < d2c_off[r + 1]; ++e) s = s + bcell[(size_t)d2c[e] * ncomp + k];
-      out[k][r] = s;
+      for (long t0 = lo; t0 < hi; t0 += kFemTile) {
+        const long t1 = t0 + kFemTile < hi ? t0 + kFemTile : hi;
+        __syncthreads();  // the previous tile is consumed
+        for (long e = t0 + threadIdx.x; e < t1; e += kBlock) tile[e - t0] = sk[e];
+        __syncthreads();
+        const long a = mlo > t0 ? mlo : t0, b = mhi < t1 ? mhi : t1;
+        for (long e = a; e < b; ++e) s = s + tile[e - t0];
+      }
+      if (p < nseg) out[k][p] = s;
     }
   }
 }

@@ -287,9 +287,10 @@ def adjoint_cost(reps=3, scale=1.0, workload="adjoint2d"):
 
 
 def calibrate_copy(n=1 << 27, reps=5):
-    """A plain device copy of n doubles (8n bytes read, 8n written), run
-    `reps` times: the known-size reference for the FETCH_SIZE/WRITE_SIZE
-    counters."""
+    """Known-size references for the FETCH_SIZE/WRITE_SIZE counters: a 1 GiB
+    device copy of n doubles (8n bytes read, 8n written) run `reps` times,
+    first as the runtime's blit copy (copy_), then as an ordinary elementwise
+    kernel with plain loads and stores (mul by 1.0 into another tensor)."""
     import torch
     a = torch.ones(n, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
@@ -297,7 +298,11 @@ def calibrate_copy(n=1 << 27, reps=5):
     for _ in range(reps):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "reps": reps}), flush=True)
+    for _ in range(reps):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "reps": reps,
+                      "kernels": ["blit copy", "elementwise mul"]}), flush=True)
 
 
 def wire_cost(workload="taylor_green2d", scale=1.0, reps=3):
@@ -621,6 +626,59 @@ def plain_run(workload, scale, steps, warmup, autotune=False):
             "wall_s": round(wall, 3), "setup_s": round(setup_s, 1)}
 
 
+def tune_all():
+    """Runs the launch-shape search (autotune.tune, forced) for the plain
+    2D and 3D workloads and the 256-instance batch, each on a scratch handle,
+    and returns the table entries (signature -> shape and times)."""
+    import torch
+    from pfem2dealii_amd import BatchHandler, ParticleHandler, workloads as W
+    from pfem2dealii_amd.autotune import tune
+    out = {}
+    for workload in ("taylor_green2d", "abc3d"):
+        cfg = W.config(workload)
+        V = torch.tensor(cfg["V"], dtype=torch.float64, device="cuda")
+        Vp = torch.empty_like(V)
+        h = ParticleHandler(cfg["mesh"], cfg["params"])
+        h.initialize_maps()
+        h.insert_particles(cfg["x"], cfg["cells"])
+        h.set_velocity_from_field(V)
+        h.reserve_particles(3 * len(cfg["cells"]))
+
+        def step():
+            h.move_particles(V)
+            h.project_particle_fields(Vp)
+            h.correct_particle_velocity(V, Vp)
+
+        sig = {"workload": workload, "scale": 1.0}
+        _, info = tune(h, step, signature=sig, force=True, log=progress)
+        out[json.dumps(sig, sort_keys=True)] = info
+        progress("tuned %s: %s" % (workload, json.dumps(info)))
+        h.close()
+        del V, Vp
+    inst = W.batched_instances(256, seed=4)
+    V = torch.tensor(np.concatenate([i["V"] for i in inst], axis=1), dtype=torch.float64, device="cuda")
+    Vp = torch.empty_like(V)
+    b = BatchHandler([i["mesh"] for i in inst], [i["params"] for i in inst])
+    b.initialize_maps()
+    co = b.cell_offsets
+    x = np.concatenate([i["x"] for i in inst], axis=1)
+    cells = np.concatenate([i["cells"] + co[k] for k, i in enumerate(inst)]).astype(np.int32)
+    b.insert_particles(x, cells)
+    b.set_velocity_from_field(V)
+    b.reserve_particles(3 * len(cells))
+    sig = {"workload": "batched", "n": 256, "scale": 1.0}
+    _, info = tune(b, lambda: b.step(V, Vp, use_graph=True), signature=sig, force=True, log=progress)
+    out[json.dumps(sig, sort_keys=True)] = info
+    progress("tuned batched: %s" % json.dumps(info))
+    b.close()
+    return out
+
+
+def progress(msg):
+    """One line per finished leg on stderr (stdout carries only the JSON)."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -634,12 +692,12 @@ def main():
     ap.add_argument("--adjoint-reps", type=int, default=3, help="repetitions of the adjoint cost measurement (0: skip)")
     ap.add_argument("--autotune", action="store_true", help="tune launch shapes instead of the shipped table")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="only run a plain 1 GiB device copy (counter calibration) and exit")
+                    help="only run 1 GiB device copies (counter calibration) and exit")
     ap.add_argument("--wire-reps", type=int, default=3,
                     help="repetitions of the particle-record pack/unpack measurement (0: skip)")
     ap.add_argument("--fem-reps", type=int, default=5,
                     help="repetitions of the FEM assembly measurement (0: skip)")
-    ap.add_argument("--section", choices=("batched", "adjoint", "wire", "fem"), default=None,
+    ap.add_argument("--section", choices=("batched", "adjoint", "wire", "fem", "tune"), default=None,
                     help="only run the batched comparison or the adjoint cost measurement (for profiling) and "
                          "print its result")
     args = ap.parse_args()
@@ -659,6 +717,9 @@ def main():
         print(json.dumps({"section": "adjoint", "adjoint": adjoint_cost(max(1, args.adjoint_reps)),
                           "adjoint3d": adjoint_cost(max(1, args.adjoint_reps), 4.0, "adjoint3d")}), flush=True)
         return
+    if args.section == "tune":
+        print(json.dumps({"section": "tune", "table": tune_all()}), flush=True)
+        return
     if args.section == "fem":
         print(json.dumps({"section": "fem", "fem": fem_cost(max(1, args.fem_reps), cpu=not args.no_cpu)}),
               flush=True)
@@ -669,6 +730,7 @@ def main():
         return
 
     plain = plain_run(args.workload, args.scale, args.steps, args.warmup, args.autotune)
+    progress("plain %s: %.4g particle-steps/s" % (args.workload, plain["value"]))
     out = {
         "metric": "particle-steps/s plain vs batched-256 on 1 GPU at 1e-11 rel err; %HBM peak",
         "value": plain["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -678,21 +740,28 @@ def main():
     }
     if args.workload == "taylor_green2d" and args.scale == 1.0 and not args.no_3d:
         d3 = plain_run("abc3d", 1.0, args.steps, args.warmup, args.autotune)
+        progress("plain abc3d: %.4g particle-steps/s" % d3["value"])
         out["abc3d"] = {k: d3[k] for k in ("value", "unit", "ms_per_step", "config", "roofline",
                                             "kernels_ms_per_step")}
     if args.batched_steps > 0:
         out["batched"] = batched_comparison(args.batched_steps, 1, force_tune=args.autotune)
+        progress("batched comparison done")
     if args.adjoint_reps > 0:
         out["adjoint"] = adjoint_cost(args.adjoint_reps)
         out["adjoint3d"] = adjoint_cost(args.adjoint_reps, 4.0, "adjoint3d")
+        progress("adjoint cost done")
     if args.wire_reps > 0:
         out["wire"] = wire_cost(args.workload, args.scale, args.wire_reps)
+        progress("record pack/unpack done")
     if args.fem_reps > 0 and args.scale == 1.0:
         out["fem"] = fem_cost(args.fem_reps, cpu=not args.no_cpu)
+        progress("FEM assembly done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload)
+        progress("CPU baseline %s done" % args.workload)
         if "abc3d" in out:
             out["abc3d"]["cpu_baseline"] = cpu_baseline("abc3d")
+            progress("CPU baseline abc3d done")
     print(json.dumps(out), flush=True)
 
 

@@ -241,6 +241,8 @@ typedef struct {
   int32_t settle_grid;  /* workgroups of the settle kernel: [1, 65536] */
   int32_t move_grid;    /* workgroups of the in-cell move pass: 0 = one particle per thread,
                            else [256, 1048576] workgroups that loop over the particles */
+  int32_t move_block;   /* threads per workgroup of the in-cell move pass: 64, 128, 256 or 512 */
+  int32_t walk_block;   /* threads per workgroup of a walk round: 64, 128, 256 or 512 */
 } pfem2gpu_tuning;
 int pfem2gpu_set_tuning(pfem2gpu_handle* h, const pfem2gpu_tuning* t);
 int pfem2gpu_get_tuning(pfem2gpu_handle* h, pfem2gpu_tuning* t);

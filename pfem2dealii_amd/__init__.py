@@ -43,10 +43,12 @@ class _Params(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     """Launch shape of the move passes and the settle kernel (pfem2gpu_tuning)."""
-    _fields_ = [("walk_grid", ctypes.c_int32), ("settle_grid", ctypes.c_int32), ("move_grid", ctypes.c_int32)]
+    _fields_ = [("walk_grid", ctypes.c_int32), ("settle_grid", ctypes.c_int32), ("move_grid", ctypes.c_int32),
+                ("move_block", ctypes.c_int32), ("walk_block", ctypes.c_int32)]
 
     def as_dict(self):
-        return {"walk_grid": self.walk_grid, "settle_grid": self.settle_grid, "move_grid": self.move_grid}
+        return {"walk_grid": self.walk_grid, "settle_grid": self.settle_grid, "move_grid": self.move_grid,
+                "move_block": self.move_block, "walk_block": self.walk_block}
 
 
 EXPORTS = [
@@ -541,8 +543,8 @@ class ParticleHandler:
         self.synchronize()
         return loss.value, gv0[:, :n0], gV
 
-    def set_tuning(self, walk_grid=4096, settle_grid=2048, move_grid=0):
-        t = Tuning(int(walk_grid), int(settle_grid), int(move_grid))
+    def set_tuning(self, walk_grid=4096, settle_grid=2048, move_grid=0, move_block=256, walk_block=256):
+        t = Tuning(int(walk_grid), int(settle_grid), int(move_grid), int(move_block), int(walk_block))
         _check(self.lib, self.lib.pfem2gpu_set_tuning(self.h, ctypes.byref(t)), "set_tuning")
 
     def get_tuning(self):

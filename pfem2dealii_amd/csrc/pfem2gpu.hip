@@ -158,6 +158,8 @@ struct pfem2gpu_handle {
   int walk_grid = 4096;     // workgroups of a walk round (multiple of kQStripes)
   int settle_grid = 2048;   // workgroups of the settle kernel
   int move_grid = 0;        // workgroups of the in-cell move pass (0: one particle per thread)
+  int move_block = kBlock;  // threads per workgroup of the in-cell move pass (64, 128, 256 or 512)
+  int walk_block = kBlock;  // threads per workgroup of a walk round (64, 128, 256 or 512)
   // graph-captured harness step
   bool capturing = false;
   hipGraph_t graph = nullptr;
@@ -325,16 +327,32 @@ void drop_graph(pfem2gpu_handle* h) {
 // one particle per thread), and the walk-queue stripe size for it: the
 // particles of the ceil(grid / kQStripes) workgroups that write to one stripe.
 long move_cell_grid(const pfem2gpu_handle* h, long n) {
-  long g = std::max(1L, std::min((n + kBlock - 1) / kBlock, 1L << 20));
+  const long b = h->move_block;
+  long g = std::max(1L, std::min((n + b - 1) / b, 1L << 20));
   if (h->move_grid > 0) g = std::min(g, (long)h->move_grid);
   return g;
 }
 
+// particles one walk-queue stripe may receive: those of the in-cell move's
+// workgroups that write to it (workgroup b writes to stripe b % kQStripes)
 long walk_qreg(const pfem2gpu_handle* h, long n) {
+  const long b = h->move_block;
   const long gc = move_cell_grid(h, n);
-  const long per_block = (long)kBlock * ((n + gc * kBlock - 1) / (gc * kBlock));
+  const long per_block = b * ((n + gc * b - 1) / (gc * b));
   return ((gc + kQStripes - 1) / kQStripes) * per_block;
 }
+
+// The move kernels are instantiated for the workgroup sizes the tuning
+// allows; bs selects one.
+#define MOVE_LAUNCH(h, name, KERNEL, D, bs, grid, ...)                                  \
+  do {                                                                                \
+    switch (bs) {                                                                     \
+      case 64: LAUNCH_B(h, name, (KERNEL<D, 64>), grid, 64, 0, __VA_ARGS__); break;    \
+      case 128: LAUNCH_B(h, name, (KERNEL<D, 128>), grid, 128, 0, __VA_ARGS__); break; \
+      case 512: LAUNCH_B(h, name, (KERNEL<D, 512>), grid, 512, 0, __VA_ARGS__); break; \
+      default: LAUNCH_B(h, name, (KERNEL<D, 256>), grid, 256, 0, __VA_ARGS__); break;  \
+    }                                                                                 \
+  } while (0)
 
 void free_queues(pfem2gpu_handle* h) {
   for (int b = 0; b < 2; ++b) {
@@ -601,13 +619,14 @@ int rebin_or_move(pfem2gpu_handle* h, int advect, int n_add, DField Vf) {
     if (h->capturing) return fail(PFEM2GPU_ERR_STATE, "walk queues too small during graph capture");
     CHK(grow_queues(h, qreg * kQStripes));
   }
-  LAUNCH(h, advect ? "k_move" : "k_rebin", k_move_cell<D>, gc, 0, advect, m, h->Vc, nptr, n_add, P, h->dVc, use_dv,
-         h->cshift, h->capc, bcount, spill_base, spill_n, T, dwalk(h, 0), h->qn, qreg, h->rec_path);
+  MOVE_LAUNCH(h, advect ? "k_move" : "k_rebin", k_move_cell, D, h->move_block, gc, advect, m, h->Vc, nptr, n_add, P,
+              h->dVc, use_dv, h->cshift, h->capc, bcount, spill_base, spill_n, T, dwalk(h, 0), h->qn, qreg,
+              h->rec_path);
   const int gw = kQStripes * std::max(1, h->walk_grid / kQStripes);
   for (int r = 0; r < rounds; ++r)
-    LAUNCH(h, "k_walk", k_move_walk<D>, gw, 0, m, h->Vc, dwalk(h, r & 1), h->qn + (long)r * kQStripes, qreg,
-           h->cshift, h->capc, bcount, spill_base, spill_n, T, dwalk(h, (r + 1) & 1),
-           h->qn + (long)(r + 1) * kQStripes, h->rec_path);
+    MOVE_LAUNCH(h, "k_walk", k_move_walk, D, h->walk_block, gw, m, h->Vc, dwalk(h, r & 1),
+                h->qn + (long)r * kQStripes, qreg, h->cshift, h->capc, bcount, spill_base, spill_n, T,
+                dwalk(h, (r + 1) & 1), h->qn + (long)(r + 1) * kQStripes, h->rec_path);
   if (use_dv) h->pending_corr = false;
   LAUNCH(h, "k_spill", k_spill_count, grid_for(nmax_p), 0, spill_n, spill_base, h->tcell, h->cshift, scount);
   CHK(scan(h, scount, nch, h->sstart));
@@ -1538,13 +1557,13 @@ int replay_move(pfem2gpu_handle* h, const int* dcnt, int s, long n_in, long n_ou
   if (qreg * kQStripes > h->qcap) CHK(grow_queues(h, qreg * kQStripes));
   if (n_in > h->tsize) return fail(PFEM2GPU_ERR_STATE, "staging arrays smaller than the particle count");
   LAUNCH(h, "k_fill", k_fill_int, grid_for((long)h->qn_len * kQStripes), 0, h->qn, (long)h->qn_len * kQStripes, 0);
-  LAUNCH(h, "k_move", k_move_cell<D>, gc, 0, 1, m, h->Vc, dcnt + s - 1, 0, P, h->dVc, 0, h->cshift, -1, nullptr, 0L,
-         nullptr, M, dwalk(h, 0), h->qn, qreg, h->rec_path);
+  MOVE_LAUNCH(h, "k_move", k_move_cell, D, h->move_block, gc, 1, m, h->Vc, dcnt + s - 1, 0, P, h->dVc, 0, h->cshift,
+              -1, nullptr, 0L, nullptr, M, dwalk(h, 0), h->qn, qreg, h->rec_path);
   const int gw = kQStripes * std::max(1, h->walk_grid / kQStripes);
   for (int r = 0; r < rounds; ++r)
-    LAUNCH(h, "k_walk", k_move_walk<D>, gw, 0, m, h->Vc, dwalk(h, r & 1), h->qn + (long)r * kQStripes, qreg,
-           h->cshift, -1, nullptr, 0L, nullptr, M, dwalk(h, (r + 1) & 1), h->qn + (long)(r + 1) * kQStripes,
-           h->rec_path);
+    MOVE_LAUNCH(h, "k_walk", k_move_walk, D, h->walk_block, gw, m, h->Vc, dwalk(h, r & 1),
+                h->qn + (long)r * kQStripes, qreg, h->cshift, -1, nullptr, 0L, nullptr, M, dwalk(h, (r + 1) & 1),
+                h->qn + (long)(r + 1) * kQStripes, h->rec_path);
   if (n_out > 0) {
     LAUNCH(h, "k_adj_replay", k_adj_replay<D>, grid_for(n_out), 0, m, h->Vc, h->pshift, (int)n_out, src, ecell, M, P,
            h->cell_start + h->nc);
@@ -1734,6 +1753,13 @@ int pfem2gpu_set_tuning(pfem2gpu_handle* h, const pfem2gpu_tuning* t) {
   if (t->settle_grid < 1 || t->settle_grid > 65536) return fail(PFEM2GPU_ERR_ARG, "settle_grid must be in [1, 65536]");
   if (t->move_grid != 0 && (t->move_grid < 256 || t->move_grid > (1 << 20)))
     return fail(PFEM2GPU_ERR_ARG, "move_grid must be 0 or in [256, 1048576]");
+  for (int b : {t->move_block, t->walk_block})
+    if (b != 64 && b != 128 && b != 256 && b != 512)
+      return fail(PFEM2GPU_ERR_ARG, "move_block and walk_block must be 64, 128, 256 or 512");
+  if (h->capturing) return fail(PFEM2GPU_ERR_STATE, "tuning changed during graph capture");
+  // the walk queues are sized for the in-cell pass's stripes (walk_qreg)
+  h->move_block = t->move_block;
+  h->walk_block = t->walk_block;
   h->walk_grid = t->walk_grid;
   h->settle_grid = t->settle_grid;
   h->move_grid = t->move_grid;
@@ -1746,6 +1772,8 @@ int pfem2gpu_get_tuning(pfem2gpu_handle* h, pfem2gpu_tuning* t) {
   t->walk_grid = h->walk_grid;
   t->settle_grid = h->settle_grid;
   t->move_grid = h->move_grid;
+  t->move_block = h->move_block;
+  t->walk_block = h->walk_block;
   return PFEM2GPU_OK;
 }
 

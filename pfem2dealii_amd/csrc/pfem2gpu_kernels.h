@@ -323,8 +323,8 @@ __device__ __forceinline__ bool substeps_in_cell(const DMesh& m, const double* _
 // re-binning (used by insertion). A pending velocity correction is applied
 // from the pre-move cell and reference position first, as
 // correct_particle_velocity would have done before the move.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const double* __restrict__ Vc,
+template <int D, int B>
+__global__ __launch_bounds__(B) void k_move_cell(int advect, DMesh m, const double* __restrict__ Vc,
                                                      const int* __restrict__ n_ptr, int n_add, DParts P,
                                                      const double* __restrict__ dVc, int use_dv, int cshift, int capc,
                                                      int* __restrict__ bcount,
@@ -379,8 +379,8 @@ __global__ __launch_bounds__(kBlock) void k_move_cell(int advect, DMesh m, const
 // Walk round: workgroup b takes stripe b % kQStripes of the input queue
 // (the grid is a multiple of kQStripes) and writes to the same stripe of the
 // output queue.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_move_walk(DMesh m, const double* __restrict__ Vc, DWalk Qin,
+template <int D, int B>
+__global__ __launch_bounds__(B) void k_move_walk(DMesh m, const double* __restrict__ Vc, DWalk Qin,
                                                      const int* __restrict__ qn_in, long qreg, int cshift, int capc,
                                                      int* __restrict__ bcount, long spill_base,
                                                      int* __restrict__ spill_n, DTmp T, DWalk Qout,

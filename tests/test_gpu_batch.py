@@ -146,9 +146,10 @@ def test_tuning_does_not_change_results(gpu_available):
     from pfem2dealii_amd import ParticleHandler
     cfg = W.config("taylor_green2d", scale=1.0 / 16)
     res = []
-    for tune in ({"walk_grid": 4096, "settle_grid": 2048, "move_grid": 0},
-                 {"walk_grid": 1024, "settle_grid": 7, "move_grid": 256},
-                 {"walk_grid": 65536, "settle_grid": 300, "move_grid": 1000}):
+    for tune in ({"walk_grid": 4096, "settle_grid": 2048, "move_grid": 0, "move_block": 256, "walk_block": 256},
+                 {"walk_grid": 1024, "settle_grid": 7, "move_grid": 256, "move_block": 64, "walk_block": 512},
+                 {"walk_grid": 65536, "settle_grid": 300, "move_grid": 1000, "move_block": 512, "walk_block": 64},
+                 {"walk_grid": 2048, "settle_grid": 4096, "move_grid": 0, "move_block": 128, "walk_block": 128}):
         h = ParticleHandler(cfg["mesh"], cfg["params"])
         h.set_tuning(**tune)
         assert h.get_tuning() == tune

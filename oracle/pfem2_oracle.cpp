@@ -14,7 +14,8 @@
 //     choices the reference leaves to std::vector order made deterministic:
 //     survivors per (cell, part) are the max+1 smallest canonical keys, the
 //     particle order inside a cell is (part, key), and projection sums run per
-//     cell in that order, then per node over incident cells in ascending id.
+//     cell in that order (in 4 contiguous parts in 2D, 2 in 3D, added in
+//     order), then per node over incident cells in ascending id.
 //     This is the mode the GPU library must match bit for bit.
 //   ORC_MODE_REFERENCE (1): the reference's own loop structure: AoS
 //     std::vector, swap-with-back deletion, std::map counting, projection in
@@ -520,18 +521,34 @@ struct Oracle {
         }
       }
     } else {
-      // per cell sequential sums in the stored order, then per node over
-      // incident cells in ascending cell id
-      std::vector<double> cs((size_t)nc * nvc * (dim + 1), 0.0);
+      // per cell sums in the stored order, each formed in Q contiguous parts
+      // (Q = 4 in 2D, 2 in 3D; part j holds the cell's particles at
+      // positions [n j / Q, n (j + 1) / Q) of its n, summed sequentially from
+      // zero; the parts added in order), then per node over incident cells
+      // in ascending cell id
+      const int Q = dim == 2 ? 4 : 2;
+      std::vector<int64_t> cnt(nc, 0), seen(nc, 0);
+      for (const auto& p : P) cnt[p.cell]++;
+      std::vector<double> part((size_t)nc * Q * nvc * (dim + 1), 0.0);
       for (const auto& p : P) {
+        const int64_t r = seen[p.cell]++;
+        int j = 0;
+        while (j + 1 < Q && r >= ((j + 1) * cnt[p.cell]) / Q) ++j;
         double N[8];
         shape(p.xi, N);
-        double* e = &cs[(size_t)p.cell * nvc * (dim + 1)];
+        double* e = &part[((size_t)p.cell * Q + j) * nvc * (dim + 1)];
         for (int i = 0; i < nvc; ++i) {
           for (int k = 0; k < dim; ++k) e[i * (dim + 1) + k] = e[i * (dim + 1) + k] + N[i] * p.v[k];
           e[i * (dim + 1) + dim] = e[i * (dim + 1) + dim] + N[i];
         }
       }
+      std::vector<double> cs((size_t)nc * nvc * (dim + 1), 0.0);
+      for (int64_t c = 0; c < nc; ++c)
+        for (int t = 0; t < nvc * (dim + 1); ++t) {
+          double a = part[((size_t)c * Q) * nvc * (dim + 1) + t];
+          for (int j = 1; j < Q; ++j) a = a + part[((size_t)c * Q + j) * nvc * (dim + 1) + t];
+          cs[(size_t)c * nvc * (dim + 1) + t] = a;
+        }
       for (int64_t c = 0; c < nc; ++c)
         for (int i = 0; i < nvc; ++i) {
           const int j = cd[(size_t)c * nvc + i];

@@ -491,12 +491,13 @@ __global__ __launch_bounds__(B) void k_move_walk(DMesh m, const double* __restri
 //  D. particles are grouped by (cell, part) and ranked inside their group by
 //     key, which gives every survivor its final slot (cell base + part base
 //     + rank, i.e. (part, key) order); refills are built in their slots;
-//  E1. the (cell, vertex) sums are accumulated sequentially in final order;
+//  E1. the (cell, vertex) sums are accumulated in final order, each in
+//      kSumParts contiguous parts (one thread per part) added in order;
 //  C. only then is the chunk's output offset resolved from the totals of
 //     the chunks before it (decoupled look-back, its first loads issued
 //     before E1), so that the wait overlaps the chunk's own work;
 //  E2. the chunk is written out coalesced in final order, with the new
-//     cell_start.
+//     cell_start, by every thread after its part of E1.
 // A chunk with more than `nmax` particles resolves its offset in B and takes
 // the remaining steps with its particles in global memory (grouping through
 // the `gord` scratch array, ranking over the group only); it leaves its
@@ -713,9 +714,16 @@ static_assert(kSettleShift <= 7, "chunk-local cell ids must fit seven bits");
 // the arrival-order arrays (key, order, xi, v, group-order key) share the
 // bytes of the final-order ones (x, v, xi, src, cell byte)
 static_assert(8 + 4 + 16 * 2 + 8 <= 24 * 2 + 5 && 8 + 4 + 16 * 3 + 8 <= 24 * 3 + 5, "arrival arrays exceed the slot");
-// the projection sums take one thread per (cell, vertex) of a chunk on the
-// wavefronts after the first, and the write-out at least the first
-static_assert((1 << kSettleShift) * 8 <= kSettleBlock - 64, "too few threads for the projection sums");
+// The projection sum of a (cell, vertex) is formed in kSumParts<D> parts:
+// part j holds the cell's particles at positions [n j / Q, n (j + 1) / Q)
+// (integer division) of its n particles in final order, summed sequentially
+// from zero, and the parts are added in order j = 0, 1, ... The settle kernel
+// takes one thread per (cell, vertex, part) of a chunk, the parts of a sum on
+// neighbouring lanes of one wavefront.
+template <int D>
+constexpr int kSumParts = kSettleBlock / ((1 << kSettleShift) << D);
+static_assert(kSumParts<2> == 4 && kSumParts<3> == 2, "parts per projection sum");
+static_assert(64 % kSumParts<2> == 0 && 64 % kSumParts<3> == 0, "the parts of a sum share a wavefront");
 
 // Cell data of a chunk (box geometry, advecting field, instance and its
 // parameters), carried in registers from its loads to its stores into an LDS
@@ -1215,64 +1223,75 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       const int nnb = min(s_misc[5], capc);
       load_parts(nnb + s_misc[7] - s_misc[6], nnb, (long)nch * capc, spill2_base + s_misc[6]);
     }
-    // E1 on the wavefronts after the first (one thread per (cell, vertex)),
-    // E2 on the others
-    const int e1_waves = (cpw * NV + 63) >> 6;
-    const int wave = threadIdx.x >> 6;
-    if (staged && wave >= 1 && wave <= e1_waves) {
+    if (staged) {
       // E1. per (cell, vertex) sums S_k = sum N_i v_k and W = sum N_i over
-      //     the cell's particles in final order, each component a sequential
-      //     sum
+      //     the cell's particles in final order, each formed in kSumParts
+      //     contiguous parts (sequential inside a part, the parts added in
+      //     order): one thread per (cell, vertex, part), the parts of one
+      //     sum on neighbouring lanes
+      constexpr int QP = kSumParts<D>;
       constexpr int G = 4;  // the loads of a group of particles issued together
-      const int e1 = (int)threadIdx.x - 64;
-      if (e1 < ncell * NV) {
-        const int lc = e1 >> D, iv = e1 & (NV - 1);
-        const int a = s_cfin[lc], b = (lc + 1 < ncell) ? s_cfin[lc + 1] : nout;
-        double S[D], W = 0.0;
+      const int chain = (int)threadIdx.x / QP, part = (int)threadIdx.x % QP;
+      const int lc = chain >> D, iv = chain & (NV - 1);
+      int a = 0, b = 0;
+      if (lc < ncell) {
+        const int ca = s_cfin[lc], cb = (lc + 1 < ncell) ? s_cfin[lc + 1] : nout;
+        a = ca + (part * (cb - ca)) / QP;
+        b = ca + ((part + 1) * (cb - ca)) / QP;
+      }
+      double S[D], W = 0.0;
 #pragma unroll
-        for (int k = 0; k < D; ++k) S[k] = 0.0;
-        int f = a;
-        for (; f + G <= b; f += G) {
-          double N[G], w[G][D];
+      for (int k = 0; k < D; ++k) S[k] = 0.0;
+      int f = a;
+      for (; f + G <= b; f += G) {
+        double N[G], w[G][D];
 #pragma unroll
-          for (int e = 0; e < G; ++e) {
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-              const double xk = s_fxi[k * ns + f + e];
-              const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
-              N[e] = (k == 0) ? l : N[e] * l;
-              w[e][k] = s_fv[k * ns + f + e];
-            }
-          }
-#pragma unroll
-          for (int e = 0; e < G; ++e) {
-#pragma unroll
-            for (int k = 0; k < D; ++k) S[k] = S[k] + N[e] * w[e][k];
-            W = W + N[e];
-          }
-        }
-        for (; f < b; ++f) {
-          double N = 1.0;
+        for (int e = 0; e < G; ++e) {
 #pragma unroll
           for (int k = 0; k < D; ++k) {
-            const double xk = s_fxi[k * ns + f];
+            const double xk = s_fxi[k * ns + f + e];
             const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
-            N = (k == 0) ? l : N * l;
+            N[e] = (k == 0) ? l : N[e] * l;
+            w[e][k] = s_fv[k * ns + f + e];
           }
-#pragma unroll
-          for (int k = 0; k < D; ++k) S[k] = S[k] + N * s_fv[k * ns + f];
-          W = W + N;
         }
+#pragma unroll
+        for (int e = 0; e < G; ++e) {
+#pragma unroll
+          for (int k = 0; k < D; ++k) S[k] = S[k] + N[e] * w[e][k];
+          W = W + N[e];
+        }
+      }
+      for (; f < b; ++f) {
+        double N = 1.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          const double xk = s_fxi[k * ns + f];
+          const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
+          N = (k == 0) ? l : N * l;
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) S[k] = S[k] + N * s_fv[k * ns + f];
+        W = W + N;
+      }
+      const int lane0 = ((int)threadIdx.x & 63) - part;  // the lane of part 0
+      double T[D], TW = __shfl(W, lane0, 64);
+#pragma unroll
+      for (int k = 0; k < D; ++k) T[k] = __shfl(S[k], lane0, 64);
+#pragma unroll
+      for (int j = 1; j < QP; ++j) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) T[k] = T[k] + __shfl(S[k], lane0 + j, 64);
+        TW = TW + __shfl(W, lane0 + j, 64);
+      }
+      if (part == 0 && lc < ncell) {
         double* co = contrib + ((size_t)(c0 + lc) * NV + iv) * (D + 1);
 #pragma unroll
-        for (int k = 0; k < D; ++k) co[k] = S[k];
-        co[D] = W;
+        for (int k = 0; k < D; ++k) co[k] = T[k];
+        co[D] = TW;
       }
-    } else if (staged) {
       // E2. coalesced write-out in final order
-      const int e2 = wave == 0 ? (int)threadIdx.x : (int)threadIdx.x - 64 * e1_waves;
-      const int ne2 = (int)blockDim.x - 64 * e1_waves;
-      for (int t = e2; t < nout; t += ne2) {
+      for (int t = threadIdx.x; t < nout; t += blockDim.x) {
 #pragma unroll
         for (int k = 0; k < D; ++k) {
           P.x[k][o0 + t] = s_fx[k * ns + t];
@@ -1406,28 +1425,43 @@ __device__ __forceinline__ void contrib_one(const DMesh& m, int c, int i, const 
   const Prm pr = m.prm[inst_of(m, c)];
   int vid[NV];
   load_ids<D>(m.cv, c, vid);
-  double S[D], W = 0.0;
+  constexpr int QP = kSumParts<D>;
+  double T[D], TW = 0.0;
 #pragma unroll
-  for (int k = 0; k < D; ++k) S[k] = 0.0;
-  const int s = cs[c], e = cs[c + 1];
-  for (int p = s; p < e; ++p) {
-    double x[D], xi[D], N[NV];
+  for (int k = 0; k < D; ++k) T[k] = 0.0;
+  const int s = cs[c], n = cs[c + 1] - s;
+  for (int j = 0; j < QP; ++j) {  // the parts of the settle kernel's sums
+    double S[D], W = 0.0;
 #pragma unroll
-    for (int k = 0; k < D; ++k) x[k] = P.x[k][p];
-    recover_xi_c<D>(m, pr, c, placed_of(P.cell[p]), x, xi);
-    shape<D>(xi, N);
-    double Ni = N[0];
+    for (int k = 0; k < D; ++k) S[k] = 0.0;
+    for (int p = s + (j * n) / QP; p < s + ((j + 1) * n) / QP; ++p) {
+      double x[D], xi[D], N[NV];
 #pragma unroll
-    for (int q = 1; q < NV; ++q)
-      if (q == i) Ni = N[q];
+      for (int k = 0; k < D; ++k) x[k] = P.x[k][p];
+      recover_xi_c<D>(m, pr, c, placed_of(P.cell[p]), x, xi);
+      shape<D>(xi, N);
+      double Ni = N[0];
 #pragma unroll
-    for (int k = 0; k < D; ++k) S[k] = S[k] + Ni * P.v[k][p];
-    W = W + Ni;
+      for (int q = 1; q < NV; ++q)
+        if (q == i) Ni = N[q];
+#pragma unroll
+      for (int k = 0; k < D; ++k) S[k] = S[k] + Ni * P.v[k][p];
+      W = W + Ni;
+    }
+    if (j == 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) T[k] = S[k];
+      TW = W;
+    } else {
+#pragma unroll
+      for (int k = 0; k < D; ++k) T[k] = T[k] + S[k];
+      TW = TW + W;
+    }
   }
   double* e_out = contrib + ((size_t)c * NV + i) * (D + 1);
 #pragma unroll
-  for (int k = 0; k < D; ++k) e_out[k] = S[k];
-  e_out[D] = W;
+  for (int k = 0; k < D; ++k) e_out[k] = T[k];
+  e_out[D] = TW;
 }
 
 template <int D>

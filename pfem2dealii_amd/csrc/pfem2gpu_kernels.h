@@ -750,15 +750,17 @@ __device__ __forceinline__ void cells_load(const DMesh& m, const double* __restr
   // the per-lane addresses below are rebuilt at every call rather than kept
   // across the settle loop (where they would be spilled)
   asm volatile("" : "+v"(t));
+  // the parameter words first: their address depends on a load (the cell's
+  // instance) in a batch, and the wait for it then covers no other load
+  r.prm = t < ncell * kPrmWords ? reinterpret_cast<const double*>(&m.prm[inst_of(m, c0 + t / kPrmWords)])[t % kPrmWords]
+                                : 0.0;
+  r.inst = t < ncell ? inst_of(m, c0 + t) : 0;
   r.geo = t < ncell * 3 * D ? m.geom[(size_t)c0 * 3 * D + t] : 0.0;
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int i = t + q * kSettleBlock;
     r.vc[q] = (do_refill && i < ncell * NV * D) ? Vc[(size_t)c0 * NV * D + i] : 0.0;
   }
-  r.prm = t < ncell * kPrmWords ? reinterpret_cast<const double*>(&m.prm[inst_of(m, c0 + t / kPrmWords)])[t % kPrmWords]
-                                : 0.0;
-  r.inst = t < ncell ? inst_of(m, c0 + t) : 0;
 }
 
 template <int D>
@@ -1262,27 +1264,45 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
           W = W + N[e];
         }
       }
-      for (; f < b; ++f) {
-        double N = 1.0;
+      {  // the last G - 1 or fewer, their loads issued together
+        double N[G - 1], w[G - 1][D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-          const double xk = s_fxi[k * ns + f];
-          const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
-          N = (k == 0) ? l : N * l;
+        for (int e = 0; e < G - 1; ++e) {
+          const int g = f + e < b ? f + e : f;
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            const double xk = s_fxi[k * ns + g];
+            const double l = ((iv >> k) & 1) ? xk : (1.0 - xk);
+            N[e] = (k == 0) ? l : N[e] * l;
+            w[e][k] = s_fv[k * ns + g];
+          }
         }
 #pragma unroll
-        for (int k = 0; k < D; ++k) S[k] = S[k] + N * s_fv[k * ns + f];
-        W = W + N;
-      }
-      const int lane0 = ((int)threadIdx.x & 63) - part;  // the lane of part 0
-      double T[D], TW = __shfl(W, lane0, 64);
+        for (int e = 0; e < G - 1; ++e) {
+          if (f + e < b) {
 #pragma unroll
-      for (int k = 0; k < D; ++k) T[k] = __shfl(S[k], lane0, 64);
+            for (int k = 0; k < D; ++k) S[k] = S[k] + N[e] * w[e][k];
+            W = W + N[e];
+          }
+        }
+      }
+      // the parts of this sum, read from their lanes together, added in order
+      const int lane0 = ((int)threadIdx.x & 63) - part;  // the lane of part 0
+      double PS[QP][D + 1];
+#pragma unroll
+      for (int j = 0; j < QP; ++j) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) PS[j][k] = __shfl(S[k], lane0 + j, 64);
+        PS[j][D] = __shfl(W, lane0 + j, 64);
+      }
+      double T[D], TW = PS[0][D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) T[k] = PS[0][k];
 #pragma unroll
       for (int j = 1; j < QP; ++j) {
 #pragma unroll
-        for (int k = 0; k < D; ++k) T[k] = T[k] + __shfl(S[k], lane0 + j, 64);
-        TW = TW + __shfl(W, lane0 + j, 64);
+        for (int k = 0; k < D; ++k) T[k] = T[k] + PS[j][k];
+        TW = TW + PS[j][D];
       }
       if (part == 0 && lc < ncell) {
         double* co = contrib + ((size_t)(c0 + lc) * NV + iv) * (D + 1);

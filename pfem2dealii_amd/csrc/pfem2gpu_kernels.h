@@ -1310,16 +1310,35 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         for (int k = 0; k < D; ++k) co[k] = T[k];
         co[D] = TW;
       }
-      // E2. coalesced write-out in final order
-      for (int t = threadIdx.x; t < nout; t += blockDim.x) {
+      // E2. coalesced write-out in final order, two particles per thread
+      //     and round (their LDS reads issued together)
+      for (int t0 = threadIdx.x; t0 < nout; t0 += 2 * kSettleBlock) {
+        double ox[2][D], ov[2][D];
+        int fc[2], fs[2];
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-          P.x[k][o0 + t] = s_fx[k * ns + t];
-          P.v[k][o0 + t] = s_fv[k * ns + t];
+        for (int h = 0; h < 2; ++h) {
+          const int t = min(t0 + h * kSettleBlock, nout - 1);
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            ox[h][k] = s_fx[k * ns + t];
+            ov[h][k] = s_fv[k * ns + t];
+          }
+          fc[h] = s_fcell[t];
+          fs[h] = s_fsrc[t];
         }
-        const int fc = s_fcell[t];
-        P.cell[o0 + t] = (c0 + (fc & 0x7f)) | ((fc & 0x80) ? kPlaced : 0);
-        if (P.src) P.src[o0 + t] = s_fsrc[t];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int t = t0 + h * kSettleBlock;
+          if (t < nout) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+              P.x[k][o0 + t] = ox[h][k];
+              P.v[k][o0 + t] = ov[h][k];
+            }
+            P.cell[o0 + t] = (c0 + (fc[h] & 0x7f)) | ((fc[h] & 0x80) ? kPlaced : 0);
+            if (P.src) P.src[o0 + t] = fs[h];
+          }
+        }
       }
     }
     for (int lc = threadIdx.x; lc < ncell; lc += blockDim.x) cs_out[c0 + lc] = o0 + s_cfin[lc];

@@ -72,6 +72,7 @@ struct pfem2gpu_handle {
   double* v2c_dir = nullptr;
   int* d2c_off = nullptr;
   int* d2c = nullptr;
+  int* bpos = nullptr;  // [nc*nvc] place of each (cell, vertex) in the d2c lists
   Prm* prm = nullptr;
   int* cell_inst = nullptr;          // [nc] instance of each cell (batches only)
   bool all_box = false;              // every cell is an axis-aligned box
@@ -105,7 +106,7 @@ struct pfem2gpu_handle {
   long tsize = 0;             // staging slots: nchunks*capc + 2*cap
   int* scan_blk = nullptr;
   int scan_nb = 0;
-  double* contrib = nullptr;  // [nc][nvc][dim+1]
+  double* contrib = nullptr;  // [nc*nvc][dim+1] per (cell, vertex) sums in d2c order (place bpos)
   double* dVc = nullptr;  // [nc][nvc][dim] pending correction V - V_old, cell-major
   bool pending_corr = false;
   double* stageF[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
@@ -137,7 +138,6 @@ struct pfem2gpu_handle {
   std::vector<int32_t> fem_col;     // [nnz] ascending columns per row (host)
   long* fem_soff = nullptr;         // [nnz+1] contributions of each entry (runs in fem_A)
   int* fem_apos = nullptr;          // [nc*nvc*nvc] place of cell-matrix entry (c, i, j) in fem_A
-  int* fem_bpos = nullptr;          // [nc*nvc] place of right-hand side entry (c, i) in fem_b
   double* fem_A = nullptr;          // cell-matrix entries in contribution order (entry, then cell)
   double* fem_b = nullptr;          // [dim][nc*nvc] cell right-hand sides in DoF -> (cell, vertex) order
   double* fem_vals = nullptr;       // [nnz] staging of a host matrix output
@@ -291,6 +291,7 @@ DMesh dmesh(pfem2gpu_handle* h) {
   m.cvo = h->cvo;
   m.cvc = h->cvc;
   m.ptab = h->ptab;
+  m.bpos = h->bpos;
   return m;
 }
 
@@ -912,9 +913,14 @@ static int create_impl(const pfem2gpu_mesh* mesh, const std::vector<Prm>& prms, 
     for (int j = 0; j < h->ndofs; ++j) off[j + 1] += off[j];
     std::vector<int> pos(off.begin(), off.end() - 1);
     for (long e = 0; e < (long)h->nc * nvc; ++e) lst[pos[cdofs[e]]++] = (int)e;
-    if ((rc = dalloc(&h->d2c_off, h->ndofs + 1)) || (rc = dalloc(&h->d2c, lst.size()))) return bail(rc);
+    std::vector<int> bp(lst.size());
+    for (size_t e = 0; e < lst.size(); ++e) bp[lst[e]] = (int)e;
+    if ((rc = dalloc(&h->d2c_off, h->ndofs + 1)) || (rc = dalloc(&h->d2c, lst.size())) ||
+        (rc = dalloc(&h->bpos, bp.size())))
+      return bail(rc);
     if (hipMemcpy(h->d2c_off, off.data(), off.size() * 4, hipMemcpyHostToDevice) ||
-        hipMemcpy(h->d2c, lst.data(), lst.size() * 4, hipMemcpyHostToDevice))
+        hipMemcpy(h->d2c, lst.data(), lst.size() * 4, hipMemcpyHostToDevice) ||
+        hipMemcpy(h->bpos, bp.data(), bp.size() * 4, hipMemcpyHostToDevice))
       return bail(fail(PFEM2GPU_ERR_HIP, "upload dof map"));
   }
   if ((rc = dalloc(&h->prm, prms.size()))) return bail(rc);
@@ -1032,6 +1038,7 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
   dfree(h->v2c_dir);
   dfree(h->d2c_off);
   dfree(h->d2c);
+  dfree(h->bpos);
   dfree(h->prm);
   dfree(h->cell_inst);
   dfree(h->geom);
@@ -1052,7 +1059,6 @@ int pfem2gpu_destroy(pfem2gpu_handle* h) {
   dfree(h->dirty);
   dfree(h->fem_soff);
   dfree(h->fem_apos);
-  dfree(h->fem_bpos);
   dfree(h->fem_A);
   dfree(h->fem_b);
   dfree(h->fem_vals);
@@ -1299,7 +1305,7 @@ int pfem2gpu_project_particle_fields(pfem2gpu_handle* h, double* const* V_out, i
              h->dirty, h->cell_start, dparts(h), h->contrib);
     else
       LAUNCH(h, "k_contrib", k_contrib<2>, grid_for(ne), 0, dmesh(h), h->nc, h->cell_start, dparts(h), h->contrib);
-    LAUNCH(h, "k_node", k_node<2>, grid_for(h->ndofs), 0, h->ndofs, h->d2c_off, h->d2c, h->contrib, out[0], out[1],
+    LAUNCH(h, "k_node", k_node<2>, grid_for(h->ndofs), 0, h->ndofs, h->d2c_off, h->contrib, out[0], out[1],
            out[2]);
   } else {
     if (h->contrib_valid)
@@ -1307,7 +1313,7 @@ int pfem2gpu_project_particle_fields(pfem2gpu_handle* h, double* const* V_out, i
              h->dirty, h->cell_start, dparts(h), h->contrib);
     else
       LAUNCH(h, "k_contrib", k_contrib<3>, grid_for(ne), 0, dmesh(h), h->nc, h->cell_start, dparts(h), h->contrib);
-    LAUNCH(h, "k_node", k_node<3>, grid_for(h->ndofs), 0, h->ndofs, h->d2c_off, h->d2c, h->contrib, out[0], out[1],
+    LAUNCH(h, "k_node", k_node<3>, grid_for(h->ndofs), 0, h->ndofs, h->d2c_off, h->contrib, out[0], out[1],
            out[2]);
   }
   h->contrib_valid = true;
@@ -1648,7 +1654,7 @@ int adjoint_impl(pfem2gpu_handle* h, int n_steps, int K, const double* const* Vu
   const DMesh m = dmesh(h);
   LAUNCH(h, "k_contrib_dirty", k_contrib_dirty<D>, std::min((h->nchunks + kBlock - 1) / kBlock, 4096), 0, m, nc, h->cpw, h->dirty,
          h->cell_start, dparts(h), h->contrib);
-  LAUNCH(h, "k_node_w", k_node_w<D>, grid_for(nd), 0, nd, h->d2c_off, h->d2c, h->contrib, Pn[0], Pn[1], Pn[2], W);
+  LAUNCH(h, "k_node_w", k_node_w<D>, grid_for(nd), 0, nd, h->d2c_off, h->contrib, Pn[0], Pn[1], Pn[2], W);
   LAUNCH(h, "k_fill", k_fill_u64, 1, 0, floss, 2L);
   DField Pf;
   for (int k = 0; k < 3; ++k) Pf.c[k] = Pn[k];
@@ -2192,18 +2198,17 @@ int fem_prepare(pfem2gpu_handle* h) {
       for (int q = off[r]; q < off[r + 1]; ++q)
         for (int j = 0; j < nvc; ++j) src[pos[rp[r] + cp[t++]]++] = lst[q] * nvc + j;
   }
-  // the places of the cell entries in contribution order (inverse lists)
-  std::vector<int> apos(src.size()), bpos(lst.size());
+  // the places of the cell-matrix entries in contribution order (the
+  // inverse list); right-hand side entries use the d2c places (bpos)
+  std::vector<int> apos(src.size());
   for (size_t e = 0; e < src.size(); ++e) apos[src[e]] = (int)e;
-  for (size_t e = 0; e < lst.size(); ++e) bpos[lst[e]] = (int)e;
   int rc;
   if ((rc = dalloc(&h->fem_soff, soff.size())) || (rc = dalloc(&h->fem_apos, apos.size())) ||
-      (rc = dalloc(&h->fem_bpos, bpos.size())) || (rc = dalloc(&h->fem_A, (size_t)nc * nvc * nvc)) ||
+      (rc = dalloc(&h->fem_A, (size_t)nc * nvc * nvc)) ||
       (rc = dalloc(&h->fem_b, (size_t)nc * nvc * h->dim)))
     return rc;
   HIPCHK(hipMemcpy(h->fem_soff, soff.data(), soff.size() * sizeof(long), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->fem_apos, apos.data(), apos.size() * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->fem_bpos, bpos.data(), bpos.size() * 4, hipMemcpyHostToDevice));
   h->fem_rowptr.swap(rp);
   h->fem_col.swap(col);
   h->fem_ready = true;
@@ -2247,13 +2252,13 @@ int fem_assemble(pfem2gpu_handle* h, bool pressure, const double* const* Vp, con
   const int grid = fem_grid(h->nc, h->nvc);
   if (pressure) {
     LAUNCH_B(h, "k_fem_pressure", (k_fem_cells<D, true>), grid, kFemBlock, 0, dmesh(h), h->nc, Vf, nullptr, c0, c1,
-             h->fem_apos, h->fem_bpos, h->fem_A, h->fem_b);
+             h->fem_apos, h->bpos, h->fem_A, h->fem_b);
     if (fd)
-      LAUNCH(h, "k_fem_faces", k_fem_faces<D>, grid_for(h->nc), 0, dmesh(h), h->nc, Vf, c0, fd, h->fem_bpos,
+      LAUNCH(h, "k_fem_faces", k_fem_faces<D>, grid_for(h->nc), 0, dmesh(h), h->nc, Vf, c0, fd, h->bpos,
              h->fem_b);
   } else {
     LAUNCH_B(h, "k_fem_vcorr", (k_fem_cells<D, false>), grid, kFemBlock, 0, dmesh(h), h->nc, Vf, Pd, c0, c1,
-             h->fem_apos, h->fem_bpos, h->fem_A, h->fem_b);
+             h->fem_apos, h->bpos, h->fem_A, h->fem_b);
   }
   const long nnz = (long)h->fem_col.size();
   if (matrix) {

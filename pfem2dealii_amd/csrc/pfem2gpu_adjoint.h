@@ -82,7 +82,7 @@ __global__ void k_fill_double(double* __restrict__ a, long n) {
 // nodal projection P = S / W and the weights W (same sums as k_node)
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_node_w(int ndofs, const int* __restrict__ d2c_off,
-                                                  const int* __restrict__ d2c, const double* __restrict__ contrib,
+                                                  const double* __restrict__ contrib,
                                                   double* p0, double* p1, double* p2, double* __restrict__ W) {
   double* out[3] = {p0, p1, p2};
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ndofs; j += gridDim.x * blockDim.x) {
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_node_w(int ndofs, const int* __restr
 #pragma unroll
     for (int k = 0; k < D; ++k) S[k] = 0.0;
     for (int e = d2c_off[j]; e < d2c_off[j + 1]; ++e) {
-      const double* c = contrib + (size_t)d2c[e] * (D + 1);
+      const double* c = contrib + (size_t)e * (D + 1);
 #pragma unroll
       for (int k = 0; k < D; ++k) S[k] = S[k] + c[k];
       w = w + c[D];

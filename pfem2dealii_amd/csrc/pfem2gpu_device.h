@@ -47,6 +47,7 @@ struct DMesh {
   const int4* cvc;               // 2D: [nc][4] the (up to 4) cells around each cell vertex, in v2c order,
                                  // -1 padded; x = -2 when the vertex has more (null in 3D)
   const int* ptab;               // [ninst][pmax]: refillable part -> INT_MIN | pi_0 | pi_1<<8 | pi_2<<16, else 0
+  const int* bpos;               // [nc][2^D]: place of (cell, vertex) in the DoF -> (cell, vertex) lists
 };
 
 struct DField {

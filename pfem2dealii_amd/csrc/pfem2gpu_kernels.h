@@ -509,7 +509,7 @@ struct SettleLayout {
                                 // and v[D] (doubles) in arrival order and the keys in group order (u64)
   int off_cnt, off_base, off_roff;  // [cpw * pmax] per (cell, part)
   int off_cbuf[2];              // cell data of the current / next chunk
-  int off_geo, off_vc, off_prm, off_inst;  // offsets inside a cell data buffer
+  int off_geo, off_vc, off_prm, off_bpos;  // offsets inside a cell data buffer
   int off_cell;                 // per-cell count and final offset; misc; scan partials
   int bytes;
 };
@@ -533,8 +533,8 @@ __host__ __device__ inline SettleLayout settle_layout(int D, int cpw, int pmax, 
   c += 8 * (1 << D) * D * cpw;
   L.off_prm = c;
   c += (int)sizeof(Prm) * cpw;
-  L.off_inst = c;
-  c += 4 * cpw;
+  L.off_bpos = c;
+  c += 4 * cpw * (1 << D);
   c = (c + 15) & ~15;
   o = (o + 15) & ~15;
   L.off_cbuf[0] = o;
@@ -739,7 +739,7 @@ struct CellRegs {
   double geo;
   double vc[2];
   double prm;
-  int inst;
+  int bpos;  // place of a (cell, vertex) projection sum in the d2c order
 };
 
 template <int D>
@@ -754,7 +754,7 @@ __device__ __forceinline__ void cells_load(const DMesh& m, const double* __restr
   // instance) in a batch, and the wait for it then covers no other load
   r.prm = t < ncell * kPrmWords ? reinterpret_cast<const double*>(&m.prm[inst_of(m, c0 + t / kPrmWords)])[t % kPrmWords]
                                 : 0.0;
-  r.inst = t < ncell ? inst_of(m, c0 + t) : 0;
+  r.bpos = t < ncell * NV ? m.bpos[(size_t)c0 * NV + t] : 0;
   r.geo = t < ncell * 3 * D ? m.geom[(size_t)c0 * 3 * D + t] : 0.0;
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
@@ -765,7 +765,7 @@ __device__ __forceinline__ void cells_load(const DMesh& m, const double* __restr
 
 template <int D>
 __device__ __forceinline__ void cells_store(const CellRegs<D>& r, int ncell, double* geo, double* vc, Prm* prm,
-                                            int* inst) {
+                                            int* bpos) {
   constexpr int NV = 1 << D;
   const int t = threadIdx.x;
   if (t < ncell * 3 * D) geo[t] = r.geo;
@@ -773,7 +773,7 @@ __device__ __forceinline__ void cells_store(const CellRegs<D>& r, int ncell, dou
   for (int q = 0; q < 2; ++q)
     if (t + q * kSettleBlock < ncell * NV * D) vc[t + q * kSettleBlock] = r.vc[q];
   if (t < ncell * kPrmWords) reinterpret_cast<double*>(prm)[t] = r.prm;
-  if (t < ncell) inst[t] = r.inst;
+  if (t < ncell * NV) bpos[t] = r.bpos;
 }
 
 // Part index lin of a part code: pi_k = (lin / stride_k) % Q_k with x
@@ -852,7 +852,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
   auto cbuf_geo = [&](int b) { return reinterpret_cast<double*>(smem_raw + (b ? cb1 : cb0) + L.off_geo); };
   auto cbuf_vc = [&](int b) { return reinterpret_cast<double*>(smem_raw + (b ? cb1 : cb0) + L.off_vc); };
   auto cbuf_prm = [&](int b) { return reinterpret_cast<Prm*>(smem_raw + (b ? cb1 : cb0) + L.off_prm); };
-  auto cbuf_inst = [&](int b) { return reinterpret_cast<int*>(smem_raw + (b ? cb1 : cb0) + L.off_inst); };
+  auto cbuf_bpos = [&](int b) { return reinterpret_cast<int*>(smem_raw + (b ? cb1 : cb0) + L.off_bpos); };
 
   // prologue: the first chunk's header, cell data and particles
   if (threadIdx.x == 0) {
@@ -872,7 +872,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       const int c0 = ch << cshift;
       CellRegs<D> cr;
       cells_load<D>(m, Vc, do_refill, c0, min(nc, c0 + cpw) - c0, cr);
-      cells_store<D>(cr, min(nc, c0 + cpw) - c0, cbuf_geo(0), cbuf_vc(0), cbuf_prm(0), cbuf_inst(0));
+      cells_store<D>(cr, min(nc, c0 + cpw) - c0, cbuf_geo(0), cbuf_vc(0), cbuf_prm(0), cbuf_bpos(0));
       const int nb = min(s_misc[1], capc);
       load_parts(nb + s_misc[3] - s_misc[2], nb, (long)ch * capc, spill2_base + s_misc[2]);
     }
@@ -1305,7 +1305,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         TW = TW + PS[j][D];
       }
       if (part == 0 && lc < ncell) {
-        double* co = contrib + ((size_t)(c0 + lc) * NV + iv) * (D + 1);
+        double* co = contrib + (size_t)cbuf_bpos(buf)[lc * NV + iv] * (D + 1);
 #pragma unroll
         for (int k = 0; k < D; ++k) co[k] = T[k];
         co[D] = TW;
@@ -1346,7 +1346,8 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     // the counts of the next chunk start at zero (read last by the refills)
     for (int i = threadIdx.x; i < cpw * pmax; i += blockDim.x) s_cnt[i] = 0;
     for (int i = threadIdx.x; i < cpw; i += blockDim.x) s_ccnt[i] = 0;
-    if (nch < nchunks) cells_store<D>(cr, nncell, cbuf_geo(buf ^ 1), cbuf_vc(buf ^ 1), cbuf_prm(buf ^ 1), cbuf_inst(buf ^ 1));
+    if (nch < nchunks)
+      cells_store<D>(cr, nncell, cbuf_geo(buf ^ 1), cbuf_vc(buf ^ 1), cbuf_prm(buf ^ 1), cbuf_bpos(buf ^ 1));
     if (threadIdx.x == 0) {
       dirty[ch] = staged ? 0 : 1;
       s_misc[0] = s_misc[4];
@@ -1497,7 +1498,7 @@ __device__ __forceinline__ void contrib_one(const DMesh& m, int c, int i, const 
       TW = TW + W;
     }
   }
-  double* e_out = contrib + ((size_t)c * NV + i) * (D + 1);
+  double* e_out = contrib + (size_t)m.bpos[(size_t)c * NV + i] * (D + 1);
 #pragma unroll
   for (int k = 0; k < D; ++k) e_out[k] = T[k];
   e_out[D] = TW;
@@ -1539,18 +1540,20 @@ __global__ __launch_bounds__(kBlock) void k_contrib_dirty(DMesh m, int nc, int c
 }
 
 // V_k[j] = (sum over incident cells, ascending id, of S_k) / (same for W);
-// 0/0 gives NaN as in src/pfem2particlehandler.cpp:207-209.
+// 0/0 gives NaN as in src/pfem2particlehandler.cpp:207-209. The sums of
+// DoF j's (cell, vertex) pairs are the contiguous run d2c_off[j] ..
+// d2c_off[j+1] of contrib (stored in d2c order).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_node(int ndofs, const int* __restrict__ d2c_off,
-                                                const int* __restrict__ d2c, const double* __restrict__ contrib,
-                                                double* o0, double* o1, double* o2) {
+                                                const double* __restrict__ contrib, double* o0, double* o1,
+                                                double* o2) {
   double* out[3] = {o0, o1, o2};
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ndofs; j += gridDim.x * blockDim.x) {
     double S[D], W = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) S[k] = 0.0;
     for (int e = d2c_off[j]; e < d2c_off[j + 1]; ++e) {
-      const double* c = contrib + (size_t)d2c[e] * (D + 1);
+      const double* c = contrib + (size_t)e * (D + 1);
 #pragma unroll
       for (int k = 0; k < D; ++k) S[k] = S[k] + c[k];
       W = W + c[D];

@@ -391,8 +391,15 @@ def fem_work_per_cell(dim):
     places = 4 * nv * nv + 4 * nv
     p_bytes = 8 * nv + places + 8 * dim + 8 * dim + 8 * nv * nv + 8 * nv
     v_bytes = 8 * nv + places + 8 * dim + 8 * dim + 8 + 8 * nv * nv + 8 * nv * dim
+    # velocity prediction: two vector fields in, dim matrices and right-hand
+    # sides out; per (row, point, column) the mass terms (3 + 3 dim), the
+    # viscous stencils (dim (2 dim + 2)) and the cross terms (2D 2 x 7, 3D
+    # 3 x 14)
+    pr_bytes = 8 * nv + places * dim + 16 * dim + 8 * nv * nv * dim + 8 * nv * dim
+    pr_ops = 3 + 3 * dim + dim * (2 * dim + 2) + (14 if dim == 2 else 42)
     return {"k_fem_pressure": (p_bytes, geo + nv * nv * nv * (4 * dim + 5)),
-            "k_fem_vcorr": (v_bytes, geo + nv * nv * nv * (3 + 4 * dim))}
+            "k_fem_vcorr": (v_bytes, geo + nv * nv * nv * (3 + 4 * dim)),
+            "k_fem_predict": (pr_bytes, geo + nv * nv * nv * pr_ops)}
 
 
 def fem_cost(reps=5, cpu=True):
@@ -424,10 +431,12 @@ def fem_cost(reps=5, cpu=True):
         V = torch.tensor(rng.standard_normal((dim, nd)), device="cuda")
         P = torch.tensor(rng.standard_normal(nd), device="cuda")
         faces = torch.tensor(box_faces(n), device="cuda")
+        Vo = torch.tensor(rng.standard_normal((dim, nd)), device="cuda")
         h.assemble_pressure_equation(V, 1.0, 0.01, boundary_faces=faces)
         h.assemble_velocity_correction(V, P, 1.0, 0.01)
+        h.assemble_velocity_prediction(Vo, V, 1.0, 0.01, 0.01, boundary_faces=faces)
         per = {}
-        walls = {"pressure": [], "velocity_correction": []}
+        walls = {"pressure": [], "velocity_correction": [], "velocity_prediction": []}
         for _ in range(reps):
             h.set_timing(True)
             t0 = time.perf_counter()
@@ -436,6 +445,9 @@ def fem_cost(reps=5, cpu=True):
             t0 = time.perf_counter()
             h.assemble_velocity_correction(V, P, 1.0, 0.01)
             walls["velocity_correction"].append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            h.assemble_velocity_prediction(Vo, V, 1.0, 0.01, 0.01, boundary_faces=faces)
+            walls["velocity_prediction"].append(time.perf_counter() - t0)
             for k, (cnt, ms) in h.timings().items():
                 if k.startswith("k_fem"):
                     per.setdefault(k, []).append(ms / cnt)
@@ -448,9 +460,10 @@ def fem_cost(reps=5, cpu=True):
             e = {"ms": round(ms, 4)}
             if k in work:
                 b, f = work[k][0] * nc, work[k][1] * nc
-            elif k in ("k_fem_gather_matrix", "k_fem_gather_mass"):
-                b, f = 8.0 * nc * nv * nv + 8.0 * nnz + 8.0 * (nnz + 1), float(nc * nv * nv)
-            elif k in ("k_fem_gather_rhs", "k_fem_gather_rhs_v"):
+            elif k in ("k_fem_gather_matrix", "k_fem_gather_mass", "k_fem_gather_pred"):
+                nm = dim if k == "k_fem_gather_pred" else 1
+                b, f = nm * (8.0 * nc * nv * nv + 8.0 * nnz) + 8.0 * (nnz + 1), float(nm * nc * nv * nv)
+            elif k in ("k_fem_gather_rhs", "k_fem_gather_rhs_v", "k_fem_gather_rhs_p"):
                 ncomp = 1 if k == "k_fem_gather_rhs" else dim
                 b, f = 8.0 * ncomp * nc * nv + 8.0 * ncomp * nd + 4.0 * (nd + 1), float(ncomp * nc * nv)
             else:
@@ -463,7 +476,8 @@ def fem_cost(reps=5, cpu=True):
             kern[k] = e
         res = {"mesh": "x".join(str(a) for a in n), "cells": nc, "dofs": nd, "nnz": nnz, "kernels": kern,
                "pressure_call_ms": round(float(np.median(walls["pressure"])) * 1e3, 3),
-               "velocity_correction_call_ms": round(float(np.median(walls["velocity_correction"])) * 1e3, 3)}
+               "velocity_correction_call_ms": round(float(np.median(walls["velocity_correction"])) * 1e3, 3),
+               "velocity_prediction_call_ms": round(float(np.median(walls["velocity_prediction"])) * 1e3, 3)}
         if cpu:
             from oracle import FemOracle
             cs = 1 if dim == 2 else 2
@@ -477,13 +491,20 @@ def fem_cost(reps=5, cpu=True):
             fo.velocity_correction(Vc, Vc[0], 1.0, 0.01)
             tv = time.perf_counter() - t0
             ccells = cm["cell_vertices"].shape[0]
+            t0 = time.perf_counter()
+            fo.prediction(Vc, Vc, 1.0, 0.01, 0.01, box_faces(cm["n"]))
+            tq = time.perf_counter() - t0
             res["cpu_baseline"] = {"pressure_cells_per_s": ccells / tp, "velocity_correction_cells_per_s": ccells / tv,
+                                   "velocity_prediction_cells_per_s": ccells / tq,
                                    "cores": 1, "kind": "port",
                                    "sample": "oracle assembly (matrix + right-hand side) of the %s mesh, one thread"
                                              % cm_name(cm)}
         gp = sum(kern[k]["ms"] for k in ("k_fem_pressure", "k_fem_faces", "k_fem_gather_matrix", "k_fem_gather_rhs")
                  if k in kern)
         gv = sum(kern[k]["ms"] for k in ("k_fem_vcorr", "k_fem_gather_mass", "k_fem_gather_rhs_v") if k in kern)
+        gq = sum(kern[k]["ms"] for k in ("k_fem_predict", "k_fem_predict_faces", "k_fem_gather_pred",
+                                         "k_fem_gather_rhs_p") if k in kern)
+        res["velocity_prediction_gpu_cells_per_s"] = nc / (gq * 1e-3)
         res["pressure_gpu_cells_per_s"] = nc / (gp * 1e-3)
         res["velocity_correction_gpu_cells_per_s"] = nc / (gv * 1e-3)
         out[name] = res

@@ -306,6 +306,23 @@ int pfem2gpu_assemble_pressure_equation(pfem2gpu_handle* h, const double* const*
                                         const uint8_t* boundary_faces, double rho, double time_step, double* matrix,
                                         double* rhs, int where);
 
+/* assemble_velocity_prediction() without SCHEMEB (src/pfem2fem.cpp:246-390),
+ * one system per velocity component k:
+ *   A^k_ij = sum of rho N_i N_j JxW + mu time_step JxW (viscous stencil of
+ *            component k: 4/3 on its own derivative, 1 on the others)
+ *   b^k_i  = sum of rho N_i N_j JxW V_old_k(j) - mu time_step JxW (explicit
+ *            cross terms of the viscous stress) V(j)
+ * plus the boundary terms on faces flagged in boundary_faces ([n_cells], bit
+ * f for face f at a boundary without a velocity Dirichlet condition; NULL for
+ * none): in 2D matrix and right-hand side terms of the x-normal traction, in
+ * 3D right-hand side traction terms computed, as the reference computes them,
+ * with the velocity's first component taken as zero. V_old, V: dim arrays
+ * [n_dofs]. matrix: dim arrays [nnz] (NULL to skip the matrices, which depend
+ * on the mesh and time_step only). rhs: dim arrays [n_dofs]. */
+int pfem2gpu_assemble_velocity_prediction(pfem2gpu_handle* h, const double* const* V_old, const double* const* V,
+                                          const uint8_t* boundary_faces, double rho, double mu, double time_step,
+                                          double* const* matrix, double* const* rhs, int where);
+
 /* assemble_velocity_correction() without SCHEMEB (src/pfem2fem.cpp:460-517):
  *   M_ij  = sum of rho N_i N_j JxW (the matrix of every component)
  *   b_i^k = sum of rho N_i N_j JxW V_pred_k(j) - time_step N_i JxW d_k N_j P(j).

@@ -8,6 +8,7 @@
 //   src/pfem2fem.h:24              QUAD_POINTS_PER_DIRECTION = 2 (QGauss, cells and faces)
 //   src/pfem2fem.cpp:18-22         FEValues flags (values, gradients, JxW; faces also normals)
 //   src/pfem2fem.cpp:78-95         DoFTools::make_sparsity_pattern (DoFs sharing a cell)
+//   src/pfem2fem.cpp:246-390       assemble_velocity_prediction (SCHEMEB off)
 //   src/pfem2fem.cpp:392-458       assemble_pressure_equation (SCHEMEB off)
 //   src/pfem2fem.cpp:460-517       assemble_velocity_correction (SCHEMEB off)
 // deal.II functions restated from their mathematics (not in the reference tree):
@@ -256,6 +257,115 @@ int orc_fem_pressure(int dim, int64_t nverts, const double* X, int64_t nc, const
           vals[p] = vals[p] + A[i][j];
         }
       rhs[dof[i]] = rhs[dof[i]] + b[i];
+    }
+  }
+  return 0;
+}
+
+// assemble_velocity_prediction (src/pfem2fem.cpp:246-390): vals [dim][nnz]
+// (may be null), rhs [dim][ndofs]; Vo (old velocity) and U (current) are
+// [dim][ndofs]; faces [nc] bit f = face f without a velocity Dirichlet
+// condition (may be null).
+int orc_fem_prediction(int dim, int64_t nverts, const double* X, int64_t nc, const int* cv, const int* cd,
+                       int64_t ndofs, const double* Vo, const double* U, const unsigned char* faces, double rho,
+                       double mu, double time_step, const int64_t* rowptr, const int32_t* cols, double* vals,
+                       double* rhs) {
+  const int nv = 1 << dim, nq = 1 << dim;
+  const int64_t nnz = rowptr[ndofs];
+  Sys S{rowptr, cols, vals};
+  if (vals) std::fill(vals, vals + dim * nnz, 0.0);
+  std::fill(rhs, rhs + dim * ndofs, 0.0);
+  std::vector<Geo> g, gf;
+  for (long c = 0; c < nc; ++c) {
+    const Cell cell = load_cell(dim, nverts, X, cv, c);
+    reinit(cell, g);
+    double A[3][8][8] = {}, b[3][8] = {};
+    const int* dof = cd + c * nv;
+    for (int q = 0; q < nq; ++q) {
+      const double weight = g[q].JxW;
+      for (int i = 0; i < nv; ++i) {
+        const double Ni = g[q].N[i];
+        const double* gi = g[q].grad[i];
+        for (int j = 0; j < nv; ++j) {
+          const int jd = dof[j];
+          const double Nj = g[q].N[j];
+          const double* gj = g[q].grad[j];
+          double aux = rho * Ni * Nj * weight;
+          for (int k = 0; k < dim; ++k) {
+            A[k][i][j] += aux;
+            b[k][i] += aux * Vo[k * ndofs + jd];
+          }
+          aux = mu * time_step * weight;
+          if (dim == 2) {
+            A[0][i][j] += aux * (gi[1] * gj[1] + 4.0 / 3.0 * gi[0] * gj[0]);
+            A[1][i][j] += aux * (gi[0] * gj[0] + 4.0 / 3.0 * gi[1] * gj[1]);
+            b[0][i] -= aux * (gi[1] * gj[0] - 2.0 / 3.0 * gi[0] * gj[1]) * U[1 * ndofs + jd];
+            b[1][i] -= aux * (gi[0] * gj[1] - 2.0 / 3.0 * gi[1] * gj[0]) * U[0 * ndofs + jd];
+          } else {
+            A[0][i][j] += aux * (4.0 / 3.0 * gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2]);
+            A[1][i][j] += aux * (gj[0] * gi[0] + 4.0 / 3.0 * gi[1] * gj[1] + gi[2] * gj[2]);
+            A[2][i][j] += aux * (gj[0] * gi[0] + gj[1] * gi[1] + 4.0 / 3.0 * gi[2] * gj[2]);
+            b[0][i] -= aux * ((gi[1] * gj[0] - 2.0 / 3.0 * gi[0] * gj[1]) * U[1 * ndofs + jd] +
+                              (gi[2] * gj[0] - 2.0 / 3.0 * gi[0] * gj[2]) * U[2 * ndofs + jd]);
+            b[1][i] -= aux * ((gi[0] * gj[1] - 2.0 / 3.0 * gi[1] * gj[0]) * U[0 * ndofs + jd] +
+                              (gi[2] * gj[1] - 2.0 / 3.0 * gi[1] * gj[2]) * U[2 * ndofs + jd]);
+            b[2][i] -= aux * ((gi[0] * gj[2] - 2.0 / 3.0 * gi[2] * gj[0]) * U[0 * ndofs + jd] +
+                              (gi[1] * gj[2] - 2.0 / 3.0 * gi[2] * gj[1]) * U[1 * ndofs + jd]);
+          }
+        }
+      }
+    }
+    const int fm = faces ? faces[c] : 0;
+    for (int f = 0; f < 2 * dim; ++f) {
+      if (!((fm >> f) & 1)) continue;
+      reinit_face(cell, f, gf);
+      for (size_t q = 0; q < gf.size(); ++q) {
+        const Geo& G = gf[q];
+        if (dim == 2) {
+          for (int i = 0; i < nv; ++i)
+            for (int j = 0; j < nv; ++j) {
+              A[0][i][j] -= mu * time_step * G.N[i] * 4.0 / 3.0 * G.grad[j][0] * G.normal[0] * G.JxW;
+              b[0][i] += mu * time_step * G.N[i] * (-2.0 / 3.0) * G.grad[j][1] * U[1 * ndofs + dof[j]] *
+                         G.normal[0] * G.JxW;
+              A[1][i][j] -= mu * time_step * G.N[i] * G.grad[j][0] * G.normal[0] * G.JxW;
+              b[1][i] += mu * time_step * G.N[i] * G.grad[j][1] * U[0 * ndofs + dof[j]] * G.normal[0] * G.JxW;
+            }
+        } else {
+          double tX[3] = {0, 0, 0}, tY[3] = {0, 0, 0}, tZ[3] = {0, 0, 0};
+          for (int i = 0; i < nv; ++i) {
+            const double* gi = G.grad[i];
+            double vel[3] = {0.0, 0.0, 0.0};  // the reference fills components 1..dim-1 only
+            for (int j = 1; j < dim; ++j) vel[j] = U[j * ndofs + dof[i]];
+            tX[0] += (4.0 / 3.0) * gi[0] * vel[0] - (2.0 / 3.0) * gi[1] * vel[1] - (2.0 / 3.0) * gi[2] * vel[2];
+            tX[1] += gi[1] * vel[0] + gi[0] * vel[1];
+            tX[2] += gi[2] * vel[0] + gi[0] * vel[2];
+            tY[0] += gi[1] * vel[0] + gi[0] * vel[1];
+            tY[1] += (-2.0 / 3.0) * gi[0] * vel[0] + (4.0 / 3.0) * gi[1] * vel[1] - (2.0 / 3.0) * gi[2] * vel[2];
+            tY[2] += gi[2] * vel[1] + gi[1] * vel[2];
+            tZ[0] += gi[2] * vel[0] + gi[0] * vel[2];
+            tZ[1] += gi[2] * vel[1] + gi[1] * vel[2];
+            tZ[2] += (-2.0 / 3.0) * gi[0] * vel[0] - (2.0 / 3.0) * gi[1] * vel[1] + (4.0 / 3.0) * gi[2] * vel[2];
+          }
+          const double* t[3] = {tX, tY, tZ};
+          for (int i = 0; i < nv; ++i) {
+            const double aux = mu * time_step * G.N[i] * G.JxW;
+            for (int k = 0; k < 3; ++k) {
+              double d = (aux * t[k][0]) * G.normal[0];
+              for (int l = 1; l < 3; ++l) d += (aux * t[k][l]) * G.normal[l];
+              b[k][i] += d;
+            }
+          }
+        }
+      }
+    }
+    for (int i = 0; i < nv; ++i) {
+      if (vals)
+        for (int j = 0; j < nv; ++j) {
+          const long p = S.find(dof[i], dof[j]);
+          if (p < 0) return -1;
+          for (int k = 0; k < dim; ++k) vals[k * nnz + p] += A[k][i][j];
+        }
+      for (int k = 0; k < dim; ++k) rhs[k * ndofs + dof[i]] += b[k][i];
     }
   }
   return 0;

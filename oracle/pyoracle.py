@@ -71,6 +71,10 @@ def _load():
         lib.orc_fem_pressure.restype = ctypes.c_int
         lib.orc_fem_pressure.argtypes = [ctypes.c_int, ctypes.c_int64, _d, ctypes.c_int64, _i, _i, ctypes.c_int64, _d,
                                          _u8, ctypes.c_double, ctypes.c_double, _l, _i, _d, _d]
+        lib.orc_fem_prediction.restype = ctypes.c_int
+        lib.orc_fem_prediction.argtypes = [ctypes.c_int, ctypes.c_int64, _d, ctypes.c_int64, _i, _i, ctypes.c_int64,
+                                           _d, _d, _u8, ctypes.c_double, ctypes.c_double, ctypes.c_double, _l, _i,
+                                           _d, _d]
         lib.orc_fem_vcorr.restype = ctypes.c_int
         lib.orc_fem_vcorr.argtypes = [ctypes.c_int, ctypes.c_int64, _d, ctypes.c_int64, _i, _i, ctypes.c_int64, _d,
                                       _d, ctypes.c_double, ctypes.c_double, _l, _i, _d, _d]
@@ -245,6 +249,21 @@ class FemOracle:
         rc = self.lib.orc_fem_pressure(self.dim, self.X.shape[1], _dp(self.X), self.nc, _ip(self.cv), _ip(self.cd),
                                        self.nd, _dp(V), f, float(rho), float(dt), self.row_ptr.ctypes.data_as(_l),
                                        _ip(self.cols), _dp(vals), _dp(rhs))
+        assert rc == 0
+        return vals, rhs
+
+    def prediction(self, Vold, V, rho, mu, dt, faces=None):
+        Vold, V = _f64(Vold), _f64(V)
+        vals = np.empty((self.dim, self.cols.shape[0]))
+        rhs = np.empty((self.dim, self.nd))
+        f = None
+        if faces is not None:
+            faces = np.ascontiguousarray(faces, dtype=np.uint8)
+            f = faces.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        rc = self.lib.orc_fem_prediction(self.dim, self.X.shape[1], _dp(self.X), self.nc, _ip(self.cv),
+                                         _ip(self.cd), self.nd, _dp(Vold), _dp(V), f, float(rho), float(mu),
+                                         float(dt), self.row_ptr.ctypes.data_as(_l), _ip(self.cols), _dp(vals),
+                                         _dp(rhs))
         assert rc == 0
         return vals, rhs
 

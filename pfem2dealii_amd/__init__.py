@@ -64,6 +64,7 @@ EXPORTS = [
     "pfem2gpu_vertex_to_cells", "pfem2gpu_particle_record_size", "pfem2gpu_pack_particles",
     "pfem2gpu_unpack_particles", "pfem2gpu_pack_particles_noalloc", "pfem2gpu_remove_particles",
     "pfem2gpu_fem_sparsity", "pfem2gpu_assemble_pressure_equation", "pfem2gpu_assemble_velocity_correction",
+    "pfem2gpu_assemble_velocity_prediction",
 ]
 
 
@@ -126,6 +127,8 @@ def load_library(path=None):
                                                         ctypes.c_int]
     lib.pfem2gpu_assemble_velocity_correction.argtypes = [_vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _vp,
                                                           _vp, ctypes.c_int]
+    lib.pfem2gpu_assemble_velocity_prediction.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                                          ctypes.c_double, _vp, _vp, ctypes.c_int]
     if path is None:
         _lib = lib
     return lib
@@ -379,6 +382,42 @@ class ParticleHandler:
         _check(self.lib, self._fem_call(dev, self.lib.pfem2gpu_assemble_pressure_equation, self.h, f.arr,
                                         self._ptr(faces), float(rho), float(time_step), self._ptr(vals),
                                         self._ptr(rhs), f.where), "assemble_pressure_equation")
+        return vals, rhs
+
+    def _faces_arg(self, boundary_faces, dev):
+        if boundary_faces is None:
+            return None
+        if dev is not None:
+            import torch
+            f = torch.as_tensor(np.asarray(boundary_faces.cpu() if _is_device(boundary_faces) else boundary_faces,
+                                           dtype=np.uint8)).to(dev)
+        else:
+            f = np.ascontiguousarray(boundary_faces, dtype=np.uint8)
+        if tuple(f.shape) != (self.n_cells,):
+            raise ValueError("boundary_faces must have one entry per cell")
+        return f
+
+    def assemble_velocity_prediction(self, V_old, V, rho, mu, time_step, boundary_faces=None, matrix=True):
+        """assemble_velocity_prediction (src/pfem2fem.cpp:246-390): returns
+        (values [dim, nnz] over fem_sparsity() or None, rhs [dim, n_dofs]),
+        one system per component, on the device (torch) when V_old is a
+        device tensor. boundary_faces: [n_cells] uint8, bit f for face f at a
+        boundary without a velocity Dirichlet condition."""
+        if getattr(self, "_fem_nnz", None) is None:
+            self.fem_sparsity()
+        a = _FieldPtrs(V_old, self.dim, self.n_dofs)
+        b = _FieldPtrs(V, self.dim, self.n_dofs)
+        if a.where != b.where:
+            raise ValueError("V_old and V must both be host or both be device arrays")
+        dev = V_old.device if a.where == DEVICE else None
+        faces = self._faces_arg(boundary_faces, dev)
+        vals = self._fem_out(dev, (self.dim, self._fem_nnz)) if matrix else None
+        rhs = self._fem_out(dev, (self.dim, self.n_dofs))
+        mp = _FieldPtrs(vals, self.dim, self._fem_nnz, writable=True) if matrix else None
+        r = _FieldPtrs(rhs, self.dim, self.n_dofs, writable=True)
+        _check(self.lib, self._fem_call(dev, self.lib.pfem2gpu_assemble_velocity_prediction, self.h, a.arr, b.arr,
+                                        self._ptr(faces), float(rho), float(mu), float(time_step),
+                                        mp.arr if mp else None, r.arr, a.where), "assemble_velocity_prediction")
         return vals, rhs
 
     def assemble_velocity_correction(self, V_pred, P, rho, time_step, matrix=True):

@@ -138,9 +138,11 @@ struct pfem2gpu_handle {
   std::vector<int32_t> fem_col;     // [nnz] ascending columns per row (host)
   long* fem_soff = nullptr;         // [nnz+1] contributions of each entry (runs in fem_A)
   int* fem_apos = nullptr;          // [nc*nvc*nvc] place of cell-matrix entry (c, i, j) in fem_A
-  double* fem_A = nullptr;          // cell-matrix entries in contribution order (entry, then cell)
+  double* fem_A = nullptr;          // cell-matrix entries in contribution order (entry, then cell); planes
+  int fem_A_planes = 0;             // matrices fem_A holds (1, or dim for the velocity prediction)
   double* fem_b = nullptr;          // [dim][nc*nvc] cell right-hand sides in DoF -> (cell, vertex) order
-  double* fem_vals = nullptr;       // [nnz] staging of a host matrix output
+  double* fem_vals = nullptr;       // [fem_vals_planes][nnz] staging of host matrix outputs
+  int fem_vals_planes = 0;
   double* fem_rhs[3] = {nullptr, nullptr, nullptr};  // [ndofs] staging of host right-hand sides
   double* fem_P = nullptr;          // [ndofs] staging of a host pressure input
   unsigned char* fem_faces = nullptr;  // [nc] staging of host face flags
@@ -2209,6 +2211,7 @@ int fem_prepare(pfem2gpu_handle* h) {
     return rc;
   HIPCHK(hipMemcpy(h->fem_soff, soff.data(), soff.size() * sizeof(long), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->fem_apos, apos.data(), apos.size() * 4, hipMemcpyHostToDevice));
+  h->fem_A_planes = 1;
   h->fem_rowptr.swap(rp);
   h->fem_col.swap(col);
   h->fem_ready = true;
@@ -2222,9 +2225,14 @@ int fem_grid(long nc, int nvc) {
 }
 
 // Host-side staging of the optional host outputs and inputs.
-int fem_stage(pfem2gpu_handle* h, bool vals, int nrhs, bool P, bool faces) {
+int fem_stage(pfem2gpu_handle* h, int vals, int nrhs, bool P, bool faces) {
   int rc;
-  if (vals && !h->fem_vals && (rc = dalloc(&h->fem_vals, h->fem_col.size()))) return rc;
+  if (vals > h->fem_vals_planes) {
+    dfree(h->fem_vals);
+    h->fem_vals_planes = 0;
+    if ((rc = dalloc(&h->fem_vals, (size_t)vals * h->fem_col.size()))) return rc;
+    h->fem_vals_planes = vals;
+  }
   for (int k = 0; k < nrhs; ++k)
     if (!h->fem_rhs[k] && (rc = dalloc(&h->fem_rhs[k], h->ndofs))) return rc;
   if (P && !h->fem_P && (rc = dalloc(&h->fem_P, h->ndofs))) return rc;
@@ -2238,7 +2246,7 @@ int fem_assemble(pfem2gpu_handle* h, bool pressure, const double* const* Vp, con
   const bool host = where == PFEM2GPU_HOST;
   const int nrhs = pressure ? 1 : D;
   CHK(fem_prepare(h));
-  CHK(fem_stage(h, host && matrix, host ? nrhs : 0, host && P, host && faces));
+  CHK(fem_stage(h, (host && matrix) ? 1 : 0, host ? nrhs : 0, host && P, host && faces));
   DField Vf;
   CHK(field_in(h, Vp, where, 0, &Vf));
   const double* Pd = P;
@@ -2279,6 +2287,58 @@ int fem_assemble(pfem2gpu_handle* h, bool pressure, const double* const* Vp, con
   return PFEM2GPU_OK;
 }
 
+// assemble_velocity_prediction: one matrix and one right-hand side per
+// component.
+template <int D>
+int fem_predict(pfem2gpu_handle* h, const double* const* Vo, const double* const* U, const unsigned char* faces,
+                double rho, double mu, double dt, double* const* matrix, double* const* rhs, int where) {
+  const bool host = where == PFEM2GPU_HOST;
+  CHK(fem_prepare(h));
+  const long nnz = (long)h->fem_col.size();
+  const long aplane = (long)h->nc * h->nvc * h->nvc;
+  if (h->fem_A_planes < D) {
+    dfree(h->fem_A);
+    h->fem_A_planes = 0;
+    int rc;
+    if ((rc = dalloc(&h->fem_A, (size_t)D * aplane))) return rc;
+    h->fem_A_planes = D;
+  }
+  CHK(fem_stage(h, (host && matrix) ? D : 0, host ? D : 0, false, host && faces));
+  DField Vf, Uf;
+  CHK(field_in(h, Vo, where, 0, &Vf));
+  CHK(field_in(h, U, where, 1, &Uf));
+  const unsigned char* fd = faces;
+  if (host && faces) {
+    HIPCHK(hipMemcpyAsync(h->fem_faces, faces, h->nc, hipMemcpyHostToDevice, h->stream));
+    fd = h->fem_faces;
+  }
+  LAUNCH_B(h, "k_fem_predict", k_fem_predict<D>, fem_grid(h->nc, h->nvc), kFemBlock, 0, dmesh(h), h->nc, Vf, Uf, rho,
+           mu, dt, h->fem_apos, h->bpos, h->fem_A, h->fem_b);
+  if (fd)
+    LAUNCH(h, "k_fem_predict_faces", k_fem_predict_faces<D>, grid_for(h->nc), 0, dmesh(h), h->nc, Uf, mu, dt, fd,
+           h->fem_apos, h->bpos, h->fem_A, h->fem_b);
+  double* md[3] = {nullptr, nullptr, nullptr};
+  double* rd[3] = {nullptr, nullptr, nullptr};
+  for (int k = 0; k < D; ++k) {
+    if (matrix) md[k] = host ? h->fem_vals + (size_t)k * nnz : matrix[k];
+    rd[k] = host ? h->fem_rhs[k] : rhs[k];
+  }
+  if (matrix)
+    LAUNCH(h, "k_fem_gather_pred", k_fem_segsum<long>, grid_for(nnz), 0, nnz, h->fem_soff, D, aplane, h->fem_A,
+           md[0], md[1], md[2]);
+  LAUNCH(h, "k_fem_gather_rhs_p", k_fem_segsum<int>, grid_for(h->ndofs), 0, (long)h->ndofs, h->d2c_off, D,
+         (long)h->nc * h->nvc, h->fem_b, rd[0], rd[1], rd[2]);
+  if (host) {
+    for (int k = 0; k < D; ++k) {
+      if (matrix)
+        HIPCHK(hipMemcpyAsync(matrix[k], md[k], nnz * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(hipMemcpyAsync(rhs[k], rd[k], h->ndofs * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  return PFEM2GPU_OK;
+}
+
 int fem_check(pfem2gpu_handle* h, const double* const* V, const void* rhs, int where) {
   if (!h) return fail(PFEM2GPU_ERR_ARG, "null handle");
   if (!V || !rhs) return fail(PFEM2GPU_ERR_ARG, "null field or right-hand side");
@@ -2307,6 +2367,17 @@ int pfem2gpu_assemble_pressure_equation(pfem2gpu_handle* h, const double* const*
   double* r[1] = {rhs};
   return h->dim == 2 ? fem_assemble<2>(h, true, V_pred, nullptr, boundary_faces, coeff, 0.0, matrix, r, where)
                      : fem_assemble<3>(h, true, V_pred, nullptr, boundary_faces, coeff, 0.0, matrix, r, where);
+}
+
+int pfem2gpu_assemble_velocity_prediction(pfem2gpu_handle* h, const double* const* V_old, const double* const* V,
+                                          const uint8_t* boundary_faces, double rho, double mu, double time_step,
+                                          double* const* matrix, double* const* rhs, int where) {
+  CHK(fem_check(h, V_old, rhs, where));
+  if (!V) return fail(PFEM2GPU_ERR_ARG, "null velocity");
+  for (int k = 0; k < h->dim; ++k)
+    if (!rhs[k] || (matrix && !matrix[k])) return fail(PFEM2GPU_ERR_ARG, "null output component");
+  return h->dim == 2 ? fem_predict<2>(h, V_old, V, boundary_faces, rho, mu, time_step, matrix, rhs, where)
+                     : fem_predict<3>(h, V_old, V, boundary_faces, rho, mu, time_step, matrix, rhs, where);
 }
 
 int pfem2gpu_assemble_velocity_correction(pfem2gpu_handle* h, const double* const* V_pred, const double* P,

@@ -1,5 +1,6 @@
-"""GPU FEM assembly (pfem2gpu_assemble_pressure_equation and
-pfem2gpu_assemble_velocity_correction) against the oracle (oracle/
+"""GPU FEM assembly (pfem2gpu_assemble_pressure_equation,
+pfem2gpu_assemble_velocity_correction and
+pfem2gpu_assemble_velocity_prediction) against the oracle (oracle/
 fem_oracle.cpp, itself pinned by closed forms in tests/test_fem_oracle.py):
 matrix values, right-hand sides and the sparsity pattern bit for bit on box,
 renumbered, perturbed and 3D meshes with random fields and random boundary
@@ -20,7 +21,7 @@ from tests.test_fem_oracle import _all_faces, _renumbered
 
 pytestmark = pytest.mark.gpu
 
-RHO, DT = 1.3, 0.02
+RHO, DT, MU = 1.3, 0.02, 0.7
 
 
 def _say(msg):
@@ -92,6 +93,16 @@ def test_fem_assembly_matches_oracle(gpu_available, kind):
     assert np.array_equal(Md.cpu().numpy(), Mo) and np.array_equal(cd.cpu().numpy(), co)
     _, b0 = h.assemble_pressure_equation(V, RHO, DT)
     assert np.array_equal(b0, fo.pressure(V, RHO, DT)[1]), "no faces flagged"
+    # velocity prediction: one system per component
+    Vo = rng.standard_normal((dim, nd))
+    Ao, bo = fo.prediction(Vo, V, RHO, MU, DT, faces)
+    A, b = h.assemble_velocity_prediction(Vo, V, RHO, MU, DT, boundary_faces=faces)
+    assert np.array_equal(A, Ao), "prediction matrices"
+    assert np.array_equal(b, bo), "prediction rhs"
+    Ad, bd = h.assemble_velocity_prediction(torch.tensor(Vo, device="cuda"), Vd, RHO, MU, DT, boundary_faces=faces)
+    assert np.array_equal(Ad.cpu().numpy(), Ao) and np.array_equal(bd.cpu().numpy(), bo)
+    none, b0 = h.assemble_velocity_prediction(Vo, V, RHO, MU, DT, matrix=False)
+    assert none is None and np.array_equal(b0, fo.prediction(Vo, V, RHO, MU, DT)[1]), "no faces flagged"
     h.close()
 
 
@@ -137,4 +148,18 @@ def test_fem_assembly_full_size_properties(gpu_available, name):
     # the entries are differences of cell terms of size coeff |V| int |grad N_i|
     scale = coeff * np.abs(B @ mesh["X"]).max() * lumped.max() / min(mesh["h"])
     assert np.abs(b.cpu().numpy() - want).max() <= 1e-11 * scale, "linear field"
+    # velocity prediction: repeat calls give the same bits; with every
+    # boundary face flagged the boundary terms close the traction, so in 2D
+    # A^k x = rho M x for the coordinate field x
+    Vo = torch.tensor(np.random.default_rng(3).standard_normal((dim, nd)), device="cuda")
+    P1, c1 = h.assemble_velocity_prediction(Vo, V, RHO, MU, DT, boundary_faces=faces)
+    P2, c2 = h.assemble_velocity_prediction(Vo, V, RHO, MU, DT, boundary_faces=faces)
+    assert torch.equal(P1, P2) and torch.equal(c1, c2), "prediction repeat calls"
+    if dim == 2:
+        x = mesh["X"][0]
+        Mx = sp.csr_matrix((M.cpu().numpy(), cols, rp), shape=(nd, nd)) @ x
+        for k in range(dim):
+            Ak = sp.csr_matrix((P1[k].cpu().numpy(), cols, rp), shape=(nd, nd))
+            err = np.abs(Ak @ x - RHO * Mx).max()
+            assert err <= 1e-11 * (RHO * np.abs(Mx).max() + MU * DT * np.abs(x).max() * 8), k
     h.close()

@@ -228,30 +228,8 @@ __global__ __launch_bounds__(kBlock) void k_fem_faces(DMesh m, int nc, DField Vp
     for (int f = 0; f < 2 * D; ++f) {
       if (!((fm >> f) & 1)) continue;
       for (int q = 0; q < (1 << (D - 1)); ++q) {
-        double xi[D], w = 1.0;
-        face_point<D>(f, q, xi);
-#pragma unroll
-        for (int k = 0; k < D - 1; ++k) w = w * 0.5;
-        double N[NV], dN[NV][3];
-        shape<D>(xi, N);
-        shape_grad<D>(xi, dN);
-        const Q1Jac<D> jac = q1_jacobian<D>(Xr, dN);
-        const int kf = f >> 1;
-        const double sgn = (f & 1) ? 1.0 : -1.0;
-        double nv[D], n2 = 0.0;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-          double a = jac.inv[0][k];
-#pragma unroll
-          for (int l = 1; l < D; ++l)
-            if (l == kf) a = jac.inv[l][k];
-          nv[k] = sgn * a;
-          n2 = n2 + nv[k] * nv[k];
-        }
-        const double nn = sqrt(n2);
-#pragma unroll
-        for (int k = 0; k < D; ++k) nv[k] = nv[k] / nn;
-        const double JxW = jac.det * nn * w;
+        double N[NV], g[NV][3], nv[D];
+        const double JxW = fem_face_geometry<D>(Xr, f, q, N, g, nv);
         double Vq[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) Vq[k] = 0.0;

@@ -15,9 +15,13 @@ SRC = os.path.join(HERE, "handler_run.cpp")
 OUT = os.path.join(HERE, "handler_run.bin")
 
 
-def compile_command(out=OUT, syntax_only=False):
+def compile_command(out=OUT, syntax_only=False, sanitize=False):
+    """g++ command for the program (host code only; `sanitize` adds the
+    address and undefined-behaviour sanitizers)."""
     cxx = shutil.which("g++") or "g++"
     cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), SRC]
+    if sanitize:
+        cmd[1:2] = ["-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer"]
     if syntax_only:
         return cmd + ["-fsyntax-only"]
     return cmd + ["-L" + os.path.join(ROOT, "pfem2dealii_amd"), "-lpfem2gpu",

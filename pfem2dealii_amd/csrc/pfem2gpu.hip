@@ -123,7 +123,6 @@ struct pfem2gpu_handle {
 
   // walk queues of the split move (two, used in turn by the walk rounds)
   double* qx[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-  double* qxo[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double* qv[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   int* qcell[2] = {nullptr, nullptr};
   int* qs[2] = {nullptr, nullptr};
@@ -361,7 +360,6 @@ void free_queues(pfem2gpu_handle* h) {
   for (int b = 0; b < 2; ++b) {
     for (int k = 0; k < 3; ++k) {
       dfree(h->qx[b][k]);
-      dfree(h->qxo[b][k]);
       dfree(h->qv[b][k]);
     }
     dfree(h->qcell[b]);
@@ -380,7 +378,6 @@ int grow_queues(pfem2gpu_handle* h, long need) {
   for (int b = 0; b < 2; ++b) {
     for (int k = 0; k < h->dim; ++k) {
       CHK(dalloc(&h->qx[b][k], need));
-      CHK(dalloc(&h->qxo[b][k], need));
       CHK(dalloc(&h->qv[b][k], need));
     }
     CHK(dalloc(&h->qcell[b], need));
@@ -395,7 +392,6 @@ DWalk dwalk(pfem2gpu_handle* h, int b) {
   DWalk q;
   for (int k = 0; k < 3; ++k) {
     q.x[k] = h->qx[b][k];
-    q.xo[k] = h->qxo[b][k];
     q.v[k] = h->qv[b][k];
   }
   q.cell = h->qcell[b];

@@ -69,13 +69,14 @@ struct DTmp {
 };
 
 // Walk queue: particles that left their cell in a substep, with the state
-// the neighbour walk needs (position after the substep, reference position
-// before it, the substep index) and the carried velocity and source index.
-// The substep's velocity is not queued: the walk recomputes it bit for bit
-// from the left cell's field at the queued reference position.
+// the neighbour walk needs (the position before the failing substep, the
+// left cell, the substep index) and the carried velocity and source index.
+// The walk recomputes the failing substep bit for bit from the queued
+// position: the left cell's inverse map gives its reference position (the
+// cell word's kPlaced flag marks a first substep that started from a snapped
+// part centre), the field gives its velocity, and the Euler update its end.
 struct DWalk {
   double* x[3];
-  double* xo[3];
   double* v[3];
   int* cell;
   int* s;

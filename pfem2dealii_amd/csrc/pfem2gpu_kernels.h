@@ -234,7 +234,7 @@ constexpr int kQStripes = 1024;
 // when the move needs it (keep_src: source maps, paths or a replay).
 template <int D>
 __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int tflag, int s, int src, bool keep_src,
-                                          const double* x, const double* xo, const double* v, int capc,
+                                          const double* x, int qcell, const double* v, int capc,
                                           int* __restrict__ bcount, long spill_base, int* __restrict__ spill_n,
                                           DTmp& T, DWalk& Q, int* __restrict__ qn, long qreg) {
   const int lane = threadIdx.x & 63;
@@ -257,10 +257,9 @@ __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int tfla
 #pragma unroll
       for (int k = 0; k < D; ++k) {
         Q.x[k][q] = x[k];
-        Q.xo[k][q] = xo[k];
         Q.v[k][q] = v[k];
       }
-      Q.cell[q] = cell;
+      Q.cell[q] = qcell;
       Q.s[q] = s;
       if (keep_src) Q.src[q] = src;
     }
@@ -289,8 +288,8 @@ __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int tfla
 }
 
 // Substeps s, s+1, ... of a particle in cell cc until n_sub or until it
-// leaves the cell (returns true, x is then the position after the failing
-// substep, xi the reference position before it and u its velocity).
+// leaves the cell (returns true; x and xi are then the position and the
+// reference position before the failing substep, u its velocity).
 template <int D>
 __device__ __forceinline__ bool substeps_in_cell(const DMesh& m, const double* __restrict__ Vc, const CellCache<D>& cc,
                                                  int n_sub, double dt_sub, int& s, double* x, double* xi, double* u,
@@ -303,15 +302,19 @@ __device__ __forceinline__ bool substeps_in_cell(const DMesh& m, const double* _
       for (int k = 0; k < D; ++k) path.xi[k][o] = xi[k];
     }
     gather_cached<D>(Vc, cc, xi, u);
+    double xe[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       const double t = u[k] * dt_sub;
-      x[k] = x[k] + t;
+      xe[k] = x[k] + t;
     }
     double xn[D];
-    if (inv_cached<D>(m, cc, x, xn) && inside<D>(xn)) {
+    if (inv_cached<D>(m, cc, xe, xn) && inside<D>(xn)) {
 #pragma unroll
-      for (int k = 0; k < D; ++k) xi[k] = xn[k];
+      for (int k = 0; k < D; ++k) {
+        x[k] = xe[k];
+        xi[k] = xn[k];
+      }
     } else {
       return true;
     }
@@ -334,7 +337,7 @@ __global__ __launch_bounds__(B) void k_move_cell(int advect, DMesh m, const doub
   const bool keep_src = T.src || capc < 0 || path.c;
   for (long base = (long)blockIdx.x * blockDim.x; base < n; base += (long)gridDim.x * blockDim.x) {
     const int p = (int)(base + threadIdx.x);
-    int key = -1, s = 0, tflag = 0;
+    int key = -1, s = 0, tflag = 0, qflag = 0;
     bool walk = false;
     CellCache<D> cc;
     cc.c = -1;
@@ -370,9 +373,13 @@ __global__ __launch_bounds__(B) void k_move_cell(int advect, DMesh m, const doub
         }
         walk = substeps_in_cell<D>(m, Vc, cc, advect ? pr.n_sub : 0, pr.dt_sub, s, x, xi, u, path, p);
         if (!walk) key = c >> cshift;
+        // a leaver's queued reference position is recomputed from x by the
+        // walk: the snap of the first substep travels as the cell word's flag
+        if (walk && s == 0 && placed_of(w)) qflag = kPlaced;
       }
     }
-    move_emit<D>(key, walk, cc.c, tflag, s, p, keep_src, x, xi, v, capc, bcount, spill_base, spill_n, T, Q, qn, qreg);
+    move_emit<D>(key, walk, cc.c, tflag, s, p, keep_src, x, cc.c | qflag, v, capc, bcount, spill_base, spill_n, T, Q,
+                 qn, qreg);
   }
 }
 
@@ -405,14 +412,13 @@ __global__ __launch_bounds__(B) void k_move_walk(DMesh m, const double* __restri
       u[k] = 0.0;
     }
     if (in) {
-      const int c = Qin.cell[i];
+      const int cw = Qin.cell[i];
+      const int c = cell_of(cw);
       s = Qin.s[i];
       if (keep_src) src = Qin.src[i];
-      double xo[D];
 #pragma unroll
       for (int k = 0; k < D; ++k) {
         x[k] = Qin.x[k][i];
-        xo[k] = Qin.xo[k][i];
         v[k] = Qin.v[k][i];
       }
       int4 r0 = make_int4(-2, -1, -1, -1), r1 = r0, r2 = r0, r3 = r0;
@@ -425,8 +431,20 @@ __global__ __launch_bounds__(B) void k_move_walk(DMesh m, const double* __restri
         }
       }
       load_cell<D>(m, Vc, c, cc, true);
-      gather_cached<D>(Vc, cc, xo, u);  // the failing substep's velocity, bit for bit
       const Prm& pr = m.prm[inst_of(m, c)];
+      // the failing substep again, bit for bit: its reference position (the
+      // left cell's inverse map of the queued position, snapped to the part
+      // centre for a placed particle's first substep), its velocity and its
+      // end position
+      double xo[D];
+      inv_cached<D>(m, cc, x, xo);
+      if (placed_of(cw)) snap_part<D>(pr, xo);
+      gather_cached<D>(Vc, cc, xo, u);
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const double t = u[k] * pr.dt_sub;
+        x[k] = x[k] + t;
+      }
       double xn[D];
       int c2 = -2;
       bool have_g = false;
@@ -467,7 +485,7 @@ __global__ __launch_bounds__(B) void k_move_walk(DMesh m, const double* __restri
         if (!walk) key = c2 >> cshift;
       }
     }
-    move_emit<D>(key, walk, cc.c, 0, s, src, keep_src, x, xi, v, capc, bcount, spill_base, spill_n, T, Qout, qn_out,
+    move_emit<D>(key, walk, cc.c, 0, s, src, keep_src, x, cc.c, v, capc, bcount, spill_base, spill_n, T, Qout, qn_out,
                  qreg);
   }
 }

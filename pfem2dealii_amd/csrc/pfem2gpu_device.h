@@ -7,6 +7,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #pragma clang fp contract(off)
@@ -19,18 +20,23 @@ constexpr double kNewtonTol = 1e-12;
 
 // Per-instance particle parameters (one entry for a plain handle, one per
 // instance for a batch). src/pfem2particlehandler.cpp:34-39,47,362-364
-struct Prm {
+// The fields a move reads per particle come first, in the first 64 bytes, so
+// that four 16-byte loads issued together fetch them (load_move_prm).
+struct alignas(16) Prm {
   double dt_sub;   // dt / (double)n_sub
-  double h[3];     // 1.0 / q_k (sub-cell part width in reference coordinates)
   int n_sub;
+  int qpow2;       // every q_k is a power of two: xi / h_k == xi * q_k exactly
+  double h[3];     // 1.0 / q_k (sub-cell part width in reference coordinates)
+  double qf[3];    // q_k as double
   int maxpp;       // max_particles_per_cell_part
   int q[3];        // parts per direction
   int Q[3];        // q_k + 1: part buckets per direction (xi_k == 1 gives index q_k)
   int P;           // prod Q_k: part buckets per cell
   int Pvalid;      // prod q_k: refillable parts per cell
-  int qpow2;       // every q_k is a power of two: xi / h_k == xi * q_k exactly
-  double qf[3];    // q_k as double
 };
+static_assert(sizeof(Prm) % 16 == 0 && offsetof(Prm, n_sub) == 8 && offsetof(Prm, qpow2) == 12 &&
+                  offsetof(Prm, h) == 16 && offsetof(Prm, qf) == 40,
+              "load_move_prm reads the first 64 bytes of a Prm row in this layout");
 
 struct DMesh {
   const double* X[3];            // vertex coordinates, SoA
@@ -294,9 +300,61 @@ __device__ __forceinline__ double div_by(double d, double h, double r) {
   return __builtin_fma(rem, r, q0);
 }
 
-// part index (int)(xi_k / h_k) with h_k = 1.0/q_k (src/pfem2particlehandler.cpp:370)
-__device__ __forceinline__ int part_index(const Prm& p, int k, double xi) {
+// part index (int)(xi_k / h_k) with h_k = 1.0/q_k (src/pfem2particlehandler.cpp:370);
+// PP is Prm or MovePrm<D>
+template <class PP>
+__device__ __forceinline__ int part_index(const PP& p, int k, double xi) {
   return p.qpow2 ? (int)(xi * p.qf[k]) : (int)(xi / p.h[k]);
+}
+
+// The parameters of a move (substep count and length, part widths), held in
+// registers: loaded in one round of four 16-byte loads from a Prm row, or
+// made wave-uniform (scalar registers) for a handle with one instance.
+template <int D>
+struct MovePrm {
+  double dt_sub;
+  int n_sub, qpow2;
+  double h[D], qf[D];
+};
+
+template <int D>
+__device__ __forceinline__ MovePrm<D> load_move_prm(const Prm* __restrict__ p) {
+  const double2* q = reinterpret_cast<const double2*>(p);
+  const double2 a = q[0], b = q[1], c = q[2], d = q[3];  // bytes 0-63
+  MovePrm<D> r;
+  r.dt_sub = a.x;
+  const long long nq = __double_as_longlong(a.y);
+  r.n_sub = (int)(nq & 0xffffffffLL);
+  r.qpow2 = (int)(nq >> 32);
+  const double hq[6] = {b.x, b.y, c.x, c.y, d.x, d.y};  // h[0..2], qf[0..2]
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    r.h[k] = hq[k];
+    r.qf[k] = hq[3 + k];
+  }
+  return r;
+}
+
+__device__ __forceinline__ double uniform_double(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// The same values in scalar registers (every lane loads the same row).
+template <int D>
+__device__ __forceinline__ MovePrm<D> uniform_move_prm(const Prm* __restrict__ p) {
+  MovePrm<D> r = load_move_prm<D>(p);
+  r.dt_sub = uniform_double(r.dt_sub);
+  r.n_sub = __builtin_amdgcn_readfirstlane(r.n_sub);
+  r.qpow2 = __builtin_amdgcn_readfirstlane(r.qpow2);
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    r.h[k] = uniform_double(r.h[k]);
+    r.qf[k] = uniform_double(r.qf[k]);
+  }
+  return r;
 }
 
 // u_k = sum_i N_i(xi) F_k[dof_i], vertex-outer, component-inner, from 0.0.
@@ -333,8 +391,8 @@ __device__ __forceinline__ int part_lin(const Prm& p, const double* xi, bool* va
 // within a few ulps of that centre, well inside the part. The centre is
 // (idx + 1/2) * h_k as seeding and refill compute it
 // (src/pfem2particlehandler.cpp:88-91,390-394).
-template <int D>
-__device__ __forceinline__ void snap_part(const Prm& p, double* xi) {
+template <int D, class PP>
+__device__ __forceinline__ void snap_part(const PP& p, double* xi) {
 #pragma unroll
   for (int k = 0; k < D; ++k) xi[k] = ((double)part_index(p, k, xi[k]) + 1.0 / 2) * p.h[k];
 }

@@ -150,16 +150,18 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const int* __restrict__ in
 }
 
 // -------------------------------------------------------------- move passes
-// Adds, for every distinct key >= 0 held by the lanes of the wavefront, the
-// number of lanes holding it to arr[key] (one atomic per key) and returns each
-// lane's old value plus its rank among the lanes with the same key. The keys
-// are grouped first (ballots only); the group leaders then issue their
-// atomics together, so the wavefront waits for one round trip however many
+// Groups the lanes of the wavefront by key (ballots only, no memory access):
+// for a lane with key >= 0, leader_of is the lowest lane holding the same key
+// and rank its position among those lanes; a leader's cnt is the size of its
+// group. The caller's group leaders then add cnt to a per-key counter, all in
+// one instruction, so the wavefront waits for one round trip however many
 // keys it holds. Must be called by all 64 lanes.
-__device__ __forceinline__ int wave_group_add(int* __restrict__ arr, int key) {
+__device__ __forceinline__ void wave_groups(int key, int& leader_of, int& rank, int& cnt) {
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (1ULL << lane) - 1ULL;
-  int leader_of = lane, rank = 0, cnt = 0;
+  leader_of = lane;
+  rank = 0;
+  cnt = 0;
   unsigned long long pending = __ballot(key >= 0);
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
@@ -172,10 +174,6 @@ __device__ __forceinline__ int wave_group_add(int* __restrict__ arr, int key) {
     if (lane == leader) cnt = __popcll(grp);
     pending &= ~grp;
   }
-  int base = 0;
-  if (key >= 0 && leader_of == lane) base = atomicAdd(&arr[key], cnt);
-  base = __shfl(base, leader_of, 64);
-  return base + rank;
 }
 
 // Spill binning: particles that did not fit their chunk's bucket are counted
@@ -242,12 +240,19 @@ __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int tfla
   const unsigned long long wm = __ballot(walk);
   const int stripe = blockIdx.x % kQStripes;
   const int qleader = wm ? __ffsll((unsigned long long)wm) - 1 : 0;
-  // the queue reservation and the bucket reservations are issued before
-  // either is waited for (one round trip for both)
-  int qb = 0;
-  if (wm && lane == qleader) qb = atomicAdd(&qn[stripe], __popcll(wm));
-  int slot = 0;
-  if (capc >= 0) slot = wave_group_add(bcount, key);
+  // lanes grouped by bucket first; then the queue reservation and the bucket
+  // reservations are issued together and waited for once
+  int leader_of = lane, rank = 0, cnt = 0;
+  if (capc >= 0) wave_groups(key, leader_of, rank, cnt);
+  int qb = 0, base = 0;
+  // (the stripe index passes through a vector register: with a wave-uniform
+  // address the compiler would rebuild the atomic as a wave reduction whose
+  // result is read back at once, a second round trip)
+  int qi = stripe;
+  asm volatile("" : "+v"(qi));
+  if (wm && lane == qleader) qb = atomicAdd(&qn[qi], __popcll(wm));
+  if (capc >= 0 && key >= 0 && leader_of == lane) base = atomicAdd(&bcount[key], cnt);
+  const int slot = __shfl(base, leader_of, 64) + rank;
   if (wm) {
     qb = __shfl(qb, qleader, 64);
     if (qb + __popcll(wm) > qreg) {  // the stripe is full: report, never write past it
@@ -333,8 +338,13 @@ __global__ __launch_bounds__(B) void k_move_cell(int advect, DMesh m, const doub
                                                      int* __restrict__ bcount,
                                                      long spill_base, int* __restrict__ spill_n, DTmp T, DWalk Q,
                                                      int* __restrict__ qn, long qreg, DPath path) {
+  constexpr int NV = 1 << D;
   const int n = *n_ptr + n_add;
   const bool keep_src = T.src || capc < 0 || path.c;
+  // a handle with one instance holds its move parameters in scalar registers
+  const bool single = m.cell_inst == nullptr;
+  MovePrm<D> mp1{};
+  if (single) mp1 = uniform_move_prm<D>(m.prm);
   for (long base = (long)blockIdx.x * blockDim.x; base < n; base += (long)gridDim.x * blockDim.x) {
     const int p = (int)(base + threadIdx.x);
     int key = -1, s = 0, tflag = 0, qflag = 0;
@@ -349,34 +359,64 @@ __global__ __launch_bounds__(B) void k_move_cell(int advect, DMesh m, const doub
       v[k] = 0.0;
       u[k] = 0.0;
     }
+    // first round of loads: the cell word, position and velocity together
+    int w = -1;
     if (p < n) {
-      const int w = P.cell[p];
-      if (w >= 0) {
-        const int c = cell_of(w);
-        load_cell<D>(m, Vc, c, cc, advect != 0);
-        const Prm& pr = m.prm[inst_of(m, c)];
+      w = P.cell[p];
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-          x[k] = P.x[k][p];
-          v[k] = P.v[k][p];
-        }
-        inv_cached<D>(m, cc, x, xi);
-        if (placed_of(w)) snap_part<D>(pr, xi);
-        // a move locates every particle at least once (n_sub >= 1); a
-        // re-binning keeps the flag
-        if (!advect) tflag = w & kPlaced;
-        if (use_dv) {
-          double corr[D];
-          gather_table<D>(dVc, c, xi, corr);
-#pragma unroll
-          for (int k = 0; k < D; ++k) v[k] = v[k] + corr[k];
-        }
-        walk = substeps_in_cell<D>(m, Vc, cc, advect ? pr.n_sub : 0, pr.dt_sub, s, x, xi, u, path, p);
-        if (!walk) key = c >> cshift;
-        // a leaver's queued reference position is recomputed from x by the
-        // walk: the snap of the first substep travels as the cell word's flag
-        if (walk && s == 0 && placed_of(w)) qflag = kPlaced;
+      for (int k = 0; k < D; ++k) {
+        x[k] = P.x[k][p];
+        v[k] = P.v[k][p];
       }
+    }
+    if (w >= 0) {
+      const int c = cell_of(w);
+      // second round: the cell's geometry, field, correction table and
+      // (batch) instance parameters, all before the first use
+      load_cell<D>(m, Vc, c, cc, advect != 0);
+      // (2D: the correction table too; 3D reads it after the inverse map,
+      // where its 24 values need not be held across it)
+      double dvc[D == 2 ? NV * D : 1];
+      if constexpr (D == 2) {
+        if (use_dv) {
+          const double2* f = reinterpret_cast<const double2*>(dVc + (size_t)c * NV * D);
+#pragma unroll
+          for (int i = 0; i < NV; ++i) {
+            const double2 t = f[i];
+            dvc[2 * i] = t.x;
+            dvc[2 * i + 1] = t.y;
+          }
+        }
+      }
+      const MovePrm<D> mp = single ? mp1 : load_move_prm<D>(m.prm + m.cell_inst[c]);
+      inv_cached<D>(m, cc, x, xi);
+      if (placed_of(w)) snap_part<D>(mp, xi);
+      // a move locates every particle at least once (n_sub >= 1); a
+      // re-binning keeps the flag
+      if (!advect) tflag = w & kPlaced;
+      if (use_dv) {
+        double corr[D];
+        if constexpr (D == 2) {
+          // gather_table's sum from the preloaded table
+          double N[NV];
+          shape<D>(xi, N);
+#pragma unroll
+          for (int k = 0; k < D; ++k) corr[k] = 0.0;
+#pragma unroll
+          for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int k = 0; k < D; ++k) corr[k] = corr[k] + N[i] * dvc[i * D + k];
+        } else {
+          gather_table<D>(dVc, c, xi, corr);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) v[k] = v[k] + corr[k];
+      }
+      walk = substeps_in_cell<D>(m, Vc, cc, advect ? mp.n_sub : 0, mp.dt_sub, s, x, xi, u, path, p);
+      if (!walk) key = c >> cshift;
+      // a leaver's queued reference position is recomputed from x by the
+      // walk: the snap of the first substep travels as the cell word's flag
+      if (walk && s == 0 && placed_of(w)) qflag = kPlaced;
     }
     move_emit<D>(key, walk, cc.c, tflag, s, p, keep_src, x, cc.c | qflag, v, capc, bcount, spill_base, spill_n, T, Q,
                  qn, qreg);
@@ -396,6 +436,9 @@ __global__ __launch_bounds__(B) void k_move_walk(DMesh m, const double* __restri
   const int per = gridDim.x / kQStripes;
   const int nq = qn_in[stripe];
   const bool keep_src = T.src || capc < 0 || path.c;
+  const bool single = m.cell_inst == nullptr;
+  MovePrm<D> mp1{};
+  if (single) mp1 = uniform_move_prm<D>(m.prm);
   for (long base = (long)(blockIdx.x / kQStripes) * blockDim.x; base < nq; base += (long)per * blockDim.x) {
     const int i = (int)(stripe * qreg + base + threadIdx.x);
     const bool in = base + threadIdx.x < nq;
@@ -431,7 +474,7 @@ __global__ __launch_bounds__(B) void k_move_walk(DMesh m, const double* __restri
         }
       }
       load_cell<D>(m, Vc, c, cc, true);
-      const Prm& pr = m.prm[inst_of(m, c)];
+      const MovePrm<D> pr = single ? mp1 : load_move_prm<D>(m.prm + m.cell_inst[c]);
       // the failing substep again, bit for bit: its reference position (the
       // left cell's inverse map of the queued position, snapped to the part
       // centre for a placed particle's first substep), its velocity and its

@@ -307,6 +307,20 @@ __device__ __forceinline__ int part_index(const PP& p, int k, double xi) {
   return p.qpow2 ? (int)(xi * p.qf[k]) : (int)(xi / p.h[k]);
 }
 
+// A whole Prm row copied into registers with its 16-byte reads issued
+// together (from LDS or global memory), so that code branching on some of
+// its fields does not wait for each read in turn.
+__device__ __forceinline__ Prm prm_regs(const Prm* __restrict__ p) {
+  union Rows {
+    Prm prm;
+    double2 v[sizeof(Prm) / 16];
+  } u;
+  const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(Prm) / 16); ++i) u.v[i] = q[i];
+  return u.prm;
+}
+
 // The parameters of a move (substep count and length, part widths), held in
 // registers: loaded in one round of four 16-byte loads from a Prm row, or
 // made wave-uniform (scalar registers) for a handle with one instance.
@@ -888,13 +902,25 @@ __device__ __forceinline__ int walk_box2(const DMesh& m, const CellCache<2>& cc,
   int first = -1;
   double xf[2] = {0.0, 0.0};
   double gf[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // corners of the first containing candidate
+  // every candidate's corners in one round of loads (an absent candidate, or
+  // the left cell itself, reads the left cell's row and is skipped below)
+  double gc[4][6];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int id = ids[j] < 0 ? cc.c : ids[j];
+    const double2* gp = reinterpret_cast<const double2*>(m.geom + (size_t)id * 6);
+    const double2 a = gp[0], b = gp[1], c = gp[2];
+    gc[j][0] = a.x;
+    gc[j][1] = a.y;
+    gc[j][2] = b.x;
+    gc[j][3] = b.y;
+    gc[j][4] = c.x;
+    gc[j][5] = c.y;
+  }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     if (ids[j] < 0 || ids[j] == cc.c) continue;
-    double g[6];
-    const double* gp = m.geom + (size_t)ids[j] * 6;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) g[q] = gp[q];
+    const double* g = gc[j];
     double xn[2];
     bool ok;
     if (g[0] == g[0]) {

@@ -427,7 +427,7 @@ __global__ __launch_bounds__(B) void k_move_cell(int advect, DMesh m, const doub
 // (the grid is a multiple of kQStripes) and writes to the same stripe of the
 // output queue.
 template <int D, int B>
-__global__ __launch_bounds__(B) void k_move_walk(DMesh m, const double* __restrict__ Vc, DWalk Qin,
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(D == 2 ? 3 : 1))) void k_move_walk(DMesh m, const double* __restrict__ Vc, DWalk Qin,
                                                      const int* __restrict__ qn_in, long qreg, int cshift, int capc,
                                                      int* __restrict__ bcount, long spill_base,
                                                      int* __restrict__ spill_n, DTmp T, DWalk Qout,
@@ -980,7 +980,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       ppl[u] = 0;
       if (j < n) {
         const int lc = cell_of(pc[u]) - c0;
-        const Prm& pr = s_prm[lc];
+        const Prm pr = prm_regs(&s_prm[lc]);
         double xi[D];
         settle_xi<D>(m, pr, cell_of(pc[u]), placed_of(pc[u]), s_geo + lc * 3 * D, px[u], xi);
         const int part = part_lin<D>(pr, xi, nullptr);
@@ -1008,7 +1008,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       const int cw = T.cell[i];
       const int c = cell_of(cw);
       const int lc = c - c0;
-      const Prm& pr = s_prm[lc];
+      const Prm pr = prm_regs(&s_prm[lc]);
       double x[D], xi[D];
 #pragma unroll
       for (int k = 0; k < D; ++k) x[k] = T.x[k][i];
@@ -1033,8 +1033,9 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       const int lc = t >> pshift;
       const int k = s_cnt[t];
       int pi[D];
-      const bool valid = ((t & (pmax - 1)) < s_prm[lc].P) && part_code<D>(s_prm[lc], t & (pmax - 1), pi);
-      const int kept = apply_cap ? min(k, s_prm[lc].maxpp + 1) : k;
+      const Prm pr = prm_regs(&s_prm[lc]);
+      const bool valid = ((t & (pmax - 1)) < pr.P) && part_code<D>(pr, t & (pmax - 1), pi);
+      const int kept = apply_cap ? min(k, pr.maxpp + 1) : k;
       const int refill = (do_refill && s_ccnt[lc] > 0 && k == 0 && valid) ? 1 : 0;
       s_base[t] = kept + refill;
       mine += ((unsigned long long)(kept + refill) << 32) | (unsigned)k;
@@ -1213,7 +1214,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       for (int t = threadIdx.x; t < ncell * pmax; t += blockDim.x) {
         const int lc = t >> pshift, pp = t & (pmax - 1);
         if (s_cnt[t] != 0 || s_ccnt[lc] == 0) continue;
-        const Prm& pr = s_prm[lc];
+        const Prm pr = prm_regs(&s_prm[lc]);
         int pi[D];
         if (pp >= pr.P || !part_code<D>(pr, pp, pi)) continue;  // padding or no refillable part
         const int c = c0 + lc;

@@ -750,7 +750,10 @@ static Prm make_prm(int dim, const pfem2gpu_params* params) {
     p.h[k] = 1.0 / (double)(unsigned)p.q[k];
   }
   p.qpow2 = 1;
-  for (int k = 0; k < 3; ++k) p.qf[k] = (double)p.q[k];
+  for (int k = 0; k < 3; ++k) {
+    p.qf[k] = (double)p.q[k];
+    p.Qm[k] = k < dim ? (unsigned)((0x100000000ULL + (unsigned)p.Q[k] - 1) / (unsigned)p.Q[k]) : 0u;
+  }
   for (int k = 0; k < dim; ++k) {
     p.P *= p.Q[k];
     p.Pvalid *= p.q[k];

@@ -33,8 +33,9 @@ struct alignas(16) Prm {
   int Q[3];        // q_k + 1: part buckets per direction (xi_k == 1 gives index q_k)
   int P;           // prod Q_k: part buckets per cell
   int Pvalid;      // prod q_k: refillable parts per cell
+  unsigned Qm[3];  // ceil(2^32 / Q_k): l / Q_k == umulhi(l, Qm_k) for 0 <= l < 2^16 (0 for k >= dim)
 };
-static_assert(sizeof(Prm) % 16 == 0 && offsetof(Prm, n_sub) == 8 && offsetof(Prm, qpow2) == 12 &&
+static_assert(sizeof(Prm) == 112 && offsetof(Prm, n_sub) == 8 && offsetof(Prm, qpow2) == 12 &&
                   offsetof(Prm, h) == 16 && offsetof(Prm, qf) == 40,
               "load_move_prm reads the first 64 bytes of a Prm row in this layout");
 

@@ -841,11 +841,15 @@ __device__ __forceinline__ void cells_store(const CellRegs<D>& r, int ncell, dou
 // slowest; valid when every pi_k < q_k.
 template <int D>
 __device__ __forceinline__ bool part_code(const Prm& pr, int lin, int* pi) {
+  // the divisions by Q_k as multiplications by Qm_k: exact for lin < 2^16
+  // (part buckets per cell stay below 2^8, bounded by the settle's LDS)
   bool ok = true;
+  unsigned l = (unsigned)lin;
 #pragma unroll
   for (int k = D - 1; k >= 0; --k) {
-    pi[k] = lin % pr.Q[k];
-    lin /= pr.Q[k];
+    const unsigned qd = __umulhi(l, pr.Qm[k]);
+    pi[k] = (int)(l - qd * (unsigned)pr.Q[k]);
+    l = qd;
     ok = ok && pi[k] < pr.q[k];
   }
   return ok;
@@ -937,6 +941,12 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       const int nb = min(s_misc[1], capc);
       load_parts(nb + s_misc[3] - s_misc[2], nb, (long)ch * capc, spill2_base + s_misc[2]);
     }
+#pragma unroll
+    for (int u = 0; u < PRE; ++u) {  // complete before the loop (as at the end of every iteration)
+      asm volatile("" ::"v"(pc[u]));
+#pragma unroll
+      for (int k = 0; k < D; ++k) asm volatile("" ::"v"(px[u][k]));
+    }
     for (int i = threadIdx.x; i < cpw * pmax; i += blockDim.x) s_cnt[i] = 0;
     for (int i = threadIdx.x; i < cpw; i += blockDim.x) s_ccnt[i] = 0;
   }
@@ -945,7 +955,14 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     const int ch = s_misc[0];
     if (ch >= nchunks) break;
     int next = 0;
-    if (threadIdx.x == 0) next = atomicAdd(ticket, 1);  // the next chunk, drawn early
+    if (threadIdx.x == 0) {  // the next chunk, drawn early and waited for only where its header is read
+      // (the counter's index passes through a vector register: with a
+      // wave-uniform address the compiler would rebuild the atomic as a wave
+      // reduction whose result is read back, and waited for, at once)
+      int z = 0;
+      asm volatile("" : "+v"(z));
+      next = atomicAdd(ticket + z, 1);
+    }
     const int c0 = ch << cshift;
     const int c1 = min(nc, c0 + cpw);
     const int ncell = c1 - c0;
@@ -963,16 +980,16 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     //    (cell, part)
     int ppl[PRE];
     double pv[PRE][D];  // velocities: loaded here, stored with the classification
+    // (loaded by every lane: a lane past the chunk's particles reads slot 0
+    // of the chunk's bucket, in bounds and unused, so that the number of
+    // loads in flight is the same on every path and the waits for them stay
+    // at their uses)
 #pragma unroll
     for (int u = 0; u < PRE; ++u) {
       const int j = threadIdx.x + u * blockDim.x;
+      const long i = (j < n && staged_in) ? chunk_index(j, nb, bucket0, spill0) : bucket0;
 #pragma unroll
-      for (int k = 0; k < D; ++k) pv[u][k] = 0.0;
-      if (j < n && staged_in) {
-        const long i = chunk_index(j, nb, bucket0, spill0);
-#pragma unroll
-        for (int k = 0; k < D; ++k) pv[u][k] = T.v[k][i];
-      }
+      for (int k = 0; k < D; ++k) pv[u][k] = T.v[k][i];
     }
 #pragma unroll
     for (int u = 0; u < PRE; ++u) {
@@ -1187,6 +1204,14 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     // C. output offset of a staged chunk by decoupled look-back on the first
     //    wavefront: its first round of status loads is issued here and
     //    waited for after the wavefront's placement and refills
+    // (the source indices of a move that records them are read before the
+    // look-back's loads, so that the placement below waits for neither)
+    int psrc[PRE];
+#pragma unroll
+    for (int u = 0; u < PRE; ++u) {
+      psrc[u] = 0;
+      if (P.src && staged && pf[u] >= 0) psrc[u] = T.src[chunk_index(threadIdx.x + u * blockDim.x, nb, bucket0, spill0)];
+    }
     const bool lb_on = staged && threadIdx.x < 64 && ch > 0;
     unsigned long long lbw = 0;
     if (lb_on) lbw = lb_word(lb, ch - 1 - (int)threadIdx.x);
@@ -1204,7 +1229,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
             s_fxi[k * ns + f] = qxi[u][k];
           }
           s_fcell[f] = (unsigned char)((ppl[u] >> 16) | ((ppl[u] & 0x8000) ? 0x80 : 0));
-          if (P.src) s_fsrc[f] = T.src[chunk_index(threadIdx.x + u * blockDim.x, nb, bucket0, spill0)];
+          if (P.src) s_fsrc[f] = psrc[u];
         }
       }
     }
@@ -1287,6 +1312,21 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       const int nnb = min(s_misc[5], capc);
       load_parts(nnb + s_misc[7] - s_misc[6], nnb, (long)nch * capc, spill2_base + s_misc[6]);
     }
+    // The next chunk's cell data into the other buffer, before any store of
+    // this chunk (a wait for these loads also waits for every store issued
+    // before it); its particles, loaded with them, are complete too: waiting
+    // for them here (an empty asm that reads them) keeps the next iteration's
+    // first wait from covering its velocity loads.
+    auto next_chunk_in = [&]() {
+      if (nch < nchunks)
+        cells_store<D>(cr, nncell, cbuf_geo(buf ^ 1), cbuf_vc(buf ^ 1), cbuf_prm(buf ^ 1), cbuf_bpos(buf ^ 1));
+#pragma unroll
+      for (int u = 0; u < PRE; ++u) {
+        asm volatile("" ::"v"(pc[u]));
+#pragma unroll
+        for (int k = 0; k < D; ++k) asm volatile("" ::"v"(px[u][k]));
+      }
+    };
     if (staged) {
       // E1. per (cell, vertex) sums S_k = sum N_i v_k and W = sum N_i over
       //     the cell's particles in final order, each formed in kSumParts
@@ -1295,7 +1335,12 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       //     sum on neighbouring lanes
       constexpr int QP = kSumParts<D>;
       constexpr int G = 4;  // the loads of a group of particles issued together
-      const int chain = (int)threadIdx.x / QP, part = (int)threadIdx.x % QP;
+      // (the lane's chain and part are rebuilt from the thread index here:
+      // values held across the chunk loop are spilled, and a reload at this
+      // point would wait for the next chunk's loads issued above)
+      int tix = (int)threadIdx.x;
+      asm volatile("" : "+v"(tix));
+      const int chain = tix / QP, part = tix % QP;
       const int lc = chain >> D, iv = chain & (NV - 1);
       int a = 0, b = 0;
       if (lc < ncell) {
@@ -1366,12 +1411,19 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         for (int k = 0; k < D; ++k) T[k] = T[k] + PS[j][k];
         TW = TW + PS[j][D];
       }
+      // (the next chunk's cell data goes to the other buffer here, between
+      // the sums and their store, see below)
+      next_chunk_in();
       if (part == 0 && lc < ncell) {
         double* co = contrib + (size_t)cbuf_bpos(buf)[lc * NV + iv] * (D + 1);
 #pragma unroll
         for (int k = 0; k < D; ++k) co[k] = T[k];
         co[D] = TW;
       }
+    } else {
+      next_chunk_in();
+    }
+    if (staged) {
       // E2. coalesced write-out in final order, two particles per thread
       //     and round (their LDS reads issued together)
       for (int t0 = threadIdx.x; t0 < nout; t0 += 2 * kSettleBlock) {
@@ -1408,8 +1460,6 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     // the counts of the next chunk start at zero (read last by the refills)
     for (int i = threadIdx.x; i < cpw * pmax; i += blockDim.x) s_cnt[i] = 0;
     for (int i = threadIdx.x; i < cpw; i += blockDim.x) s_ccnt[i] = 0;
-    if (nch < nchunks)
-      cells_store<D>(cr, nncell, cbuf_geo(buf ^ 1), cbuf_vc(buf ^ 1), cbuf_prm(buf ^ 1), cbuf_bpos(buf ^ 1));
     if (threadIdx.x == 0) {
       dirty[ch] = staged ? 0 : 1;
       s_misc[0] = s_misc[4];

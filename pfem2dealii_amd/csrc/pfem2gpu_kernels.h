@@ -769,6 +769,12 @@ struct SettlePre {
 // Settle workgroup: four wavefronts per chunk of 2^kSettleShift cells.
 constexpr int kSettleBlock = 256;
 constexpr int kSettleShift = 4;
+// The look-back (publishing the chunk's total and prefix, and waiting for
+// its predecessors) runs on the last wavefront, the one with the fewest of a
+// chunk's particles (particle j sits on thread j % kSettleBlock, so the last
+// wavefront holds the high j of the second round, usually past the chunk's
+// count); thread 0 draws the tickets and reads the chunk headers.
+constexpr int kLbLead = kSettleBlock - 64;
 // the final-order cell byte holds the chunk-local cell in bits 0-6 and the
 // placed flag in bit 7
 static_assert(kSettleShift <= 7, "chunk-local cell ids must fit seven bits");
@@ -806,6 +812,9 @@ struct CellRegs {
 template <int D>
 __device__ __forceinline__ void cells_load(const DMesh& m, const double* __restrict__ Vc, int do_refill, int c0,
                                            int ncell, CellRegs<D>& r) {
+  // (a handle with one instance keeps its parameter words in both buffers
+  // from the start: no dependent load here)
+  const bool single = m.cell_inst == nullptr;
   constexpr int NV = 1 << D;
   int t = threadIdx.x;
   // the per-lane addresses below are rebuilt at every call rather than kept
@@ -813,8 +822,9 @@ __device__ __forceinline__ void cells_load(const DMesh& m, const double* __restr
   asm volatile("" : "+v"(t));
   // the parameter words first: their address depends on a load (the cell's
   // instance) in a batch, and the wait for it then covers no other load
-  r.prm = t < ncell * kPrmWords ? reinterpret_cast<const double*>(&m.prm[inst_of(m, c0 + t / kPrmWords)])[t % kPrmWords]
-                                : 0.0;
+  r.prm = (!single && t < ncell * kPrmWords)
+              ? reinterpret_cast<const double*>(&m.prm[m.cell_inst[c0 + t / kPrmWords]])[t % kPrmWords]
+              : 0.0;
   r.bpos = t < ncell * NV ? m.bpos[(size_t)c0 * NV + t] : 0;
   r.geo = t < ncell * 3 * D ? m.geom[(size_t)c0 * 3 * D + t] : 0.0;
 #pragma unroll
@@ -825,15 +835,15 @@ __device__ __forceinline__ void cells_load(const DMesh& m, const double* __restr
 }
 
 template <int D>
-__device__ __forceinline__ void cells_store(const CellRegs<D>& r, int ncell, double* geo, double* vc, Prm* prm,
-                                            int* bpos) {
+__device__ __forceinline__ void cells_store(const CellRegs<D>& r, bool single, int ncell, double* geo, double* vc,
+                                            Prm* prm, int* bpos) {
   constexpr int NV = 1 << D;
   const int t = threadIdx.x;
   if (t < ncell * 3 * D) geo[t] = r.geo;
 #pragma unroll
   for (int q = 0; q < 2; ++q)
     if (t + q * kSettleBlock < ncell * NV * D) vc[t + q * kSettleBlock] = r.vc[q];
-  if (t < ncell * kPrmWords) reinterpret_cast<double*>(prm)[t] = r.prm;
+  if (!single && t < ncell * kPrmWords) reinterpret_cast<double*>(prm)[t] = r.prm;
   if (t < ncell * NV) bpos[t] = r.bpos;
 }
 
@@ -937,7 +947,8 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       const int c0 = ch << cshift;
       CellRegs<D> cr;
       cells_load<D>(m, Vc, do_refill, c0, min(nc, c0 + cpw) - c0, cr);
-      cells_store<D>(cr, min(nc, c0 + cpw) - c0, cbuf_geo(0), cbuf_vc(0), cbuf_prm(0), cbuf_bpos(0));
+      cells_store<D>(cr, m.cell_inst == nullptr, min(nc, c0 + cpw) - c0, cbuf_geo(0), cbuf_vc(0), cbuf_prm(0),
+                     cbuf_bpos(0));
       const int nb = min(s_misc[1], capc);
       load_parts(nb + s_misc[3] - s_misc[2], nb, (long)ch * capc, spill2_base + s_misc[2]);
     }
@@ -949,6 +960,12 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     }
     for (int i = threadIdx.x; i < cpw * pmax; i += blockDim.x) s_cnt[i] = 0;
     for (int i = threadIdx.x; i < cpw; i += blockDim.x) s_ccnt[i] = 0;
+    if (m.cell_inst == nullptr)  // one instance: its parameters for every cell of both buffers, once
+      for (int i = threadIdx.x; i < cpw * kPrmWords; i += blockDim.x) {
+        const double w = reinterpret_cast<const double*>(m.prm)[i % kPrmWords];
+        reinterpret_cast<double*>(cbuf_prm(0))[i] = w;
+        reinterpret_cast<double*>(cbuf_prm(1))[i] = w;
+      }
   }
   __syncthreads();
   for (;;) {
@@ -1070,7 +1087,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     }
     const int nout = (int)(tot >> 32);
     const bool staged = staged_in && nout <= nslots;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == kLbLead) {
       lb_publish(lb, ch, ch == 0 ? 2u : 1u, (unsigned)nout);
       if (ch == 0) s_misc[8] = 0;
       if (!staged && ch > 0) {  // the global-memory path writes as it goes
@@ -1078,6 +1095,8 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         lb_publish(lb, ch, 2u, prefix + (unsigned)nout);
         s_misc[8] = (int)prefix;
       }
+    }
+    if (threadIdx.x == 0) {
       s_misc[4] = next;
       s_misc[5] = nh0;
       s_misc[6] = nh1;
@@ -1124,28 +1143,45 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
           const int t = (lc << pshift) + (ppl[u] & 0x7fff);
           const unsigned long long key = s_key[j];
           const int b = s_roff[t], a = b - s_cnt[t];
-          int r = 0;
-          for (int q = a; q < b; ++q) {
-            const unsigned long long kq = s_gkey[q];
-            if (kq < key) {
-              ++r;
-            } else if (kq == key) {
-              const int o = s_ord[q];
-              if (o == j) continue;
-              double xq[D], vq[D];
-              const long iq = chunk_index(o, nb, bucket0, spill0);
+          const int maxpp = s_prm[lc].maxpp;
+          // rank = keys below this one in the group, four keys read per
+          // round; equal keys (the particle's own and any other) counted too
+          int r = 0, eq = 0;
+          for (int q = a; q < b; q += 4) {
+            unsigned long long kq[4];
 #pragma unroll
-              for (int k = 0; k < D; ++k) {
-                xq[k] = T.x[k][iq];
-                vq[k] = T.v[k][iq];
+            for (int e = 0; e < 4; ++e) kq[e] = s_gkey[min(q + e, b - 1)];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if (q + e < b) {
+                r += kq[e] < key ? 1 : 0;
+                eq += kq[e] == key ? 1 : 0;
               }
-              double vj[D];
+          }
+          if (eq > 1) {  // another particle with the same key: position, velocity, index decide (rare)
+            r = 0;
+            for (int q = a; q < b; ++q) {
+              const unsigned long long kq = s_gkey[q];
+              if (kq < key) {
+                ++r;
+              } else if (kq == key) {
+                const int o = s_ord[q];
+                if (o == j) continue;
+                double xq[D], vq[D];
+                const long iq = chunk_index(o, nb, bucket0, spill0);
 #pragma unroll
-              for (int k = 0; k < D; ++k) vj[k] = s_sv[k * ns + j];
-              if (tie_less<D>(xq, vq, px[u], vj, o, j)) ++r;
+                for (int k = 0; k < D; ++k) {
+                  xq[k] = T.x[k][iq];
+                  vq[k] = T.v[k][iq];
+                }
+                double vj[D];
+#pragma unroll
+                for (int k = 0; k < D; ++k) vj[k] = s_sv[k * ns + j];
+                if (tie_less<D>(xq, vq, px[u], vj, o, j)) ++r;
+              }
             }
           }
-          if (!(apply_cap && r > s_prm[lc].maxpp)) pf[u] = s_base[t] + r;
+          if (!(apply_cap && r > maxpp)) pf[u] = s_base[t] + r;
         }
       }
     } else {
@@ -1201,7 +1237,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         }
       }
     }
-    // C. output offset of a staged chunk by decoupled look-back on the first
+    // C. output offset of a staged chunk by decoupled look-back on the last
     //    wavefront: its first round of status loads is issued here and
     //    waited for after the wavefront's placement and refills
     // (the source indices of a move that records them are read before the
@@ -1212,9 +1248,9 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       psrc[u] = 0;
       if (P.src && staged && pf[u] >= 0) psrc[u] = T.src[chunk_index(threadIdx.x + u * blockDim.x, nb, bucket0, spill0)];
     }
-    const bool lb_on = staged && threadIdx.x < 64 && ch > 0;
+    const bool lb_on = staged && (int)threadIdx.x >= kLbLead && (int)threadIdx.x < kLbLead + 64 && ch > 0;
     unsigned long long lbw = 0;
-    if (lb_on) lbw = lb_word(lb, ch - 1 - (int)threadIdx.x);
+    if (lb_on) lbw = lb_word(lb, ch - 1 - ((int)threadIdx.x & 63));
     __syncthreads();  // the arrival-order arrays are dead from here on
     // final placement of the staged survivors
     if (staged) {
@@ -1292,7 +1328,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     }
     if (lb_on) {  // (its loads complete before the next chunk's, issued below)
       const unsigned prefix = lb_prefix_wave(lb, ch, lbw, err);
-      if (threadIdx.x == 0) {
+      if (threadIdx.x == kLbLead) {
         lb_publish(lb, ch, 2u, prefix + (unsigned)nout);
         s_misc[8] = (int)prefix;
       }
@@ -1319,7 +1355,8 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     // first wait from covering its velocity loads.
     auto next_chunk_in = [&]() {
       if (nch < nchunks)
-        cells_store<D>(cr, nncell, cbuf_geo(buf ^ 1), cbuf_vc(buf ^ 1), cbuf_prm(buf ^ 1), cbuf_bpos(buf ^ 1));
+        cells_store<D>(cr, m.cell_inst == nullptr, nncell, cbuf_geo(buf ^ 1), cbuf_vc(buf ^ 1), cbuf_prm(buf ^ 1),
+                       cbuf_bpos(buf ^ 1));
 #pragma unroll
       for (int u = 0; u < PRE; ++u) {
         asm volatile("" ::"v"(pc[u]));

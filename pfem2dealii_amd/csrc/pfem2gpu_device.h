@@ -825,13 +825,48 @@ __device__ int walk_box(const DMesh& m, const CellCache<D>& cc, const double* xi
   };
   if constexpr (D == 3) {
     // the candidate ids in one round of loads (nn >= 1: the list holds the
-    // current cell), then the candidates in list order
+    // current cell), then the candidates in list order, the corners of
+    // kCandRound of them read in one round (an absent candidate, or the
+    // current cell, reads the current cell's row and is skipped)
     int ids[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) ids[j] = m.v2c[o0 + (j < nn ? j : 0)];
+    constexpr int kCandRound = 2;
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (j < nn) test(j, ids[j]);
+    for (int j0 = 0; j0 < 8; j0 += kCandRound) {
+      double g[kCandRound][3 * D];
+#pragma unroll
+      for (int e = 0; e < kCandRound; ++e) {
+        const int j = j0 + e;
+        const int c2 = (j < nn) ? ids[j] : cc.c;
+        const double* gp = m.geom + (size_t)c2 * 3 * D;
+#pragma unroll
+        for (int q = 0; q < 3 * D; ++q) g[e][q] = gp[q];
+      }
+#pragma unroll
+      for (int e = 0; e < kCandRound; ++e) {
+        const int j = j0 + e;
+        if (j >= nn || ids[j] == cc.c) continue;  // the current cell already failed the same test
+        double xn[D];
+        bool ok;
+        if (g[e][0] == g[e][0]) {
+#pragma unroll
+          for (int k = 0; k < D; ++k) xn[k] = div_by(x[k] - g[e][k], g[e][D + k] - g[e][k], g[e][2 * D + k]);
+          ok = true;
+        } else {
+          ok = inv_candidate<D>(m, ids[j], x, xn);
+        }
+        if (ok && inside<D>(xn)) {
+          hits |= 1u << j;
+          if (first < 0) {
+            first = j;
+            cfirst = ids[j];
+#pragma unroll
+            for (int k = 0; k < D; ++k) xf[k] = xn[k];
+          }
+        }
+      }
+    }
   } else {
     for (int j = 0; j < nn; ++j) test(j, m.v2c[o0 + j]);
   }

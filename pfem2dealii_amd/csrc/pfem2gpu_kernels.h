@@ -875,6 +875,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
                                                   double* __restrict__ contrib, unsigned char* __restrict__ dirty) {
   constexpr int NV = 1 << D;
   constexpr int PRE = SettlePre<D>::n;
+  constexpr int PRV = D == 2 ? 2 : 0;  // register particles whose velocities are prefetched
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int cpw = 1 << cshift;
   const int pmax = 1 << pshift;  // part buckets per cell, padded to a power of two
@@ -908,6 +909,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
   // registers: the particles of this thread in the current chunk
   int pc[PRE];
   double px[PRE][D];
+  double pvr[PRV > 0 ? PRV : 1][D];  // velocities of the first PRV register particles, loaded with their positions
   auto load_parts = [&](int n, int nb, long bucket0, long spill0) {
 #pragma unroll
     for (int u = 0; u < PRE; ++u) {
@@ -915,11 +917,17 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       pc[u] = -1;
 #pragma unroll
       for (int k = 0; k < D; ++k) px[u][k] = 0.0;
+      if (u < PRV)
+#pragma unroll
+        for (int k = 0; k < D; ++k) pvr[u < PRV ? u : 0][k] = 0.0;
       if (j < n) {
         const long i = chunk_index(j, nb, bucket0, spill0);
         pc[u] = T.cell[i];
 #pragma unroll
         for (int k = 0; k < D; ++k) px[u][k] = T.x[k][i];
+        if (u < PRV)
+#pragma unroll
+          for (int k = 0; k < D; ++k) pvr[u < PRV ? u : 0][k] = T.v[k][i];
       }
     }
   };
@@ -957,6 +965,9 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       asm volatile("" ::"v"(pc[u]));
 #pragma unroll
       for (int k = 0; k < D; ++k) asm volatile("" ::"v"(px[u][k]));
+      if (u < PRV)
+#pragma unroll
+        for (int k = 0; k < D; ++k) asm volatile("" ::"v"(pvr[u < PRV ? u : 0][k]));
     }
     for (int i = threadIdx.x; i < cpw * pmax; i += blockDim.x) s_cnt[i] = 0;
     for (int i = threadIdx.x; i < cpw; i += blockDim.x) s_ccnt[i] = 0;
@@ -1003,10 +1014,15 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     // at their uses)
 #pragma unroll
     for (int u = 0; u < PRE; ++u) {
-      const int j = threadIdx.x + u * blockDim.x;
-      const long i = (j < n && staged_in) ? chunk_index(j, nb, bucket0, spill0) : bucket0;
+      if (u < PRV) {
 #pragma unroll
-      for (int k = 0; k < D; ++k) pv[u][k] = T.v[k][i];
+        for (int k = 0; k < D; ++k) pv[u][k] = pvr[u < PRV ? u : 0][k];
+      } else {
+        const int j = threadIdx.x + u * blockDim.x;
+        const long i = (j < n && staged_in) ? chunk_index(j, nb, bucket0, spill0) : bucket0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) pv[u][k] = T.v[k][i];
+      }
     }
 #pragma unroll
     for (int u = 0; u < PRE; ++u) {
@@ -1362,6 +1378,9 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         asm volatile("" ::"v"(pc[u]));
 #pragma unroll
         for (int k = 0; k < D; ++k) asm volatile("" ::"v"(px[u][k]));
+      if (u < PRV)
+#pragma unroll
+        for (int k = 0; k < D; ++k) asm volatile("" ::"v"(pvr[u < PRV ? u : 0][k]));
       }
     };
     if (staged) {

@@ -938,34 +938,44 @@ __device__ __forceinline__ int walk_box2(const DMesh& m, const CellCache<2>& cc,
   int first = -1;
   double xf[2] = {0.0, 0.0};
   double gf[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // corners of the first containing candidate
-  // every candidate's corners in one round of loads (an absent candidate, or
-  // the left cell itself, reads the left cell's row and is skipped below)
-  double gc[4][6];
+  // The current cell is one of the four (it holds the vertex) and already
+  // failed the same test: the other three candidates' corners are read in one
+  // round of loads (an absent candidate reads the current cell's row and is
+  // skipped), slot k of the three being list position k + (k >= self).
+  int self = -1;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int id = ids[j] < 0 ? cc.c : ids[j];
+  for (int j = 0; j < 4; ++j)
+    if (ids[j] == cc.c) self = j;
+  if (self < 0) return -2;  // (not a list of this cell's vertex: the generic walk decides)
+  int cid[3];
+  double gc[3][6];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    cid[k] = k >= self ? ids[k + 1] : ids[k];
+    const int id = cid[k] < 0 ? cc.c : cid[k];
     const double2* gp = reinterpret_cast<const double2*>(m.geom + (size_t)id * 6);
     const double2 a = gp[0], b = gp[1], c = gp[2];
-    gc[j][0] = a.x;
-    gc[j][1] = a.y;
-    gc[j][2] = b.x;
-    gc[j][3] = b.y;
-    gc[j][4] = c.x;
-    gc[j][5] = c.y;
+    gc[k][0] = a.x;
+    gc[k][1] = a.y;
+    gc[k][2] = b.x;
+    gc[k][3] = b.y;
+    gc[k][4] = c.x;
+    gc[k][5] = c.y;
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    if (ids[j] < 0 || ids[j] == cc.c) continue;
-    const double* g = gc[j];
+  for (int k = 0; k < 3; ++k) {
+    if (cid[k] < 0) continue;
+    const int j = k >= self ? k + 1 : k;
+    const double* g = gc[k];
     double xn[2];
     bool ok;
     if (g[0] == g[0]) {
 #pragma unroll
-      for (int k = 0; k < 2; ++k) xn[k] = div_by(x[k] - g[k], g[2 + k] - g[k], g[4 + k]);
+      for (int q = 0; q < 2; ++q) xn[q] = div_by(x[q] - g[q], g[2 + q] - g[q], g[4 + q]);
       ok = true;
     } else {
       int vid2[4];
-      load_ids<2>(m.cv, ids[j], vid2);
+      load_ids<2>(m.cv, cid[k], vid2);
       ok = inv_map_newton<2>(m, vid2, x, xn);
     }
     if (ok && inside<2>(xn)) {

@@ -82,7 +82,7 @@ struct pfem2gpu_handle {
   std::vector<long> inst_dof_off;    // [ninst+1]
   double* geom = nullptr;  // [nc][2*dim] box corners (X_0[0] = NaN for other cells)
   int2* cvo = nullptr;     // [nc][nvc] (v2c offset, count) per cell vertex
-  int4* cvc = nullptr;     // 2D: [nc][4] cells around each cell vertex (null in 3D)
+  int4* cvc = nullptr;     // [nc][2^dim][2^dim / 4]: the cells around each cell vertex
   int* ptab = nullptr;     // [pmax] part decode table
   double* Vc = nullptr;    // [nc][nvc][dim] advecting field, cell-major
   int pshift = 0;
@@ -1132,13 +1132,13 @@ int pfem2gpu_initialize_maps(pfem2gpu_handle* h) {
     CHK(dalloc(&h->v2c, total));
     CHK(dalloc(&h->v2c_dir, (size_t)total * dim));
     CHK(dalloc(&h->cvo, ne));
-    if (dim == 2) CHK(dalloc(&h->cvc, ne));
+    CHK(dalloc(&h->cvc, (size_t)ne * (nvc / 4)));
     const double inv_n = 1.0 / (double)nvc;
     if (dim == 2)
       LAUNCH(h, "k_maps", k_v2c_emit<2>, grid_for(nv), 0, m, nv, off, lst, off2, h->v2c, h->v2c_dir, inv_n);
     else
       LAUNCH(h, "k_maps", k_v2c_emit<3>, grid_for(nv), 0, m, nv, off, lst, off2, h->v2c, h->v2c_dir, inv_n);
-    LAUNCH(h, "k_maps", k_cell_vertex_maps, grid_for(ne), 0, ne, h->cv, off2, h->v2c, h->cvo, h->cvc);
+    LAUNCH(h, "k_maps", k_cell_vertex_maps, grid_for(ne), 0, ne, nvc, h->cv, off2, h->v2c, h->cvo, h->cvc);
     HIPCHK(hipStreamSynchronize(h->stream));
     h->v2c_off = off2;  // kept: the per-vertex offsets
     off2 = nullptr;

@@ -51,8 +51,8 @@ struct DMesh {
   const Prm* prm;                // per-instance parameters
   const double* geom;            // [nc][3D]: X_0, X_{2^D-1}, 1/(X_{2^D-1} - X_0) of box cells; X_0[0] = NaN otherwise
   const int2* cvo;               // [nc][2^D]: (v2c offset, count) of each cell vertex
-  const int4* cvc;               // 2D: [nc][4] the (up to 4) cells around each cell vertex, in v2c order,
-                                 // -1 padded; x = -2 when the vertex has more (null in 3D)
+  const int4* cvc;               // [nc][2^D][2^D / 4]: the (up to 2^D) cells around each cell vertex, in
+                                 // v2c order, -1 padded; first id -2 when the vertex has more
   const int* ptab;               // [ninst][pmax]: refillable part -> INT_MIN | pi_0 | pi_1<<8 | pi_2<<16, else 0
   const int* bpos;               // [nc][2^D]: place of (cell, vertex) in the DoF -> (cell, vertex) lists
 };
@@ -805,8 +805,18 @@ __device__ int walk_box(const DMesh& m, const CellCache<D>& cc, const double* xi
   fwd_box<D>(cc, xi, xold);
   const int bestv = closest_vertex_box<D>(cc, xold, u);
   const int2 oc = m.cvo[(size_t)cc.c * (1 << D) + bestv];
+  // 3D: the list itself in the same round of loads as its extent (the
+  // per-(cell, vertex) copy; read before the extent is tested, so that the
+  // two loads are waited for together)
+  int ids[8];
+  if constexpr (D == 3) {
+    const int4* cl = m.cvc + ((size_t)cc.c * 8 + bestv) * 2;
+    const int4 l0 = cl[0], l1 = cl[1];
+    ids[0] = l0.x; ids[1] = l0.y; ids[2] = l0.z; ids[3] = l0.w;
+    ids[4] = l1.x; ids[5] = l1.y; ids[6] = l1.z; ids[7] = l1.w;
+  }
   const int o0 = oc.x, nn = oc.y;
-  if (nn > 8) return -2;  // caller falls back to the generic walk
+  if (nn > 8 || (D == 3 && ids[0] == -2)) return -2;  // caller falls back to the generic walk
   unsigned hits = 0;
   int first = -1, cfirst = -1;
   double xf[D];
@@ -824,13 +834,9 @@ __device__ int walk_box(const DMesh& m, const CellCache<D>& cc, const double* xi
     }
   };
   if constexpr (D == 3) {
-    // the candidate ids in one round of loads (nn >= 1: the list holds the
-    // current cell), then the candidates in list order, the corners of
-    // kCandRound of them read in one round (an absent candidate, or the
-    // current cell, reads the current cell's row and is skipped)
-    int ids[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) ids[j] = m.v2c[o0 + (j < nn ? j : 0)];
+    // the candidates in list order, the corners of kCandRound of them read
+    // in one round (an absent candidate, or the current cell, reads the
+    // current cell's row and is skipped)
     constexpr int kCandRound = 2;
 #pragma unroll
     for (int j0 = 0; j0 < 8; j0 += kCandRound) {

@@ -1603,22 +1603,22 @@ __global__ void k_v2c_emit(DMesh m, int nv, const int* __restrict__ off, const i
   }
 }
 
-// per (cell, vertex): (offset, count) of the vertex's list; in 2D also the
-// list itself when it has at most four cells (x = -2 otherwise)
-__global__ void k_cell_vertex_maps(long ne, const int* __restrict__ cv, const int* __restrict__ off2,
+// per (cell, vertex): (offset, count) of the vertex's list, and the list
+// itself when it has at most 2^dim cells (first id -2 otherwise), -1 padded:
+// one int4 per (cell, vertex) in 2D, two in 3D
+__global__ void k_cell_vertex_maps(long ne, int nvc, const int* __restrict__ cv, const int* __restrict__ off2,
                                    const int* __restrict__ lst2, int2* __restrict__ cvo, int4* __restrict__ cvc) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < ne; e += (long)gridDim.x * blockDim.x) {
     const int v = cv[e];
     const int o = off2[v], n = off2[v + 1] - o;
     cvo[e] = make_int2(o, n);
-    if (cvc) {
-      int ids[4] = {-1, -1, -1, -1};
-      if (n > 4)
-        ids[0] = -2;
-      else
-        for (int j = 0; j < n; ++j) ids[j] = lst2[o + j];
-      cvc[e] = make_int4(ids[0], ids[1], ids[2], ids[3]);
-    }
+    int ids[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    if (n > nvc)
+      ids[0] = -2;
+    else
+      for (int j = 0; j < n; ++j) ids[j] = lst2[o + j];
+    cvc[e * (nvc / 4)] = make_int4(ids[0], ids[1], ids[2], ids[3]);
+    if (nvc == 8) cvc[e * 2 + 1] = make_int4(ids[4], ids[5], ids[6], ids[7]);
   }
 }
 

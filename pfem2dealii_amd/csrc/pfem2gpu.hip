@@ -1694,8 +1694,14 @@ int adjoint_impl(pfem2gpu_handle* h, int n_steps, int K, const double* const* Vu
       LAUNCH(h, "k_fill", k_fill_adj, grid_for(cnt[s - 1]), 0, cnt[s - 1], xb[0], xb[1], xb[2], vb[0], vb[1], vb[2]);
       LAUNCH(h, "k_adj_unmap", k_adj_unmap<D>, grid_for(cnt[s]), 0, m, h->pshift, (int)cnt[s], srcs[s], ecell[s],
              xa[0], xa[1], xa[2], va[0], va[1], va[2], xb[0], xb[1], xb[2], vb[0], vb[1], vb[2], gcv, h->d_err);
-      LAUNCH(h, "k_adj_move", k_adj_move<D>, grid_for(cnt[s - 1]), 0, m, h->Vc, (int)cnt[s - 1], max_sub,
-             paths[s - s0], xb[0], xb[1], xb[2], gcv, h->d_err);
+      // (the path held for five substeps when that covers every instance,
+      // the workloads' count; else for kAdjMaxSub)
+      if (max_sub <= 5)
+        LAUNCH(h, "k_adj_move", (k_adj_move<D, 5>), grid_for(cnt[s - 1]), 0, m, h->Vc, (int)cnt[s - 1], max_sub,
+               paths[s - s0], xb[0], xb[1], xb[2], gcv, h->d_err);
+      else
+        LAUNCH(h, "k_adj_move", (k_adj_move<D, kAdjMaxSub>), grid_for(cnt[s - 1]), 0, m, h->Vc, (int)cnt[s - 1],
+               max_sub, paths[s - s0], xb[0], xb[1], xb[2], gcv, h->d_err);
       for (int k = 0; k < D; ++k) {
         std::swap(xa[k], xb[k]);
         std::swap(va[k], vb[k]);

@@ -414,9 +414,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_unmap(DMesh m, int pshift, int n
 // the move deleted has no adjoint (it is no survivor), so lanes with a zero
 // adjoint skip their substeps (a zero adjoint contributes nothing). The
 // aggregated scatters (at a cell change and at the end) are reached by every
-// lane of the wavefront.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __restrict__ Vc, int n_start, int max_sub,
+// lane of the wavefront. MS bounds the substep count (max_sub <= MS): the
+// path is held in registers for MS substeps.
+template <int D, int MS>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_adj_move(DMesh m, const double* __restrict__ Vc, int n_start, int max_sub,
                                                     DPath path, double* xt0, double* xt1, double* xt2,
                                                     unsigned long long* __restrict__ gcv, int* __restrict__ err) {
   constexpr int NV = 1 << D;
@@ -434,20 +435,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __re
       live = live || a[k] != 0.0;
     }
     // the whole path in one round of loads, issued together with the
-    // adjoint's (a lane that turns out dead ignores what it loaded)
-    int pc[kAdjMaxSub];
-    double pxi[kAdjMaxSub][D];
+    // adjoint's (a lane that turns out dead ignores what it loaded; a lane
+    // past the end or a substep past max_sub reads entry (0, 0), in bounds, so
+    // that every load is issued unconditionally)
+    int pc[MS];
+    double pxi[MS][D];
 #pragma unroll
-    for (int s = 0; s < kAdjMaxSub; ++s) {
-      pc[s] = -1;
+    for (int s = 0; s < MS; ++s) {
+      const bool in = p < n_start && s < max_sub;
+      const long o = in ? s * path.stride + p : 0;
+      pc[s] = path.c[o];
 #pragma unroll
-      for (int k = 0; k < D; ++k) pxi[s][k] = 0.0;
-      if (p < n_start && s < max_sub) {
-        const long o = s * path.stride + p;
-        pc[s] = path.c[o];
-#pragma unroll
-        for (int k = 0; k < D; ++k) pxi[s][k] = path.xi[k][o];
-      }
+      for (int k = 0; k < D; ++k) pxi[s][k] = path.xi[k][o];
+      if (!in) pc[s] = -1;
     }
     // substep count and length: one instance's (a uniform load) or the
     // instance of the particle's first cell
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_move(DMesh m, const double* __re
 #pragma unroll
       for (int k = 0; k < 3; ++k) acc[i][k] = 0.0;
 #pragma unroll
-    for (int s = kAdjMaxSub - 1; s >= 0; --s) {
+    for (int s = MS - 1; s >= 0; --s) {
       if (s >= max_sub) continue;
       const bool act = live && s < n_sub;
       // a lane whose cell changes here hands its sums for the previous cell

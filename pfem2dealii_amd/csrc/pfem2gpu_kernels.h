@@ -667,19 +667,32 @@ __device__ __forceinline__ unsigned lb_prefix(unsigned long long* lb, int ch, in
   return prefix;
 }
 
+// Inclusive scan of one 32-bit value per lane over the wavefront with DPP
+// moves (row shifts inside each row of 16 lanes, then the row broadcasts of
+// lanes 15 and 31), no LDS traffic.
+__device__ __forceinline__ unsigned wave_scan_u32(unsigned v) {
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
 // Inclusive-to-exclusive scan of one 64-bit value per thread over the
-// workgroup; `part` holds one value per wavefront. Returns the exclusive
-// prefix and sets *total.
+// workgroup, as two 32-bit halves (the callers' halves are counts that stay
+// far below 2^32, so no carry crosses them); `part` holds one value per
+// wavefront. Returns the exclusive prefix and sets *total.
 __device__ __forceinline__ unsigned long long block_scan_u64(unsigned long long v, unsigned long long* part,
                                                              unsigned long long* total) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  unsigned long long incl = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const unsigned long long t = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += t;
-  }
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));  // (rebuilt here rather than held: see cells_load)
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const unsigned lo = wave_scan_u32((unsigned)(v & 0xffffffffULL));
+  const unsigned hi = wave_scan_u32((unsigned)(v >> 32));
+  const unsigned long long incl = ((unsigned long long)hi << 32) | lo;
   if (lane == 63) part[wave] = incl;
   __syncthreads();
   unsigned long long wbase = 0, tot = 0;
@@ -838,7 +851,8 @@ template <int D>
 __device__ __forceinline__ void cells_store(const CellRegs<D>& r, bool single, int ncell, double* geo, double* vc,
                                             Prm* prm, int* bpos) {
   constexpr int NV = 1 << D;
-  const int t = threadIdx.x;
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));  // (rebuilt here rather than held: see cells_load)
   if (t < ncell * 3 * D) geo[t] = r.geo;
 #pragma unroll
   for (int q = 0; q < 2; ++q)
@@ -1076,19 +1090,25 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     __syncthreads();
     // B. final (survivors + refill) and raw counts per (cell, part), scanned
     //    over the chunk in (cell, part) order; the chunk total is published
+    // (a thread's per_thread entries lie in one cell: per_thread divides
+    // pmax)
     unsigned long long mine = 0;
-    for (int e = 0; e < per_thread; ++e) {
-      const int t = threadIdx.x * per_thread + e;
-      if (t >= ncell * pmax) break;
-      const int lc = t >> pshift;
-      const int k = s_cnt[t];
-      int pi[D];
-      const Prm pr = prm_regs(&s_prm[lc]);
-      const bool valid = ((t & (pmax - 1)) < pr.P) && part_code<D>(pr, t & (pmax - 1), pi);
-      const int kept = apply_cap ? min(k, pr.maxpp + 1) : k;
-      const int refill = (do_refill && s_ccnt[lc] > 0 && k == 0 && valid) ? 1 : 0;
-      s_base[t] = kept + refill;
-      mine += ((unsigned long long)(kept + refill) << 32) | (unsigned)k;
+    {
+      const int t0 = threadIdx.x * per_thread;
+      const int lc = t0 >> pshift;
+      const Prm pr = prm_regs(&s_prm[lc < cpw ? lc : 0]);
+      const int ccnt = s_ccnt[lc < cpw ? lc : 0];
+      for (int e = 0; e < per_thread; ++e) {
+        const int t = t0 + e;
+        if (t >= ncell * pmax) break;
+        const int k = s_cnt[t];
+        int pi[D];
+        const bool valid = ((t & (pmax - 1)) < pr.P) && part_code<D>(pr, t & (pmax - 1), pi);
+        const int kept = apply_cap ? min(k, pr.maxpp + 1) : k;
+        const int refill = (do_refill && ccnt > 0 && k == 0 && valid) ? 1 : 0;
+        s_base[t] = kept + refill;
+        mine += ((unsigned long long)(kept + refill) << 32) | (unsigned)k;
+      }
     }
     unsigned long long tot = 0;
     unsigned long long run = block_scan_u64(mine, s_part, &tot);
@@ -1096,10 +1116,11 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       const int t = threadIdx.x * per_thread + e;
       if (t >= ncell * pmax) break;
       const int fin = s_base[t];
+      const int k = s_cnt[t];
       s_base[t] = (int)(run >> 32);
       s_roff[t] = (int)(run & 0xffffffffULL);
       if ((t & (pmax - 1)) == 0) s_cfin[t >> pshift] = (int)(run >> 32);
-      run += ((unsigned long long)fin << 32) | (unsigned)s_cnt[t];
+      run += ((unsigned long long)fin << 32) | (unsigned)k;
     }
     const int nout = (int)(tot >> 32);
     const bool staged = staged_in && nout <= nslots;

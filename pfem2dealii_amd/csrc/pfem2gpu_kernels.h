@@ -667,6 +667,27 @@ __device__ __forceinline__ unsigned lb_prefix(unsigned long long* lb, int ch, in
   return prefix;
 }
 
+// The value of part j of the lane's group of QP consecutive lanes (QP = 2 or
+// 4, groups aligned inside quads of lanes), by a DPP quad permute of each
+// 32-bit half.
+template <int QP>
+__device__ __forceinline__ double quad_from(double v, int j) {
+  static_assert(QP == 2 || QP == 4, "parts of a sum within a quad");
+  const long long b = __double_as_longlong(v);
+  int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+  // quad_perm controls: lane j of each quad to all four (QP = 4), or lane j
+  // of each pair to both (QP = 2)
+  switch (QP == 4 ? j : 4 + j) {
+    case 0: lo = __builtin_amdgcn_mov_dpp(lo, 0x00, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0x00, 0xf, 0xf, false); break;
+    case 1: lo = __builtin_amdgcn_mov_dpp(lo, 0x55, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0x55, 0xf, 0xf, false); break;
+    case 2: lo = __builtin_amdgcn_mov_dpp(lo, 0xaa, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0xaa, 0xf, 0xf, false); break;
+    case 3: lo = __builtin_amdgcn_mov_dpp(lo, 0xff, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0xff, 0xf, 0xf, false); break;
+    case 4: lo = __builtin_amdgcn_mov_dpp(lo, 0xa0, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0xa0, 0xf, 0xf, false); break;
+    default: lo = __builtin_amdgcn_mov_dpp(lo, 0xf5, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0xf5, 0xf, 0xf, false); break;
+  }
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Inclusive scan of one 32-bit value per lane over the wavefront with DPP
 // moves (row shifts inside each row of 16 lanes, then the row broadcasts of
 // lanes 15 and 31), no LDS traffic.
@@ -1470,14 +1491,15 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
           }
         }
       }
-      // the parts of this sum, read from their lanes together, added in order
-      const int lane0 = ((int)threadIdx.x & 63) - part;  // the lane of part 0
+      // the parts of this sum, read from their lanes (the QP lanes of a sum
+      // are aligned inside a quad of lanes) with DPP quad permutes, added in
+      // order
       double PS[QP][D + 1];
 #pragma unroll
       for (int j = 0; j < QP; ++j) {
 #pragma unroll
-        for (int k = 0; k < D; ++k) PS[j][k] = __shfl(S[k], lane0 + j, 64);
-        PS[j][D] = __shfl(W, lane0 + j, 64);
+        for (int k = 0; k < D; ++k) PS[j][k] = quad_from<QP>(S[k], j);
+        PS[j][D] = quad_from<QP>(W, j);
       }
       double T[D], TW = PS[0][D];
 #pragma unroll

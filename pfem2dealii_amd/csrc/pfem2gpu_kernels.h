@@ -165,7 +165,7 @@ __device__ __forceinline__ void wave_groups(int key, int& leader_of, int& rank, 
   unsigned long long pending = __ballot(key >= 0);
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
-    const int lk = __shfl(key, leader, 64);
+    const int lk = __builtin_amdgcn_readlane(key, leader);  // (a wave-uniform lane: a scalar read)
     const unsigned long long grp = __ballot(key == lk) & pending;
     if ((grp >> lane) & 1ULL) {
       leader_of = leader;
@@ -254,7 +254,7 @@ __device__ __forceinline__ void move_emit(int key, bool walk, int cell, int tfla
   if (capc >= 0 && key >= 0 && leader_of == lane) base = atomicAdd(&bcount[key], cnt);
   const int slot = __shfl(base, leader_of, 64) + rank;
   if (wm) {
-    qb = __shfl(qb, qleader, 64);
+    qb = __builtin_amdgcn_readlane(qb, qleader);
     if (qb + __popcll(wm) > qreg) {  // the stripe is full: report, never write past it
       if (lane == qleader) atomicOr(Q.err, kErrQueue);
     } else if (walk) {

@@ -939,6 +939,9 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
   unsigned long long* s_part = reinterpret_cast<unsigned long long*>(s_misc + 16);  // [4] scan partials
 
   const int nchunks = (nc + cpw - 1) >> cshift;
+  // with one instance every cell's parameter words are the same: read those
+  // of cell 0, one LDS address for the whole wavefront (a broadcast)
+  const int psel = m.cell_inst == nullptr ? 0 : ~0;
   const int per_thread = ((pmax << cshift) + blockDim.x - 1) / blockDim.x;
 
   // registers: the particles of this thread in the current chunk
@@ -1065,7 +1068,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       ppl[u] = 0;
       if (j < n) {
         const int lc = cell_of(pc[u]) - c0;
-        const Prm pr = prm_regs(&s_prm[lc]);
+        const Prm pr = prm_regs(&s_prm[lc & psel]);
         double xi[D];
         settle_xi<D>(m, pr, cell_of(pc[u]), placed_of(pc[u]), s_geo + lc * 3 * D, px[u], xi);
         const int part = part_lin<D>(pr, xi, nullptr);
@@ -1093,7 +1096,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       const int cw = T.cell[i];
       const int c = cell_of(cw);
       const int lc = c - c0;
-      const Prm pr = prm_regs(&s_prm[lc]);
+      const Prm pr = prm_regs(&s_prm[lc & psel]);
       double x[D], xi[D];
 #pragma unroll
       for (int k = 0; k < D; ++k) x[k] = T.x[k][i];
@@ -1117,7 +1120,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
     {
       const int t0 = threadIdx.x * per_thread;
       const int lc = t0 >> pshift;
-      const Prm pr = prm_regs(&s_prm[lc < cpw ? lc : 0]);
+      const Prm pr = prm_regs(&s_prm[(lc < cpw ? lc : 0) & psel]);
       const int ccnt = s_ccnt[lc < cpw ? lc : 0];
       for (int e = 0; e < per_thread; ++e) {
         const int t = t0 + e;
@@ -1183,8 +1186,8 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
         double x[D], xi[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) x[k] = T.x[k][i];
-        settle_xi<D>(m, s_prm[lc], c, placed_of(cw), s_geo + lc * 3 * D, x, xi);
-        const int t = (lc << pshift) + part_lin<D>(s_prm[lc], xi, nullptr);
+        settle_xi<D>(m, s_prm[lc & psel], c, placed_of(cw), s_geo + lc * 3 * D, x, xi);
+        const int t = (lc << pshift) + part_lin<D>(s_prm[lc & psel], xi, nullptr);
         gord[chunk_index(atomicAdd(&s_roff[t], 1), nb, bucket0, spill0)] = j;
       }
     }
@@ -1201,7 +1204,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
           const int t = (lc << pshift) + (ppl[u] & 0x7fff);
           const unsigned long long key = s_key[j];
           const int b = s_roff[t], a = b - s_cnt[t];
-          const int maxpp = s_prm[lc].maxpp;
+          const int maxpp = s_prm[lc & psel].maxpp;
           // rank = keys below this one in the group, four keys read per
           // round; equal keys (the particle's own and any other) counted too
           int r = 0, eq = 0;
@@ -1254,8 +1257,8 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
           xj[k] = T.x[k][i];
           vj[k] = T.v[k][i];
         }
-        settle_xi<D>(m, s_prm[lc], c, placed_of(cw), s_geo + lc * 3 * D, xj, xi);
-        const int t = (lc << pshift) + part_lin<D>(s_prm[lc], xi, nullptr);
+        settle_xi<D>(m, s_prm[lc & psel], c, placed_of(cw), s_geo + lc * 3 * D, xj, xi);
+        const int t = (lc << pshift) + part_lin<D>(s_prm[lc & psel], xi, nullptr);
         const unsigned long long key = particle_key<D>(xj);
         const int b = s_roff[t], a = b - s_cnt[t];
         int r = 0;
@@ -1272,7 +1275,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
           const unsigned long long kq = particle_key<D>(xq);
           if (kq < key || (kq == key && tie_less<D>(xq, vq, xj, vj, o, j))) ++r;
         }
-        if (apply_cap && r > s_prm[lc].maxpp) continue;
+        if (apply_cap && r > s_prm[lc & psel].maxpp) continue;
         const int fpos = o0n + s_base[t] + r;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
@@ -1333,7 +1336,7 @@ __global__ __launch_bounds__(kSettleBlock, D == 2 ? 3 : 2) void k_settle(int app
       for (int t = threadIdx.x; t < ncell * pmax; t += blockDim.x) {
         const int lc = t >> pshift, pp = t & (pmax - 1);
         if (s_cnt[t] != 0 || s_ccnt[lc] == 0) continue;
-        const Prm pr = prm_regs(&s_prm[lc]);
+        const Prm pr = prm_regs(&s_prm[lc & psel]);
         int pi[D];
         if (pp >= pr.P || !part_code<D>(pr, pp, pi)) continue;  // padding or no refillable part
         const int c = c0 + lc;
